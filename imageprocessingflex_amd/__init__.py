@@ -1,0 +1,254 @@
+"""ipf's filtering path on MI355X (gfx950): Python mirror of the ipf filter API.
+
+Images are numpy arrays of shape (H, W) or (H, W, C) (the layout of
+ipf::Image<T>, core.hpp:220-222) or DeviceImage objects that stay resident
+in HBM. Functions keep ipf's names, argument meaning and defaults
+(include/ipf/filter.hpp:157-387) and raise IpfError with ipf's messages.
+All compute runs in libipfg.so; nothing here computes pixels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from enum import IntEnum
+
+import numpy as np
+
+from . import _lib
+from ._lib import IpfError, DeviceError  # noqa: F401
+
+__all__ = [
+    "BorderMode", "Precision", "DeviceImage", "Stream", "IpfError", "DeviceError",
+    "gaussian_kernel_1d", "sep_filter2d", "gaussian_blur", "box_filter", "blur", "filter2d",
+    "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
+]
+
+
+class BorderMode(IntEnum):
+    """ipf::BorderMode (core.hpp:183)."""
+    CONSTANT = _lib.CONSTANT
+    REPLICATE = _lib.REPLICATE
+    REFLECT = _lib.REFLECT
+    REFLECT101 = _lib.REFLECT101
+    WRAP = _lib.WRAP
+
+
+class Precision(IntEnum):
+    DEFAULT = _lib.PREC_DEFAULT
+    EXACT = _lib.PREC_EXACT
+    FP32 = _lib.PREC_FP32
+    BF16X3 = _lib.PREC_BF16X3
+    FP16X2 = _lib.PREC_FP16X2
+
+
+_NP2DT = {
+    np.dtype(np.uint8): _lib.U8,
+    np.dtype(np.uint16): _lib.U16,
+    np.dtype(np.int16): _lib.I16,
+    np.dtype(np.int32): _lib.I32,
+    np.dtype(np.float32): _lib.F32,
+    np.dtype(np.float64): _lib.F64,
+}
+_DT2NP = {v: k for k, v in _NP2DT.items()}
+
+
+def init(device: int = 0) -> None:
+    _lib.check(_lib.load().ipfg_init(device))
+
+
+def synchronize() -> None:
+    _lib.check(_lib.load().ipfg_device_synchronize())
+
+
+class Stream:
+    """A HIP stream owned by the library (default priority)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _lib.check(_lib.load().ipfg_stream_create(C.byref(h)))
+        self.handle = h
+
+    def synchronize(self) -> None:
+        _lib.check(_lib.load().ipfg_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().ipfg_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class DeviceImage:
+    """An image resident in device memory (HWC interleaved, packed rows)."""
+
+    def __init__(self, height: int, width: int, channels: int = 1, dtype=np.uint8):
+        self.height, self.width, self.channels = int(height), int(width), int(channels)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.height * self.width * self.channels * self.dtype.itemsize
+        p = C.c_void_p()
+        _lib.check(_lib.load().ipfg_malloc(C.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, stream: Stream | None = None) -> "DeviceImage":
+        a = np.ascontiguousarray(a)
+        h, w, c = _shape(a)
+        d = cls(h, w, c, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    @classmethod
+    def like(cls, other: "DeviceImage", dtype=None) -> "DeviceImage":
+        return cls(other.height, other.width, other.channels, dtype or other.dtype)
+
+    def upload(self, a: np.ndarray, stream: Stream | None = None) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _lib.check(_lib.load().ipfg_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes, _sh(stream)))
+
+    def download(self, stream: Stream | None = None) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _lib.check(_lib.load().ipfg_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, _sh(stream)))
+        return out
+
+    @property
+    def shape(self):
+        return (self.height, self.width) if self.channels == 1 else (self.height, self.width, self.channels)
+
+    def descriptor(self) -> _lib.IpfgImage:
+        return _lib.IpfgImage(self.ptr, self.width, self.height, self.channels, _NP2DT[self.dtype], 0, 1)
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.load().ipfg_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _sh(stream):
+    return stream.handle if stream is not None else None
+
+
+def _shape(a: np.ndarray):
+    if a.ndim == 2:
+        return a.shape[0], a.shape[1], 1
+    if a.ndim == 3:
+        return a.shape
+    raise IpfError("ipf: image arrays must have shape (H, W) or (H, W, C)")
+
+
+def _desc(a: np.ndarray) -> _lib.IpfgImage:
+    if a.dtype not in _NP2DT:
+        raise IpfError(f"ipfg: unsupported pixel type {a.dtype}")
+    h, w, c = _shape(a)
+    return _lib.IpfgImage(a.ctypes.data, w, h, c, _NP2DT[a.dtype], a.strides[0], 0)
+
+
+def _io(src, out_dtype):
+    """Descriptors for src and a new destination of the same shape (out_dtype or src's)."""
+    if isinstance(src, DeviceImage):
+        dst = DeviceImage.like(src, out_dtype)
+        return src.descriptor(), dst.descriptor(), dst
+    src = np.asarray(src)
+    if not src.flags.c_contiguous:
+        src = np.ascontiguousarray(src)
+    dst = np.empty(src.shape, np.dtype(out_dtype) if out_dtype is not None else src.dtype)
+    return _desc(src), _desc(dst), (dst, src)
+
+
+def _ret(holder):
+    return holder if isinstance(holder, DeviceImage) else holder[0]
+
+
+def last_fixups() -> int:
+    """Samples recomputed in FP64 by this thread's last certified call."""
+    n = C.c_longlong()
+    _lib.check(_lib.load().ipfg_last_fixups(C.byref(n)))
+    return n.value
+
+
+def gaussian_kernel_1d(sigma: float, ksize: int = 0) -> np.ndarray:
+    """gaussianKernel1D (filter.hpp:129-145)."""
+    lib = _lib.load()
+    n = lib.ipfg_gaussian_kernel_1d(float(sigma), int(ksize), None, 0)
+    if n <= 0:
+        _lib.check(-n)
+    out = np.empty(n, np.float64)
+    lib.ipfg_gaussian_kernel_1d(float(sigma), int(ksize), out.ctypes.data_as(C.POINTER(C.c_double)), n)
+    return out
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def sep_filter2d(src, kx, ky, border=BorderMode.REFLECT101, delta=0.0, border_value=0.0, out_dtype=None,
+                 precision=Precision.DEFAULT, anchor=None, stream: Stream | None = None):
+    """sepFilter2D (filter.hpp:196-203); anchor=(ax, ay) reaches detail::sepFilterImpl (34-90)."""
+    kx = np.ascontiguousarray(kx, np.float64)
+    ky = np.ascontiguousarray(ky, np.float64)
+    ax, ay = anchor if anchor is not None else (len(kx) // 2, len(ky) // 2)
+    s, d, holder = _io(src, out_dtype)
+    _lib.check(_lib.load().ipfg_sep_filter(C.byref(s), C.byref(d), _dptr(kx), len(kx), _dptr(ky), len(ky), ax, ay,
+                                           int(border), float(delta), float(border_value), int(precision),
+                                           _sh(stream)))
+    return _ret(holder)
+
+
+def gaussian_blur(src, sigma_x, sigma_y=0.0, ksize=0, border=BorderMode.REFLECT101, out_dtype=None,
+                  precision=Precision.DEFAULT, stream: Stream | None = None):
+    """gaussianBlur (filter.hpp:261-267)."""
+    s, d, holder = _io(src, out_dtype)
+    _lib.check(_lib.load().ipfg_gaussian_blur(C.byref(s), C.byref(d), float(sigma_x), float(sigma_y), int(ksize),
+                                              int(border), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def box_filter(src, kw, kh, normalize=True, border=BorderMode.REFLECT101, border_value=0.0, out_dtype=None,
+               precision=Precision.DEFAULT, stream: Stream | None = None):
+    """boxFilter (filter.hpp:205-253)."""
+    s, d, holder = _io(src, out_dtype)
+    _lib.check(_lib.load().ipfg_box_filter(C.byref(s), C.byref(d), int(kw), int(kh), int(bool(normalize)),
+                                           int(border), float(border_value), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def blur(src, ksize, border=BorderMode.REFLECT101, out_dtype=None, precision=Precision.DEFAULT,
+         stream: Stream | None = None):
+    """blur (filter.hpp:255-259)."""
+    return box_filter(src, ksize, ksize, True, border, 0.0, out_dtype, precision, stream)
+
+
+def filter2d(src, kernel, border=BorderMode.REFLECT101, delta=0.0, anchor=(-1, -1), border_value=0.0,
+             out_dtype=None, precision=Precision.DEFAULT, stream: Stream | None = None):
+    """filter2D (filter.hpp:157-194); kernel is a 2-D float array (kh, kw)."""
+    k = np.ascontiguousarray(kernel, np.float32)
+    if k.ndim != 2:
+        raise IpfError("ipf: filter2D: kernel must be a non-empty single-channel image")
+    kh, kw = k.shape
+    s, d, holder = _io(src, out_dtype)
+    _lib.check(_lib.load().ipfg_filter2d(C.byref(s), C.byref(d), k.ctypes.data_as(C.POINTER(C.c_float)), kw, kh,
+                                         int(anchor[0]), int(anchor[1]), int(border), float(delta),
+                                         float(border_value), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def median_blur(src, ksize, border=BorderMode.REPLICATE, stream: Stream | None = None):
+    """medianBlur (filter.hpp:300-332)."""
+    s, d, holder = _io(src, None)
+    _lib.check(_lib.load().ipfg_median(C.byref(s), C.byref(d), int(ksize), int(border), _sh(stream)))
+    return _ret(holder)
+
+
+def bilateral_filter(src, d, sigma_color, sigma_space, border=BorderMode.REFLECT101, precision=Precision.DEFAULT,
+                     stream: Stream | None = None):
+    """bilateralFilter (filter.hpp:334-387)."""
+    s, dd, holder = _io(src, None)
+    _lib.check(_lib.load().ipfg_bilateral(C.byref(s), C.byref(dd), int(d), float(sigma_color), float(sigma_space),
+                                          int(border), int(precision), _sh(stream)))
+    return _ret(holder)

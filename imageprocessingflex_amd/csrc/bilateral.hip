@@ -1,0 +1,211 @@
+// bilateralFilter for gfx950.
+//
+// Reference: include/ipf/filter.hpp:334-387. Circular taps i^2 + j^2 <= r^2
+// (j outer, i inner) with FP64 space weights; the range weight of the L1
+// colour distance comes from an FP64 table for integer images; Constant
+// border skips taps outside the image; out = saturate_cast(acc / wsum).
+//
+// Fast path (uint8, 1..4 channels): pixels are packed into one 32-bit word in
+// an LDS tile with an r-pixel halo, so the L1 distance over all channels is a
+// single v_sad_u8; the range table (FP32, <= 1021 entries) sits in LDS and
+// the space weights are wave-uniform scalars. One thread per output pixel.
+// Every channel's FP32 result is checked against its error bound; doubtful
+// pixels are recomputed in FP64 in the reference's order.
+//
+// Exact path: the reference loop in FP64 per pixel, for every pixel type.
+#include "kernels.hpp"
+
+namespace ipfg {
+namespace {
+
+constexpr int TX = 64;
+
+__host__ __device__ inline int bil_xp(int r) { return (TX + 2 * r) | 1; }
+
+__host__ inline size_t bil_lds_bytes(int TH, int r, int lut_len) {
+    return sizeof(uint32_t) * static_cast<size_t>(TH + 2 * r) * bil_xp(r) + sizeof(float) * lut_len;
+}
+
+template <int CN, bool CONST_BORDER>
+__global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(const uint8_t* __restrict__ src, size_t sstride, int W,
+                                                              int H, uint8_t* __restrict__ dst, size_t dstride,
+                                                              const int* __restrict__ taps,
+                                                              const float* __restrict__ spacef, int ntaps,
+                                                              const float* __restrict__ lutg, int lut_len, int r,
+                                                              int border, int TH, int certify, double bound,
+                                                              FixList fix) {
+    extern __shared__ uint32_t bsm[];
+    const int XP = bil_xp(r);
+    const int HR = TH + 2 * r;
+    uint32_t* tile = bsm;
+    float* lut = reinterpret_cast<float*>(bsm + HR * XP);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TH;
+    for (int i = tid; i < lut_len; i += kBlock) lut[i] = lutg[i];
+    const int XW = TX + 2 * r;
+    for (int rr = wave; rr < HR; rr += kBlock / 64) {
+        const int gy = y0 - r + rr;
+        const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, border);
+        uint32_t* trow = tile + rr * XP;
+        const uint8_t* srow = src + static_cast<size_t>(sy < 0 ? 0 : sy) * sstride;
+        for (int px = lane; px < XW; px += 64) {
+            const int gx = x0 - r + px;
+            const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, border);
+            uint32_t v = 0;
+            if (sy >= 0 && sx >= 0) {
+                const uint8_t* p = srow + static_cast<size_t>(sx) * CN;
+#pragma unroll
+                for (int c = 0; c < CN; ++c) v |= static_cast<uint32_t>(p[c]) << (8 * c);
+            }
+            trow[px] = v;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < TX * TH; t += kBlock) {
+        const int lx = t % TX, ly = t / TX;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x >= W || y >= H) continue;
+        const uint32_t* cen = tile + (ly + r) * XP + (lx + r);
+        const uint32_t p0 = *cen;
+        float wsum = 0.f;
+        float acc[CN];
+#pragma unroll
+        for (int c = 0; c < CN; ++c) acc[c] = 0.f;
+        for (int k = 0; k < ntaps; ++k) {
+            const int dx = taps[2 * k], dy = taps[2 * k + 1];
+            if (CONST_BORDER) {
+                if (static_cast<unsigned>(x + dx) >= static_cast<unsigned>(W) ||
+                    static_cast<unsigned>(y + dy) >= static_cast<unsigned>(H))
+                    continue;
+            }
+            const uint32_t p = cen[dy * XP + dx];
+            const uint32_t dist = __builtin_amdgcn_sad_u8(p, p0, 0u);
+            const float w = spacef[k] * lut[dist];
+            wsum += w;
+#pragma unroll
+            for (int c = 0; c < CN; ++c) acc[c] = fmaf(w, static_cast<float>((p >> (8 * c)) & 255u), acc[c]);
+        }
+        uint8_t* o = dst + static_cast<size_t>(y) * dstride + static_cast<size_t>(x) * CN;
+        bool doubt = false;
+#pragma unroll
+        for (int c = 0; c < CN; ++c) {
+            const float v = acc[c] / wsum;
+            o[c] = saturate_from_double<uint8_t>(v);
+            doubt |= needs_exact(v, bound, 0.0, 255.0);
+        }
+        if (certify && doubt) fix_push(fix, static_cast<unsigned long long>(y) * W + x);
+    }
+}
+
+// One pixel, one channel, in the reference's FP64 order. The range weight of
+// an integer image comes from the table while the distance is inside it.
+__device__ inline double bil_exact_channel(const DevImage& src, int x, int y, int c, const int* taps,
+                                           const double* spaced, int ntaps, const double* lutd, int lut_len, int r,
+                                           int border, double cc) {
+    const size_t es = dtype_size(src.dt);
+    const int cn = src.cn;
+    const char* p0 = static_cast<const char*>(src.data) + static_cast<size_t>(y) * src.stride + static_cast<size_t>(x) * cn * es;
+    double wsum = 0.0, acc = 0.0;
+    for (int k = 0; k < ntaps; ++k) {
+        const int sx = border_index(x + taps[2 * k], src.w, border);
+        const int sy = border_index(y + taps[2 * k + 1], src.h, border);
+        if (sx < 0 || sy < 0) continue;
+        const char* p = static_cast<const char*>(src.data) + static_cast<size_t>(sy) * src.stride + static_cast<size_t>(sx) * cn * es;
+        double dist = 0.0;
+        for (int ch = 0; ch < cn; ++ch)
+            dist = dist + fabs(load_sample(p, ch * es, src.dt) - load_sample(p0, ch * es, src.dt));
+        double rw;
+        if (lutd != nullptr && dist < static_cast<double>(lut_len))
+            rw = lutd[static_cast<size_t>(dist)];
+        else
+            rw = exp(dist * dist * cc);
+        const double w = spaced[k] * rw;
+        wsum = wsum + w;
+        acc = acc + w * load_sample(p, c * es, src.dt);
+    }
+    (void)r;
+    return acc / wsum;
+}
+
+__global__ __launch_bounds__(kBlock) void bilateral_exact_kernel(DevImage src, DevImage dst,
+                                                                 const int* __restrict__ taps,
+                                                                 const double* __restrict__ spaced, int ntaps,
+                                                                 const double* __restrict__ lutd, int lut_len, int r,
+                                                                 int border, double cc, FixList fix, int from_list) {
+    const size_t es = dtype_size(dst.dt);
+    unsigned long long N = static_cast<unsigned long long>(dst.w) * dst.h;
+    bool all = true;
+    if (from_list) {
+        const unsigned long long n = *fix.count;
+        all = n > fix.capacity;
+        if (!all) N = n;
+    }
+    for (unsigned long long k = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; k < N;
+         k += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+        const unsigned long long pix = all ? k : fix.index[k];
+        const int y = static_cast<int>(pix / dst.w);
+        const int x = static_cast<int>(pix - static_cast<unsigned long long>(y) * dst.w);
+        for (int c = 0; c < dst.cn; ++c) {
+            const double v = bil_exact_channel(src, x, y, c, taps, spaced, ntaps, lutd, lut_len, r, border, cc);
+            store_sample(dst.data, static_cast<size_t>(y) * dst.stride + (static_cast<size_t>(x) * dst.cn + c) * es,
+                         dst.dt, v);
+        }
+    }
+}
+
+int grid_for(size_t n) {
+    const size_t b = (n + kBlock - 1) / kBlock;
+    return static_cast<int>(b < 16384 ? (b ? b : 1) : 16384);
+}
+
+template <int CN, bool CB>
+hipError_t bil_fast_launch(const BilateralLaunch& a, int TH, size_t lds, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>((a.src.w + TX - 1) / TX), static_cast<unsigned>((a.src.h + TH - 1) / TH));
+    hipLaunchKernelGGL((bilateral_u8_kernel<CN, CB>), grid, dim3(kBlock), lds, s,
+                       static_cast<const uint8_t*>(a.src.data), a.src.stride, a.src.w, a.src.h,
+                       static_cast<uint8_t*>(a.dst.data), a.dst.stride, a.taps, a.spacef, a.ntaps, a.lutf, a.lut_len,
+                       a.r, a.border, TH, a.mode == MODE_CERTIFY ? 1 : 0, a.bound, a.fix);
+    return hipGetLastError();
+}
+
+template <bool CB>
+hipError_t bil_fast_cn(const BilateralLaunch& a, int TH, size_t lds, hipStream_t s) {
+    switch (a.src.cn) {
+        case 1: return bil_fast_launch<1, CB>(a, TH, lds, s);
+        case 2: return bil_fast_launch<2, CB>(a, TH, lds, s);
+        case 3: return bil_fast_launch<3, CB>(a, TH, lds, s);
+        case 4: return bil_fast_launch<4, CB>(a, TH, lds, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+// True when the uint8 fast kernel can run this call (tile fits LDS).
+bool bilateral_fast_ok(int dt, int cn, int r, int lut_len) {
+    return dt == IPFG_U8 && cn >= 1 && cn <= 4 && lut_len > 0 && bil_lds_bytes(4, r, lut_len) <= 64 * 1024;
+}
+
+hipError_t launch_bilateral(const BilateralLaunch& a, hipStream_t s) {
+    const size_t npix = static_cast<size_t>(a.src.w) * a.src.h;
+    if (a.mode == MODE_EXACT) {
+        hipLaunchKernelGGL(bilateral_exact_kernel, dim3(grid_for(npix)), dim3(kBlock), 0, s, a.src, a.dst, a.taps,
+                           a.spaced, a.ntaps, a.lutd, a.lut_len, a.r, a.border, a.cc, a.fix, 0);
+        return hipGetLastError();
+    }
+    int TH = 32;
+    while (TH > 4 && bil_lds_bytes(TH, a.r, a.lut_len) > 64 * 1024) TH /= 2;
+    const size_t lds = bil_lds_bytes(TH, a.r, a.lut_len);
+    hipError_t e;
+    if (a.mode == MODE_CERTIFY) {
+        e = launch_zero_counter(a.fix.count, s);
+        if (e != hipSuccess) return e;
+    }
+    e = a.border == IPFG_BORDER_CONSTANT ? bil_fast_cn<true>(a, TH, lds, s) : bil_fast_cn<false>(a, TH, lds, s);
+    if (e != hipSuccess || a.mode != MODE_CERTIFY) return e;
+    hipLaunchKernelGGL(bilateral_exact_kernel, dim3(kFixBlocks), dim3(kBlock), 0, s, a.src, a.dst, a.taps, a.spaced,
+                       a.ntaps, a.lutd, a.lut_len, a.r, a.border, a.cc, a.fix, 1);
+    return hipGetLastError();
+}
+
+}  // namespace ipfg

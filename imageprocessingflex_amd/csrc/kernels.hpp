@@ -1,0 +1,84 @@
+// Launch interfaces between the C-ABI runtime (runtime.hip) and the kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace ipfg {
+
+enum Mode { MODE_FAST = 0, MODE_CERTIFY = 1, MODE_EXACT = 2 };
+
+// ---------------------------------------------------------------- separable
+struct SepLaunch {
+    DevImage src, dst;
+    const float* kxf;   // kw FP32 weights (device)
+    const float* kyf;   // kh FP32 weights (device)
+    const double* kxd;  // kw FP64 weights (device)
+    const double* kyd;  // kh FP64 weights (device)
+    int kw, kh, ax, ay, border;
+    double delta, bval;
+    int mode;
+    double bound;  // certification bound (MODE_CERTIFY)
+    FixList fix;
+    double* tmp;  // MODE_EXACT: (H + kh - 1) * W * cn doubles
+};
+hipError_t launch_sep(const SepLaunch& a, hipStream_t s);
+size_t sep_exact_tmp_bytes(int w, int h, int cn, int kh);
+
+// ---------------------------------------------------------------- box
+struct BoxLaunch {
+    DevImage src, dst;
+    int kw, kh, border;
+    int normalize;
+    double bval;
+    int mode;      // MODE_FAST (window sums) or MODE_EXACT (reference running sums)
+    double* tmp;   // MODE_EXACT: (H + kh - 1) * W * cn doubles
+};
+hipError_t launch_box(const BoxLaunch& a, hipStream_t s);
+
+// ---------------------------------------------------------------- filter2D
+struct F2dLaunch {
+    DevImage src, dst;
+    const float* kf;      // kw*kh FP32 weights, row-major (device)
+    const int* taps;      // non-zero taps: ntaps (dx, dy) pairs (device)
+    const double* tapw;   // non-zero tap weights as double (device)
+    int ntaps;
+    int kw, kh, ax, ay, border;
+    int has_zero;         // the kernel has zero weights (they must not take part)
+    double delta, bval;
+    int mode;
+    double bound;
+    FixList fix;
+    int precision;        // IPFG_PREC_* for the MFMA variants
+    const void* mfma_w;   // packed split weights for the MFMA path (device)
+};
+hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s);
+
+// ---------------------------------------------------------------- median
+struct MedianLaunch {
+    DevImage src, dst;
+    int k, border;
+};
+hipError_t launch_median(const MedianLaunch& a, hipStream_t s);
+
+// ---------------------------------------------------------------- bilateral
+struct BilateralLaunch {
+    DevImage src, dst;
+    int r, ntaps, border;
+    const int* taps;        // (dx, dy) pairs (device)
+    const double* spaced;   // FP64 space weights (device)
+    const float* spacef;    // FP32 space weights (device)
+    const double* lutd;     // FP64 range table (integer types) or null
+    const float* lutf;      // FP32 range table or null
+    int lut_len;
+    double cc;              // -1 / (2 sigma_color^2)
+    int mode;
+    double bound;
+    FixList fix;
+};
+hipError_t launch_bilateral(const BilateralLaunch& a, hipStream_t s);
+bool bilateral_fast_ok(int dt, int cn, int r, int lut_len);
+
+// Zeroes the fix-up counter with a kernel (usable inside graph capture).
+hipError_t launch_zero_counter(unsigned long long* count, hipStream_t s);
+
+}  // namespace ipfg

@@ -1,0 +1,744 @@
+// C-ABI entry points of libipfg.so (declared in include/ipfg.h): argument
+// checks with the reference's messages, host<->device staging, per-call
+// parameter upload, precision selection and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace ipfg {
+namespace {
+
+// ------------------------------------------------------------------ errors
+struct ArgError {
+    std::string msg;
+};
+struct DeviceError {
+    hipError_t err;
+    std::string where;
+};
+struct Unsupported {
+    std::string msg;
+};
+
+thread_local std::string t_error;
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw ArgError{std::string("ipf: ") + msg};
+}
+void abi_check(bool ok, const char* msg) {
+    if (!ok) throw ArgError{std::string("ipfg: ") + msg};
+}
+void hip_check(hipError_t e, const char* where) {
+    if (e != hipSuccess) throw DeviceError{e, where};
+}
+#define HIPCALL(x) hip_check((x), #x)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return IPFG_OK;
+    } catch (const ArgError& e) {
+        t_error = e.msg;
+        return IPFG_ERR_ARG;
+    } catch (const Unsupported& e) {
+        t_error = e.msg;
+        return IPFG_ERR_UNSUPPORTED;
+    } catch (const DeviceError& e) {
+        t_error = std::string("ipfg: HIP error '") + hipGetErrorString(e.err) + "' in " + e.where;
+        return e.err == hipErrorOutOfMemory ? IPFG_ERR_NOMEM : IPFG_ERR_DEVICE;
+    } catch (const std::bad_alloc&) {
+        t_error = "ipfg: host allocation failed";
+        return IPFG_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        t_error = std::string("ipfg: ") + e.what();
+        return IPFG_ERR_DEVICE;
+    }
+}
+
+// ------------------------------------------------------------------ state
+std::mutex g_init_mu;
+std::atomic<int> g_device{-1};
+std::atomic<int> g_threads{0};
+
+void init_device(int device) {
+    std::lock_guard<std::mutex> lock(g_init_mu);
+    if (g_device.load() == device) return;
+    int n = 0;
+    HIPCALL(hipGetDeviceCount(&n));
+    abi_check(device >= 0 && device < n, "device index out of range");
+    HIPCALL(hipSetDevice(device));
+    // Keep freed stream-ordered allocations cached: per-call staging and
+    // workspace come from this pool.
+    hipMemPool_t pool;
+    HIPCALL(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t threshold = UINT64_MAX;
+    HIPCALL(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold));
+    g_device = device;
+}
+
+void ensure_init() {
+    if (g_device.load() < 0) init_device(0);
+    int cur = -1;
+    HIPCALL(hipGetDevice(&cur));
+    if (cur != g_device.load()) HIPCALL(hipSetDevice(g_device.load()));
+}
+
+// Per-thread default stream (default priority) and a pinned slot receiving the
+// fix-up count of the last certified call.
+struct ThreadCtx {
+    hipStream_t stream = nullptr;
+    unsigned long long* fix_host = nullptr;  // pinned
+    bool fix_valid = false;
+};
+thread_local ThreadCtx t_ctx;
+
+hipStream_t pick_stream(ipfg_stream user) {
+    if (user) return static_cast<hipStream_t>(user);
+    if (!t_ctx.stream) HIPCALL(hipStreamCreateWithFlags(&t_ctx.stream, hipStreamNonBlocking));
+    return t_ctx.stream;
+}
+
+unsigned long long* fix_host_slot() {
+    if (!t_ctx.fix_host) {
+        void* p = nullptr;
+        HIPCALL(hipHostMalloc(&p, sizeof(unsigned long long), hipHostMallocDefault));
+        t_ctx.fix_host = static_cast<unsigned long long*>(p);
+        *t_ctx.fix_host = 0;
+    }
+    return t_ctx.fix_host;
+}
+
+// ------------------------------------------------------------------ pinned ring
+// Small per-call parameter blocks (kernel weights, tap lists, lookup tables)
+// are copied into a pinned slot and DMA'd to a stream-ordered device
+// allocation; a slot is reused once its copy event has completed.
+class PinnedRing {
+public:
+    static constexpr size_t kSlot = 256 << 10;
+    static constexpr int kSlots = 64;
+
+    // Copies `bytes` of `data` to `dev` on `s`. Returns false if the block
+    // does not fit a slot (the caller then copies from pageable memory).
+    bool copy(void* dev, const void* data, size_t bytes, hipStream_t s) {
+        if (bytes > kSlot) return false;
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!base_) {
+            void* p = nullptr;
+            HIPCALL(hipHostMalloc(&p, kSlot * kSlots, hipHostMallocDefault));
+            base_ = static_cast<char*>(p);
+            events_.resize(kSlots, nullptr);
+            for (auto& e : events_) HIPCALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            armed_.assign(kSlots, 0);
+        }
+        const int slot = next_;
+        next_ = (next_ + 1) % kSlots;
+        if (armed_[slot]) HIPCALL(hipEventSynchronize(events_[slot]));
+        char* h = base_ + static_cast<size_t>(slot) * kSlot;
+        std::memcpy(h, data, bytes);
+        HIPCALL(hipMemcpyAsync(dev, h, bytes, hipMemcpyHostToDevice, s));
+        HIPCALL(hipEventRecord(events_[slot], s));
+        armed_[slot] = 1;
+        return true;
+    }
+
+private:
+    std::mutex mu_;
+    char* base_ = nullptr;
+    std::vector<hipEvent_t> events_;
+    std::vector<char> armed_;
+    int next_ = 0;
+};
+PinnedRing g_ring;
+
+// ------------------------------------------------------------------ per call
+// Device memory for one call, allocated and freed in stream order.
+class CallScratch {
+public:
+    explicit CallScratch(hipStream_t s) : s_(s) {}
+    ~CallScratch() {
+        for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
+    }
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        HIPCALL(hipMallocAsync(&p, bytes ? bytes : 16, s_));
+        ptrs_.push_back(p);
+        return p;
+    }
+    void* upload(const void* data, size_t bytes) {
+        void* d = alloc(bytes);
+        if (bytes && !g_ring.copy(d, data, bytes, s_)) HIPCALL(hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, s_));
+        return d;
+    }
+    FixList fixlist(unsigned long long capacity) {
+        FixList f;
+        f.count = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long)));
+        f.capacity = capacity;
+        f.index = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long) * (capacity ? capacity : 1)));
+        return f;
+    }
+    hipStream_t stream() const { return s_; }
+
+private:
+    hipStream_t s_;
+    std::vector<void*> ptrs_;
+};
+
+size_t packed_stride(const ipfg_image& im) {
+    return static_cast<size_t>(im.width) * im.channels * dtype_size(im.dtype);
+}
+
+void check_image(const ipfg_image* im, const char* what) {
+    abi_check(im != nullptr, what);
+    abi_check(im->dtype >= IPFG_U8 && im->dtype <= IPFG_F64, "unsupported pixel type");
+    // Image::checkDims (core.hpp:381-388)
+    require(im->width >= 0 && im->height >= 0, "Image: negative dimensions");
+    require(im->channels >= 1, "Image: channel count must be >= 1");
+    const unsigned long long n = static_cast<unsigned long long>(im->width) * im->height * im->channels;
+    require(n < (1ULL << 40), "Image: dimensions too large");
+    abi_check(n == 0 || im->data != nullptr, "null image data");
+    abi_check(im->row_stride_bytes == 0 || im->row_stride_bytes >= packed_stride(*im), "row stride too small");
+}
+
+// A call's source and destination as device images plus the copy-back step.
+class Staged {
+public:
+    Staged(const ipfg_image* src, const ipfg_image* dst, CallScratch& sc) : src_(*src), dst_(*dst), sc_(sc) {
+        abi_check(src->width == dst->width && src->height == dst->height && src->channels == dst->channels,
+                  "dst must have the shape of src");
+        sync_ = !src->on_device || !dst->on_device;
+        ds_ = to_device(src_, true);
+        dd_ = to_device(dst_, false);
+        // In-place on the device: read from a copy.
+        if (src->on_device && dst->on_device && overlaps()) {
+            const size_t rows = static_cast<size_t>(ds_.h);
+            const size_t bytes = rows * ds_.stride;
+            void* copy = sc_.alloc(bytes);
+            HIPCALL(hipMemcpyAsync(copy, ds_.data, bytes, hipMemcpyDeviceToDevice, sc_.stream()));
+            ds_.data = copy;
+        }
+    }
+    const DevImage& src() const { return ds_; }
+    const DevImage& dst() const { return dd_; }
+    // Copies the result back for host destinations and blocks when the call
+    // is synchronous.
+    void finish() {
+        const hipStream_t s = sc_.stream();
+        if (!dst_.on_device && dd_.h > 0) {
+            const size_t row = packed_stride(dst_);
+            const size_t hs = dst_.row_stride_bytes ? dst_.row_stride_bytes : row;
+            HIPCALL(hipMemcpy2DAsync(dst_.data, hs, dd_.data, dd_.stride, row, dd_.h, hipMemcpyDeviceToHost, s));
+        }
+        if (sync_) HIPCALL(hipStreamSynchronize(s));
+    }
+    bool synchronous() const { return sync_; }
+
+private:
+    DevImage to_device(const ipfg_image& im, bool copy_in) {
+        DevImage d;
+        d.w = im.width;
+        d.h = im.height;
+        d.cn = im.channels;
+        d.dt = im.dtype;
+        const size_t row = packed_stride(im);
+        if (im.on_device) {
+            d.data = im.data;
+            d.stride = im.row_stride_bytes ? im.row_stride_bytes : row;
+            return d;
+        }
+        d.stride = row;
+        d.data = sc_.alloc(row * static_cast<size_t>(im.height));
+        if (copy_in && im.height > 0 && row > 0) {
+            const size_t hs = im.row_stride_bytes ? im.row_stride_bytes : row;
+            HIPCALL(hipMemcpy2DAsync(d.data, row, im.data, hs, row, im.height, hipMemcpyHostToDevice, sc_.stream()));
+        }
+        return d;
+    }
+    bool overlaps() const {
+        const char* a0 = static_cast<const char*>(ds_.data);
+        const char* a1 = a0 + static_cast<size_t>(ds_.h) * ds_.stride;
+        const char* b0 = static_cast<const char*>(dd_.data);
+        const char* b1 = b0 + static_cast<size_t>(dd_.h) * dd_.stride;
+        return a0 < b1 && b0 < a1;
+    }
+    ipfg_image src_, dst_;
+    CallScratch& sc_;
+    DevImage ds_{}, dd_{};
+    bool sync_ = false;
+};
+
+// Fix-up list capacity: 1/64 of the samples (at least 64K entries); beyond it
+// the fix-up kernel recomputes the whole image in FP64.
+unsigned long long fix_capacity(const DevImage& d) {
+    const unsigned long long n = static_cast<unsigned long long>(d.w) * d.h * d.cn;
+    const unsigned long long c = n / 64;
+    return c < 65536 ? 65536 : c;
+}
+
+void record_fixups(const FixList& f, hipStream_t s, bool synchronous) {
+    unsigned long long* h = fix_host_slot();
+    HIPCALL(hipMemcpyAsync(h, f.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    t_ctx.fix_valid = true;
+    (void)synchronous;
+}
+
+constexpr double kU32 = 1.0 / 16777216.0;  // unit roundoff of FP32 (2^-24)
+
+// Largest magnitude an input sample (or the Constant border value) can have.
+double input_bound(int dt, double bval, int border) {
+    double x = dt == IPFG_U8 ? 255.0 : dt == IPFG_U16 ? 65535.0 : dt == IPFG_I16 ? 32768.0 : 0.0;
+    if (border == IPFG_BORDER_CONSTANT) x = std::max(x, std::fabs(bval));
+    return x;
+}
+
+// Chooses the kernel family. The FP32 fast path reads u8/u16/i16/f32 images
+// exactly; certification is used for u8 inputs with integer outputs.
+int resolve_mode(int precision, int tin, int tout) {
+    if (precision == IPFG_PREC_EXACT) return MODE_EXACT;
+    const bool fast_in = tin == IPFG_U8 || tin == IPFG_U16 || tin == IPFG_I16 || tin == IPFG_F32;
+    if (!fast_in || tout == IPFG_F64) return MODE_EXACT;
+    if (dtype_is_int(tout)) {
+        if (precision == IPFG_PREC_FP32) return MODE_FAST;
+        return tin == IPFG_U8 ? MODE_CERTIFY : MODE_EXACT;
+    }
+    return MODE_FAST;  // float output
+}
+
+// gaussianKernel1D (filter.hpp:129-145), host FP64 with the platform exp.
+std::vector<double> gaussian_kernel(double sigma, int ksize) {
+    if (ksize <= 0) {
+        require(sigma > 0, "gaussianKernel1D: need sigma > 0 or ksize > 0");
+        ksize = 2 * std::max(1, static_cast<int>(3.0 * sigma + 0.5)) + 1;
+    }
+    require(ksize % 2 == 1, "gaussianKernel1D: ksize must be odd");
+    if (sigma <= 0) sigma = 0.3 * ((ksize - 1) * 0.5 - 1) + 0.8;
+    std::vector<double> k(static_cast<size_t>(ksize));
+    const int r = ksize / 2;
+    double sum = 0;
+    for (int i = 0; i < ksize; ++i) {
+        k[i] = std::exp(-0.5 * (i - r) * (i - r) / (sigma * sigma));
+        sum += k[i];
+    }
+    for (double& v : k) v /= sum;
+    return k;
+}
+
+void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky, int kh,
+              int ax, int ay, int border, double delta, double bval, int precision, ipfg_stream stream) {
+    require(kx != nullptr && ky != nullptr && kw > 0 && kh > 0, "separable filter: empty kernel");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    abi_check(ax >= 0 && ax < kw && ay >= 0 && ay < kh, "anchor outside kernel");
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    SepLaunch a{};
+    a.src = io.src();
+    a.dst = io.dst();
+    a.kw = kw;
+    a.kh = kh;
+    a.ax = ax;
+    a.ay = ay;
+    a.border = border;
+    a.delta = delta;
+    a.bval = bval;
+    a.mode = resolve_mode(precision, src->dtype, dst->dtype);
+    // Parameter block: kx, ky as double, then as float.
+    std::vector<char> blob(sizeof(double) * (kw + kh) + sizeof(float) * (kw + kh));
+    double* bd = reinterpret_cast<double*>(blob.data());
+    float* bf = reinterpret_cast<float*>(bd + kw + kh);
+    double sx = 0, sy = 0;
+    for (int i = 0; i < kw; ++i) {
+        bd[i] = kx[i];
+        bf[i] = static_cast<float>(kx[i]);
+        sx += std::fabs(kx[i]);
+    }
+    for (int j = 0; j < kh; ++j) {
+        bd[kw + j] = ky[j];
+        bf[kw + j] = static_cast<float>(ky[j]);
+        sy += std::fabs(ky[j]);
+    }
+    char* dev = static_cast<char*>(sc.upload(blob.data(), blob.size()));
+    a.kxd = reinterpret_cast<const double*>(dev);
+    a.kyd = a.kxd + kw;
+    a.kxf = reinterpret_cast<const float*>(a.kyd + kh);
+    a.kyf = a.kxf + kw;
+    if (a.mode == MODE_CERTIFY) {
+        const double X = input_bound(src->dtype, bval, border);
+        const double S = sx * sy * X;
+        a.bound = 1.01 * ((kw + kh + 5) * kU32 * S + (kh + 2) * kU32 * std::fabs(delta)) +
+                  1e-9 * (1.0 + S + std::fabs(delta));
+        a.fix = sc.fixlist(fix_capacity(a.dst));
+    }
+    if (a.mode == MODE_EXACT) a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
+    HIPCALL(launch_sep(a, s));
+    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s, io.synchronous());
+    else t_ctx.fix_valid = false;
+    io.finish();
+}
+
+
+bool is_integral_value(double v) { return std::isfinite(v) && std::floor(v) == v; }
+
+void box_impl(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int normalize, int border, double bval,
+              int precision, ipfg_stream stream) {
+    require(kw > 0 && kh > 0, "boxFilter: kernel size must be positive");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    BoxLaunch a{};
+    a.src = io.src();
+    a.dst = io.dst();
+    a.kw = kw;
+    a.kh = kh;
+    a.border = border;
+    a.normalize = normalize;
+    a.bval = bval;
+    // Integer images give exact integer window sums in any order (bit-exact);
+    // a fractional Constant value breaks that, as do float images.
+    const bool exact_sums = dtype_is_int(src->dtype) && (border != IPFG_BORDER_CONSTANT || is_integral_value(bval));
+    a.mode = MODE_FAST;
+    if (precision == IPFG_PREC_EXACT) a.mode = MODE_EXACT;
+    else if (!exact_sums && precision == IPFG_PREC_DEFAULT && dtype_is_int(dst->dtype)) a.mode = MODE_EXACT;
+    if (a.mode == MODE_EXACT) a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
+    HIPCALL(launch_box(a, s));
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
+void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel, int kw, int kh, int ax, int ay,
+              int border, double delta, double bval, int precision, ipfg_stream stream) {
+    require(kernel != nullptr && kw > 0 && kh > 0, "filter2D: kernel must be a non-empty single-channel image");
+    if (ax < 0) ax = kw / 2;
+    if (ay < 0) ay = kh / 2;
+    require(ax < kw && ay < kh, "filter2D: anchor outside kernel");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    if (precision == IPFG_PREC_BF16X3 || precision == IPFG_PREC_FP16X2)
+        throw Unsupported{"ipfg: split-precision MFMA filter2D is not available in this build"};
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    F2dLaunch a{};
+    a.src = io.src();
+    a.dst = io.dst();
+    a.kw = kw;
+    a.kh = kh;
+    a.ax = ax;
+    a.ay = ay;
+    a.border = border;
+    a.delta = delta;
+    a.bval = bval;
+    a.precision = precision;
+    a.mode = resolve_mode(precision, src->dtype, dst->dtype);
+    // Non-zero taps in row-major order (filter.hpp:171-178).
+    std::vector<int> taps;
+    std::vector<double> tw;
+    for (int j = 0; j < kh; ++j)
+        for (int i = 0; i < kw; ++i) {
+            const float w = kernel[static_cast<size_t>(j) * kw + i];
+            if (w != 0.f) {
+                taps.push_back(i);
+                taps.push_back(j);
+                tw.push_back(static_cast<double>(w));
+            }
+        }
+    a.ntaps = static_cast<int>(tw.size());
+    a.has_zero = a.ntaps < kw * kh;
+    const size_t nk = static_cast<size_t>(kw) * kh;
+    std::vector<char> blob(sizeof(double) * tw.size() + sizeof(float) * nk + sizeof(int) * taps.size());
+    std::memcpy(blob.data(), tw.data(), sizeof(double) * tw.size());
+    std::memcpy(blob.data() + sizeof(double) * tw.size(), kernel, sizeof(float) * nk);
+    std::memcpy(blob.data() + sizeof(double) * tw.size() + sizeof(float) * nk, taps.data(), sizeof(int) * taps.size());
+    char* dev = static_cast<char*>(sc.upload(blob.data(), blob.size()));
+    a.tapw = reinterpret_cast<const double*>(dev);
+    a.kf = reinterpret_cast<const float*>(dev + sizeof(double) * tw.size());
+    a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * tw.size() + sizeof(float) * nk);
+    if (a.mode == MODE_CERTIFY) {
+        const unsigned long long n = static_cast<unsigned long long>(a.dst.w) * a.dst.h * a.dst.cn;
+        a.fix = sc.fixlist(n / 8 < 65536 ? 65536 : n / 8);
+    }
+    HIPCALL(launch_filter2d(a, s));
+    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s, io.synchronous());
+    else t_ctx.fix_valid = false;
+    io.finish();
+}
+
+void median_impl(const ipfg_image* src, const ipfg_image* dst, int ksize, int border, ipfg_stream stream) {
+    require(ksize >= 1 && ksize % 2 == 1, "medianBlur: ksize must be odd and positive");
+    require(border != IPFG_BORDER_CONSTANT, "medianBlur: Constant border is not supported");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(src->dtype == dst->dtype, "medianBlur: dst must have the pixel type of src");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    t_ctx.fix_valid = false;
+    if (ksize == 1) {  // the reference returns a copy (filter.hpp:307)
+        const DevImage& a = io.src();
+        const DevImage& b = io.dst();
+        const size_t row = static_cast<size_t>(a.w) * a.cn * dtype_size(a.dt);
+        HIPCALL(hipMemcpy2DAsync(b.data, b.stride, a.data, a.stride, row, a.h, hipMemcpyDeviceToDevice, s));
+        io.finish();
+        return;
+    }
+    MedianLaunch a{};
+    a.src = io.src();
+    a.dst = io.dst();
+    a.k = ksize;
+    a.border = border;
+    HIPCALL(launch_median(a, s));
+    io.finish();
+}
+
+double nominal_max(int dt) {
+    switch (dt) {
+        case IPFG_U8: return 255.0;
+        case IPFG_U16: return 65535.0;
+        case IPFG_I16: return 32767.0;
+        case IPFG_I32: return 255.0;  // PixelTraits<int32>::max() (core.hpp:58-60)
+    }
+    return 1.0;
+}
+
+void bilateral_impl(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
+                    int border, int precision, ipfg_stream stream) {
+    require(sigma_color > 0 && sigma_space > 0, "bilateralFilter: sigmas must be positive");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(src->dtype == dst->dtype, "bilateralFilter: dst must have the pixel type of src");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    const int r = d <= 0 ? static_cast<int>(std::lround(sigma_space * 1.5)) : d / 2;
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    BilateralLaunch a{};
+    a.src = io.src();
+    a.dst = io.dst();
+    a.r = r;
+    a.border = border;
+    // Taps and FP64 space weights (filter.hpp:349-352).
+    std::vector<int> taps;
+    std::vector<double> sw;
+    for (int j = -r; j <= r; ++j)
+        for (int i = -r; i <= r; ++i)
+            if (i * i + j * j <= r * r) {
+                taps.push_back(i);
+                taps.push_back(j);
+                sw.push_back(std::exp(-(i * i + j * j) / (2 * sigma_space * sigma_space)));
+            }
+    a.ntaps = static_cast<int>(sw.size());
+    a.cc = -1.0 / (2 * sigma_color * sigma_color);
+    // Range table for integer images (filter.hpp:357-364).
+    std::vector<double> lut;
+    if (dtype_is_int(src->dtype)) {
+        const double maxd = nominal_max(src->dtype) * src->channels;
+        if (maxd <= (1 << 20)) {
+            lut.resize(static_cast<size_t>(maxd) + 1);
+            for (size_t k = 0; k < lut.size(); ++k) lut[k] = std::exp(static_cast<double>(k * k) * a.cc);
+        }
+    }
+    a.lut_len = static_cast<int>(lut.size());
+    const bool fast_ok = bilateral_fast_ok(src->dtype, src->channels, r, a.lut_len);
+    if (precision == IPFG_PREC_EXACT || !fast_ok) a.mode = MODE_EXACT;
+    else a.mode = precision == IPFG_PREC_FP32 ? MODE_FAST : MODE_CERTIFY;
+    const size_t n = sw.size(), nl = lut.size();
+    std::vector<char> blob(sizeof(double) * (n + nl) + sizeof(float) * (n + nl) + sizeof(int) * taps.size());
+    char* w = blob.data();
+    std::memcpy(w, sw.data(), sizeof(double) * n);
+    std::memcpy(w + sizeof(double) * n, lut.data(), sizeof(double) * nl);
+    float* f = reinterpret_cast<float*>(w + sizeof(double) * (n + nl));
+    for (size_t i = 0; i < n; ++i) f[i] = static_cast<float>(sw[i]);
+    for (size_t i = 0; i < nl; ++i) f[n + i] = static_cast<float>(lut[i]);
+    std::memcpy(w + sizeof(double) * (n + nl) + sizeof(float) * (n + nl), taps.data(), sizeof(int) * taps.size());
+    char* dev = static_cast<char*>(sc.upload(blob.data(), blob.size()));
+    a.spaced = reinterpret_cast<const double*>(dev);
+    a.lutd = nl ? a.spaced + n : nullptr;
+    a.spacef = reinterpret_cast<const float*>(dev + sizeof(double) * (n + nl));
+    a.lutf = nl ? a.spacef + n : nullptr;
+    a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * (n + nl) + sizeof(float) * (n + nl));
+    if (a.mode == MODE_CERTIFY) {
+        // |fp32 - FP64 reference| <= X * (2 ntaps + 10) u: weights (3 roundings),
+        // two sums of ntaps positive terms and one division.
+        const double X = 255.0;
+        a.bound = 1.01 * (2.0 * a.ntaps + 10.0) * kU32 * X + 1e-9 * X;
+        const unsigned long long np = static_cast<unsigned long long>(a.dst.w) * a.dst.h;
+        a.fix = sc.fixlist(np / 16 < 65536 ? 65536 : np / 16);
+    }
+    HIPCALL(launch_bilateral(a, s));
+    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s, io.synchronous());
+    else t_ctx.fix_valid = false;
+    io.finish();
+}
+
+}  // namespace
+}  // namespace ipfg
+
+using namespace ipfg;
+
+extern "C" {
+
+int ipfg_init(int device) {
+    return guarded([&] { init_device(device); });
+}
+
+int ipfg_shutdown(void) {
+    return guarded([&] {
+        if (g_device.load() >= 0) HIPCALL(hipDeviceSynchronize());
+    });
+}
+
+const char* ipfg_last_error(void) { return t_error.c_str(); }
+int ipfg_abi_version(void) { return IPFG_ABI_VERSION; }
+
+int ipfg_last_fixups(long long* count) {
+    return guarded([&] {
+        abi_check(count != nullptr, "null count");
+        *count = (t_ctx.fix_valid && t_ctx.fix_host) ? static_cast<long long>(*t_ctx.fix_host) : 0;
+    });
+}
+
+void ipfg_set_num_threads(int n) { g_threads = n < 0 ? 0 : n; }
+int ipfg_get_num_threads(void) {
+    const int n = g_threads.load();
+    if (n > 0) return n;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw ? static_cast<int>(hw) : 1;
+}
+
+int ipfg_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        abi_check(ptr != nullptr, "null pointer");
+        ensure_init();
+        HIPCALL(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+int ipfg_free(void* ptr) {
+    return guarded([&] {
+        ensure_init();
+        if (ptr) HIPCALL(hipFree(ptr));
+    });
+}
+int ipfg_memcpy_h2d(void* dst, const void* src, size_t bytes, ipfg_stream stream) {
+    return guarded([&] {
+        ensure_init();
+        const hipStream_t s = pick_stream(stream);
+        HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        HIPCALL(hipStreamSynchronize(s));
+    });
+}
+int ipfg_memcpy_d2h(void* dst, const void* src, size_t bytes, ipfg_stream stream) {
+    return guarded([&] {
+        ensure_init();
+        const hipStream_t s = pick_stream(stream);
+        HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HIPCALL(hipStreamSynchronize(s));
+    });
+}
+int ipfg_memset(void* dst, int value, size_t bytes, ipfg_stream stream) {
+    return guarded([&] {
+        ensure_init();
+        HIPCALL(hipMemsetAsync(dst, value, bytes, pick_stream(stream)));
+    });
+}
+int ipfg_stream_create(ipfg_stream* stream) {
+    return guarded([&] {
+        abi_check(stream != nullptr, "null stream pointer");
+        ensure_init();
+        hipStream_t s;
+        HIPCALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+int ipfg_stream_destroy(ipfg_stream stream) {
+    return guarded([&] {
+        ensure_init();
+        if (stream) HIPCALL(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    });
+}
+int ipfg_stream_synchronize(ipfg_stream stream) {
+    return guarded([&] {
+        ensure_init();
+        HIPCALL(hipStreamSynchronize(pick_stream(stream)));
+    });
+}
+int ipfg_device_synchronize(void) {
+    return guarded([&] {
+        ensure_init();
+        HIPCALL(hipDeviceSynchronize());
+    });
+}
+
+int ipfg_gaussian_kernel_1d(double sigma, int ksize, double* out, int capacity) {
+    int n = 0;
+    const int rc = guarded([&] {
+        const std::vector<double> k = gaussian_kernel(sigma, ksize);
+        for (int i = 0; i < static_cast<int>(k.size()) && i < capacity; ++i) out[i] = k[i];
+        n = static_cast<int>(k.size());
+    });
+    return rc == IPFG_OK ? n : -rc;
+}
+
+int ipfg_sep_filter(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky, int kh,
+                    int ax, int ay, int border, double delta, double border_value, int precision,
+                    ipfg_stream stream) {
+    return guarded([&] { sep_impl(src, dst, kx, kw, ky, kh, ax, ay, border, delta, border_value, precision, stream); });
+}
+
+int ipfg_gaussian_blur(const ipfg_image* src, const ipfg_image* dst, double sigma_x, double sigma_y, int ksize,
+                       int border, int precision, ipfg_stream stream) {
+    return guarded([&] {
+        if (sigma_y <= 0) sigma_y = sigma_x;
+        const std::vector<double> kx = gaussian_kernel(sigma_x, ksize);
+        const std::vector<double> ky = gaussian_kernel(sigma_y, ksize);
+        sep_impl(src, dst, kx.data(), static_cast<int>(kx.size()), ky.data(), static_cast<int>(ky.size()),
+                 static_cast<int>(kx.size()) / 2, static_cast<int>(ky.size()) / 2, border, 0.0, 0.0, precision,
+                 stream);
+    });
+}
+
+int ipfg_box_filter(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int normalize, int border,
+                    double border_value, int precision, ipfg_stream stream) {
+    return guarded([&] { box_impl(src, dst, kw, kh, normalize, border, border_value, precision, stream); });
+}
+
+int ipfg_filter2d(const ipfg_image* src, const ipfg_image* dst, const float* kernel, int kw, int kh, int ax, int ay,
+                  int border, double delta, double border_value, int precision, ipfg_stream stream) {
+    return guarded(
+        [&] { f2d_impl(src, dst, kernel, kw, kh, ax, ay, border, delta, border_value, precision, stream); });
+}
+
+int ipfg_median(const ipfg_image* src, const ipfg_image* dst, int ksize, int border, ipfg_stream stream) {
+    return guarded([&] { median_impl(src, dst, ksize, border, stream); });
+}
+
+int ipfg_bilateral(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
+                   int border, int precision, ipfg_stream stream) {
+    return guarded([&] { bilateral_impl(src, dst, d, sigma_color, sigma_space, border, precision, stream); });
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+/* ipfg.h -- C ABI of libipfg.so, the MI355X (gfx950) implementation of ipf's
+ * filtering path.
+ *
+ * The reference (ipf, a header-only C++17 library) has no FFI layer: its
+ * boundary is the set of function templates in namespace ipf declared in
+ * include/ipf/filter.hpp. Each entry point below replaces one of them; the
+ * header-only front end include/ipf/filter.hpp of this repository keeps the
+ * reference's C++ signatures and calls these functions. Citations are
+ * file:line in the reference source tree.
+ *
+ * Conventions
+ *  - Every function returns an ipfg_status; no C++ exception crosses the ABI.
+ *    On failure ipfg_last_error() returns the message for the calling thread.
+ *    Argument errors carry the reference's exact "ipf: ..." text and are raised
+ *    before any device work, in the reference's check order.
+ *  - Images are interleaved HWC, row-major: sample (x, y, c) lives at byte
+ *    y*row_stride + (x*channels + c)*sizeof(dtype)   (core.hpp:220-222).
+ *  - If src and dst are both device-resident (on_device != 0) the call is
+ *    asynchronous on `stream` (NULL: the calling thread's own stream).
+ *    If either is host memory the call stages through device memory and
+ *    returns when dst holds the result (the reference's synchronous contract).
+ *  - dst must not alias src.
+ */
+#ifndef IPFG_H
+#define IPFG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPFG_ABI_VERSION 1
+
+/* Pixel types: ipf's documented aliases ImageU8/U16/I16/I32/F/D (core.hpp:403-408). */
+typedef enum ipfg_dtype {
+    IPFG_U8 = 0,
+    IPFG_U16 = 1,
+    IPFG_I16 = 2,
+    IPFG_I32 = 3,
+    IPFG_F32 = 4,
+    IPFG_F64 = 5
+} ipfg_dtype;
+
+/* Same order and meaning as ipf::BorderMode (core.hpp:183). */
+typedef enum ipfg_border {
+    IPFG_BORDER_CONSTANT = 0,
+    IPFG_BORDER_REPLICATE = 1,
+    IPFG_BORDER_REFLECT = 2,
+    IPFG_BORDER_REFLECT101 = 3,
+    IPFG_BORDER_WRAP = 4
+} ipfg_border;
+
+/* Arithmetic used by the linear filters and the bilateral filter. */
+typedef enum ipfg_precision {
+    /* FP32 arithmetic on the GPU. Integer outputs are certified: every sample
+     * whose FP32 value lies within the FP32 error bound of a rounding tie is
+     * recomputed in FP64 in the reference's operation order, so integer
+     * outputs are bit-identical to the reference. Float outputs carry FP32
+     * rounding error (bounded, see DESIGN). FP64 images always take EXACT. */
+    IPFG_PREC_DEFAULT = 0,
+    /* FP64 in the reference's operation order for every sample, no fused
+     * multiply-add: bit-identical to the reference for every pixel type. */
+    IPFG_PREC_EXACT = 1,
+    /* FP32 arithmetic without certification (integer outputs may differ from
+     * the reference by one unit at rounding ties). */
+    IPFG_PREC_FP32 = 2,
+    /* filter2D only: MFMA implicit GEMM with split-precision weights
+     * (hi + lo), FP32 accumulation. */
+    IPFG_PREC_BF16X3 = 3,
+    IPFG_PREC_FP16X2 = 4
+} ipfg_precision;
+
+typedef enum ipfg_status {
+    IPFG_OK = 0,
+    IPFG_ERR_ARG = 1,         /* reference argument check failed (ipf::Error) */
+    IPFG_ERR_DEVICE = 2,      /* HIP runtime error */
+    IPFG_ERR_NOMEM = 3,       /* device or host allocation failed */
+    IPFG_ERR_UNSUPPORTED = 4, /* combination not provided by this build */
+    IPFG_ERR_STATE = 5        /* library not initialised / shut down */
+} ipfg_status;
+
+typedef struct ipfg_image {
+    void* data;
+    int width;
+    int height;
+    int channels;
+    int dtype;               /* ipfg_dtype */
+    size_t row_stride_bytes; /* 0: packed rows (width*channels*sizeof) */
+    int on_device;           /* 0: host memory, 1: device memory */
+} ipfg_image;
+
+typedef void* ipfg_stream; /* a hipStream_t */
+
+/* ---------------------------------------------------------------- runtime */
+/* Selects the device and creates the library state. Called implicitly by
+ * every other entry point with device 0 if not called before. */
+int ipfg_init(int device);
+int ipfg_shutdown(void);
+const char* ipfg_last_error(void);
+int ipfg_abi_version(void);
+/* Number of samples recomputed in FP64 by the last certified call of this
+ * thread (device-resident calls: after the stream has been synchronised). */
+int ipfg_last_fixups(long long* count);
+
+/* Thread count for host-side staging; same contract as ipf::setNumThreads /
+ * getNumThreads (parallel.hpp:38-51). GPU work does not depend on it. */
+void ipfg_set_num_threads(int n);
+int ipfg_get_num_threads(void);
+
+/* Device memory and streams for resident images. */
+int ipfg_malloc(void** ptr, size_t bytes);
+int ipfg_free(void* ptr);
+int ipfg_memcpy_h2d(void* dst, const void* src, size_t bytes, ipfg_stream stream);
+int ipfg_memcpy_d2h(void* dst, const void* src, size_t bytes, ipfg_stream stream);
+int ipfg_memset(void* dst, int value, size_t bytes, ipfg_stream stream);
+int ipfg_stream_create(ipfg_stream* stream);
+int ipfg_stream_destroy(ipfg_stream stream);
+int ipfg_stream_synchronize(ipfg_stream stream);
+int ipfg_device_synchronize(void);
+
+/* ---------------------------------------------------------------- filters */
+
+/* gaussianKernel1D (filter.hpp:129-145): writes min(n, capacity) weights to
+ * out and returns n (> 0), or -IPFG_ERR_ARG. Host-only. */
+int ipfg_gaussian_kernel_1d(double sigma, int ksize, double* out, int capacity);
+
+/* sepFilter2D / detail::sepFilterImpl (filter.hpp:34-90, 196-203): horizontal
+ * correlation with kx (anchor ax) then vertical with ky (anchor ay), plus
+ * delta; src/dst types may differ (TOut). */
+int ipfg_sep_filter(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky,
+                    int kh, int ax, int ay, int border, double delta, double border_value, int precision,
+                    ipfg_stream stream);
+
+/* gaussianBlur (filter.hpp:261-267): sigmaY <= 0 uses sigmaX; ksize <= 0
+ * derives the window from sigma. */
+int ipfg_gaussian_blur(const ipfg_image* src, const ipfg_image* dst, double sigma_x, double sigma_y, int ksize,
+                       int border, int precision, ipfg_stream stream);
+
+/* boxFilter (filter.hpp:205-253); blur(src, k, border) is
+ * ipfg_box_filter(src, dst, k, k, 1, border, 0.0, ...) (filter.hpp:255-259). */
+int ipfg_box_filter(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int normalize, int border,
+                    double border_value, int precision, ipfg_stream stream);
+
+/* filter2D (filter.hpp:157-194): kernel is kw*kh row-major float weights,
+ * anchor (-1,-1) = centre; zero weights do not take part. */
+int ipfg_filter2d(const ipfg_image* src, const ipfg_image* dst, const float* kernel, int kw, int kh, int ax, int ay,
+                  int border, double delta, double border_value, int precision, ipfg_stream stream);
+
+/* medianBlur (filter.hpp:300-332): exact median of each ksize x ksize window
+ * (ksize odd); Constant border is rejected. Always bit-exact. */
+int ipfg_median(const ipfg_image* src, const ipfg_image* dst, int ksize, int border, ipfg_stream stream);
+
+/* bilateralFilter (filter.hpp:334-387). */
+int ipfg_bilateral(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
+                   int border, int precision, ipfg_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IPFG_H */

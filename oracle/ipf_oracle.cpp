@@ -1,0 +1,571 @@
+// TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
+//
+// CPU restatement ("the oracle") of ipf's filtering path, used only by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker and the
+// CPU baseline. The shipped library (libipfg.so) never links or calls it.
+//
+// Every function restates the reference algorithm with the same FP64 operation
+// order, so that results are bit-identical to the reference on the same inputs.
+// Build flags: -O2 -ffp-contract=off (no fused multiply-add, no -ffast-math).
+//
+// Reference citations (paths relative to the ipf source tree):
+//   border index ............ include/ipf/core.hpp:187-214
+//   saturate_cast ........... include/ipf/core.hpp:67-103
+//   parallel row split ...... include/ipf/parallel.hpp:55-95
+//   border table ............ include/ipf/filter.hpp:27-32
+//   separable filter ........ include/ipf/filter.hpp:34-90, 196-203
+//   gaussian kernel ......... include/ipf/filter.hpp:127-145, 261-267
+//   filter2D ................ include/ipf/filter.hpp:157-194
+//   box filter / blur ....... include/ipf/filter.hpp:205-259
+//   median .................. include/ipf/filter.hpp:269-332
+//   bilateral ............... include/ipf/filter.hpp:334-387
+//
+// Pinning: every function here is pinned by tests/test_oracle_*.py against the
+// reference-built fixtures in tests/golden/ and against closed forms, brute force
+// and the reference's own known-answer tests.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+namespace {
+
+enum DType { DT_U8 = 0, DT_U16 = 1, DT_I16 = 2, DT_I32 = 3, DT_F32 = 4, DT_F64 = 5 };
+enum Border { B_CONSTANT = 0, B_REPLICATE = 1, B_REFLECT = 2, B_REFLECT101 = 3, B_WRAP = 4 };
+
+thread_local std::string g_error;
+
+struct OracleError {
+    std::string msg;
+};
+void check(bool ok, const char* msg) {
+    if (!ok) throw OracleError{std::string("ipf: ") + msg};
+}
+
+// ---------------------------------------------------------------- threads
+// Row-range split with a minimum amount of work per thread
+// (include/ipf/parallel.hpp:55-95). Results never depend on the split.
+std::atomic<int> g_threads{0};
+
+int hw_threads() {
+    const int n = g_threads.load();
+    if (n > 0) return n;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw ? static_cast<int>(hw) : 1;
+}
+
+template <class Body>
+void rows_parallel(int n, double cost_per_row, Body body) {
+    if (n <= 0) return;
+    int t = std::min(hw_threads(), n);
+    t = std::min(t, std::max(1, static_cast<int>(n * cost_per_row / 32768.0)));
+    if (t <= 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::exception_ptr first;
+    std::mutex mu;
+    auto run = [&](int a, int b) {
+        try {
+            body(a, b);
+        } catch (...) {
+            std::lock_guard<std::mutex> g(mu);
+            if (!first) first = std::current_exception();
+        }
+    };
+    for (int k = 1; k < t; ++k) {
+        const int a = static_cast<int>(static_cast<long long>(n) * k / t);
+        const int b = static_cast<int>(static_cast<long long>(n) * (k + 1) / t);
+        pool.emplace_back(run, a, b);
+    }
+    run(0, static_cast<int>(static_cast<long long>(n) / t));
+    for (auto& th : pool) th.join();
+    if (first) std::rethrow_exception(first);
+}
+
+// ---------------------------------------------------------------- core
+// core.hpp:187-214. Far-out coordinates fold by the mode's period.
+int border_index(int p, int len, int mode) {
+    if (p >= 0 && p < len) return p;
+    if (len <= 0) return -1;
+    if (mode == B_CONSTANT) return -1;
+    if (mode == B_REPLICATE) return p < 0 ? 0 : len - 1;
+    if (mode == B_REFLECT) {
+        const long long per = 2LL * len;
+        long long q = ((p % per) + per) % per;
+        return static_cast<int>(q < len ? q : per - 1 - q);
+    }
+    if (mode == B_REFLECT101) {
+        if (len == 1) return 0;
+        const long long per = 2LL * len - 2;
+        long long q = ((p % per) + per) % per;
+        return static_cast<int>(q < len ? q : per - q);
+    }
+    if (mode == B_WRAP) {
+        const int q = p % len;
+        return q < 0 ? q + len : q;
+    }
+    return -1;
+}
+
+// filter.hpp:27-32: table[p] = border_index(p - before) for the padded range.
+std::vector<int> border_table(int len, int before, int after, int mode) {
+    std::vector<int> t(static_cast<size_t>(len + before + after));
+    for (size_t p = 0; p < t.size(); ++p) t[p] = border_index(static_cast<int>(p) - before, len, mode);
+    return t;
+}
+
+// core.hpp:67-103, specialised to a double source (every filter accumulates in
+// double): integers round half away from zero and clamp, NaN maps to 0; float
+// targets use the plain conversion.
+template <class To>
+To sat(double v) {
+    if constexpr (std::is_integral<To>::value) {
+        if (v != v) return To(0);
+        const double lo = static_cast<double>(std::numeric_limits<To>::lowest());
+        const double hi = static_cast<double>(std::numeric_limits<To>::max());
+        const double d = v < 0 ? v - 0.5 : v + 0.5;
+        if (d <= lo) return std::numeric_limits<To>::lowest();
+        if (d >= hi) return std::numeric_limits<To>::max();
+        return static_cast<To>(d);
+    } else {
+        return static_cast<To>(v);
+    }
+}
+
+// core.hpp:54-65 (nominal maximum; int32 images use 255).
+template <class T>
+double nominal_max() {
+    if constexpr (std::is_floating_point<T>::value) return 1.0;
+    else if constexpr (std::is_signed<T>::value && sizeof(T) >= 4) return 255.0;
+    else return static_cast<double>(std::numeric_limits<T>::max());
+}
+
+// ---------------------------------------------------------------- separable
+// filter.hpp:34-90. Horizontal pass into a double buffer of H+kh-1 padded rows,
+// taps accumulated from 0.0 in index order; vertical pass starts at delta and
+// accumulates rows in index order; one rounding at the end.
+template <class Ti, class To>
+void sep_filter(const Ti* src, int W, int H, int cn, To* dst, const double* kx, int kw, const double* ky, int kh,
+                int ax, int ay, int border, double delta, double bval) {
+    if (static_cast<long long>(W) * H == 0) return;
+    const std::vector<int> xt = border_table(W, ax, kw - 1 - ax, border);
+    const std::vector<int> yt = border_table(H, ay, kh - 1 - ay, border);
+    const int PH = H + kh - 1;
+    const size_t row = static_cast<size_t>(W) * cn;
+    std::vector<double> mid(static_cast<size_t>(PH) * row);
+    rows_parallel(PH, static_cast<double>(row) * kw, [&](int r0, int r1) {
+        std::vector<double> line(static_cast<size_t>(W + kw - 1) * cn);
+        for (int py = r0; py < r1; ++py) {
+            const int sy = yt[py];
+            for (int px = 0; px < W + kw - 1; ++px) {
+                const int sx = xt[px];
+                for (int c = 0; c < cn; ++c)
+                    line[static_cast<size_t>(px) * cn + c] =
+                        (sy < 0 || sx < 0) ? bval : static_cast<double>(src[(static_cast<size_t>(sy) * W + sx) * cn + c]);
+            }
+            double* out = &mid[static_cast<size_t>(py) * row];
+            for (size_t q = 0; q < row; ++q) out[q] = 0.0;
+            for (int i = 0; i < kw; ++i) {
+                const double k = kx[i];
+                const double* in = &line[static_cast<size_t>(i) * cn];
+                for (size_t q = 0; q < row; ++q) out[q] = out[q] + k * in[q];
+            }
+        }
+    });
+    rows_parallel(H, static_cast<double>(row) * kh, [&](int y0, int y1) {
+        std::vector<double> acc(row);
+        for (int y = y0; y < y1; ++y) {
+            for (size_t q = 0; q < row; ++q) acc[q] = delta;
+            for (int j = 0; j < kh; ++j) {
+                const double k = ky[j];
+                const double* in = &mid[static_cast<size_t>(y + j) * row];
+                for (size_t q = 0; q < row; ++q) acc[q] = acc[q] + k * in[q];
+            }
+            To* o = dst + static_cast<size_t>(y) * row;
+            for (size_t q = 0; q < row; ++q) o[q] = sat<To>(acc[q]);
+        }
+    });
+}
+
+// ---------------------------------------------------------------- box
+// filter.hpp:205-253. Horizontal running sums (window seeded by a left-to-right
+// sum, then += entering - leaving), vertical running sums per column, then the
+// sum times 1/(kw*kh) (or 1) rounded once.
+template <class Ti, class To>
+void box_filter(const Ti* src, int W, int H, int cn, To* dst, int kw, int kh, bool normalize, int border,
+                double bval) {
+    if (static_cast<long long>(W) * H == 0) return;
+    const int ax = kw / 2, ay = kh / 2;
+    const std::vector<int> xt = border_table(W, ax, kw - 1 - ax, border);
+    const std::vector<int> yt = border_table(H, ay, kh - 1 - ay, border);
+    const int PH = H + kh - 1;
+    const size_t row = static_cast<size_t>(W) * cn;
+    std::vector<double> hs(static_cast<size_t>(PH) * row);
+    rows_parallel(PH, static_cast<double>(row) * 4, [&](int r0, int r1) {
+        for (int py = r0; py < r1; ++py) {
+            const int sy = yt[py];
+            auto at = [&](int px, int c) -> double {
+                const int sx = xt[px];
+                if (sy < 0 || sx < 0) return bval;
+                return static_cast<double>(src[(static_cast<size_t>(sy) * W + sx) * cn + c]);
+            };
+            double* out = &hs[static_cast<size_t>(py) * row];
+            for (int c = 0; c < cn; ++c) {
+                double s = 0.0;
+                for (int i = 0; i < kw; ++i) s = s + at(i, c);
+                out[c] = s;
+                for (int x = 1; x < W; ++x) {
+                    const double step = at(x + kw - 1, c) - at(x - 1, c);
+                    s = s + step;
+                    out[static_cast<size_t>(x) * cn + c] = s;
+                }
+            }
+        }
+    });
+    const double scale = normalize ? 1.0 / (static_cast<double>(kw) * kh) : 1.0;
+    rows_parallel(static_cast<int>(row), static_cast<double>(H) * 3, [&](int q0, int q1) {
+        std::vector<double> col(static_cast<size_t>(q1 - q0), 0.0);
+        for (int j = 0; j < kh; ++j)
+            for (int q = q0; q < q1; ++q) col[q - q0] = col[q - q0] + hs[static_cast<size_t>(j) * row + q];
+        for (int y = 0; y < H; ++y) {
+            To* o = dst + static_cast<size_t>(y) * row;
+            for (int q = q0; q < q1; ++q) o[q] = sat<To>(col[q - q0] * scale);
+            if (y + 1 < H)
+                for (int q = q0; q < q1; ++q) {
+                    const double step = hs[static_cast<size_t>(y + kh) * row + q] - hs[static_cast<size_t>(y) * row + q];
+                    col[q - q0] = col[q - q0] + step;
+                }
+        }
+    });
+}
+
+// ---------------------------------------------------------------- filter2D
+// filter.hpp:157-194. Only taps whose float weight compares unequal to zero take
+// part, in row-major kernel order; s starts at delta.
+template <class Ti, class To>
+void filter2d(const Ti* src, int W, int H, int cn, To* dst, const float* K, int kw, int kh, int ax, int ay, int border,
+              double delta, double bval) {
+    if (static_cast<long long>(W) * H == 0) return;
+    std::vector<int> tdx, tdy;
+    std::vector<double> tw;
+    for (int j = 0; j < kh; ++j)
+        for (int i = 0; i < kw; ++i) {
+            const float w = K[static_cast<size_t>(j) * kw + i];
+            if (w != 0.f) {
+                tdx.push_back(i);
+                tdy.push_back(j);
+                tw.push_back(static_cast<double>(w));
+            }
+        }
+    const std::vector<int> xt = border_table(W, ax, kw - 1 - ax, border);
+    const std::vector<int> yt = border_table(H, ay, kh - 1 - ay, border);
+    const size_t nt = tw.size();
+    rows_parallel(H, static_cast<double>(W) * cn * (nt + 1), [&](int y0, int y1) {
+        for (int y = y0; y < y1; ++y)
+            for (int x = 0; x < W; ++x)
+                for (int c = 0; c < cn; ++c) {
+                    double s = delta;
+                    for (size_t t = 0; t < nt; ++t) {
+                        const int sx = xt[x + tdx[t]], sy = yt[y + tdy[t]];
+                        const double v = (sx < 0 || sy < 0)
+                                             ? bval
+                                             : static_cast<double>(src[(static_cast<size_t>(sy) * W + sx) * cn + c]);
+                        s = s + tw[t] * v;
+                    }
+                    dst[(static_cast<size_t>(y) * W + x) * cn + c] = sat<To>(s);
+                }
+    });
+}
+
+// ---------------------------------------------------------------- median
+// filter.hpp:300-332. The exact order statistic of rank k*k/2 of each window.
+// uint8 uses a sliding 256-bin histogram along x (the order statistic does not
+// depend on the method); other types select with nth_element.
+template <class T>
+void median(const T* src, int W, int H, int cn, T* dst, int k, int border) {
+    if (static_cast<long long>(W) * H == 0) return;
+    const size_t total = static_cast<size_t>(W) * H * cn;
+    if (k == 1) {
+        std::memcpy(dst, src, total * sizeof(T));
+        return;
+    }
+    const int r = k / 2;
+    const std::vector<int> xt = border_table(W, r, r, border);
+    const std::vector<int> yt = border_table(H, r, r, border);
+    const int rank = k * k / 2;
+    auto S = [&](int px, int py, int c) { return src[(static_cast<size_t>(yt[py]) * W + xt[px]) * cn + c]; };
+    if constexpr (std::is_same<T, uint8_t>::value) {
+        if (k >= 5) {
+            rows_parallel(H, static_cast<double>(W) * cn * k * 2, [&](int y0, int y1) {
+                for (int y = y0; y < y1; ++y)
+                    for (int c = 0; c < cn; ++c) {
+                        int hist[256] = {0};
+                        for (int j = 0; j < k; ++j)
+                            for (int i = 0; i < k; ++i) ++hist[S(i, y + j, c)];
+                        int m = 0, below = 0;  // m: current median, below: #samples < m
+                        while (below + hist[m] <= rank) below += hist[m++];
+                        dst[(static_cast<size_t>(y) * W) * cn + c] = static_cast<uint8_t>(m);
+                        for (int x = 1; x < W; ++x) {
+                            for (int j = 0; j < k; ++j) {
+                                const int out = S(x - 1, y + j, c), in = S(x + k - 1, y + j, c);
+                                --hist[out];
+                                if (out < m) --below;
+                                ++hist[in];
+                                if (in < m) ++below;
+                            }
+                            while (below > rank) below -= hist[--m];
+                            while (below + hist[m] <= rank) below += hist[m++];
+                            dst[(static_cast<size_t>(y) * W + x) * cn + c] = static_cast<uint8_t>(m);
+                        }
+                    }
+            });
+            return;
+        }
+    }
+    rows_parallel(H, static_cast<double>(W) * cn * k * k * 3, [&](int y0, int y1) {
+        std::vector<T> win(static_cast<size_t>(k) * k);
+        for (int y = y0; y < y1; ++y)
+            for (int x = 0; x < W; ++x)
+                for (int c = 0; c < cn; ++c) {
+                    size_t n = 0;
+                    for (int j = 0; j < k; ++j)
+                        for (int i = 0; i < k; ++i) win[n++] = S(x + i, y + j, c);
+                    std::nth_element(win.begin(), win.begin() + rank, win.end());
+                    dst[(static_cast<size_t>(y) * W + x) * cn + c] = win[rank];
+                }
+    });
+}
+
+// ---------------------------------------------------------------- bilateral
+// filter.hpp:334-387. Circular taps (i^2+j^2 <= r^2, j outer, i inner) with
+// exp space weights; range weight exp(dist^2 * cc) of the L1 colour distance,
+// looked up in a table for integer images; Constant border skips the tap;
+// out = acc / wsum rounded once.
+//
+// Divergence: for int16/int32 images whose L1 distance exceeds the table
+// (values outside the nominal range) the reference reads past its table; this
+// restatement evaluates exp directly for such distances instead.
+template <class T>
+void bilateral(const T* src, int W, int H, int cn, T* dst, int d, double sc, double ss, int border) {
+    const int r = d <= 0 ? static_cast<int>(std::lround(ss * 1.5)) : d / 2;
+    if (static_cast<long long>(W) * H == 0) return;
+    std::vector<int> tdx, tdy;
+    std::vector<double> tw;
+    for (int j = -r; j <= r; ++j)
+        for (int i = -r; i <= r; ++i)
+            if (i * i + j * j <= r * r) {
+                tdx.push_back(i);
+                tdy.push_back(j);
+                tw.push_back(std::exp(-(i * i + j * j) / (2 * ss * ss)));
+            }
+    const std::vector<int> xt = border_table(W, r, r, border);
+    const std::vector<int> yt = border_table(H, r, r, border);
+    const double cc = -1.0 / (2 * sc * sc);
+    std::vector<double> lut;
+    if constexpr (std::is_integral<T>::value) {
+        const double maxd = nominal_max<T>() * cn;
+        if (maxd <= (1 << 20)) {
+            lut.resize(static_cast<size_t>(maxd) + 1);
+            for (size_t k = 0; k < lut.size(); ++k) lut[k] = std::exp(static_cast<double>(k * k) * cc);
+        }
+    }
+    const size_t nt = tw.size();
+    rows_parallel(H, static_cast<double>(W) * nt * (cn + 2), [&](int y0, int y1) {
+        std::vector<double> acc(static_cast<size_t>(cn));
+        for (int y = y0; y < y1; ++y)
+            for (int x = 0; x < W; ++x) {
+                const T* p0 = src + (static_cast<size_t>(y) * W + x) * cn;
+                std::fill(acc.begin(), acc.end(), 0.0);
+                double wsum = 0.0;
+                for (size_t t = 0; t < nt; ++t) {
+                    const int sx = xt[x + r + tdx[t]], sy = yt[y + r + tdy[t]];
+                    if (sx < 0 || sy < 0) continue;
+                    const T* p = src + (static_cast<size_t>(sy) * W + sx) * cn;
+                    double dist = 0.0;
+                    for (int c = 0; c < cn; ++c)
+                        dist = dist + std::fabs(static_cast<double>(p[c]) - static_cast<double>(p0[c]));
+                    double rw;
+                    if (!lut.empty() && dist < static_cast<double>(lut.size()))
+                        rw = lut[static_cast<size_t>(dist)];
+                    else
+                        rw = std::exp(dist * dist * cc);
+                    const double w = tw[t] * rw;
+                    wsum = wsum + w;
+                    for (int c = 0; c < cn; ++c) acc[c] = acc[c] + w * static_cast<double>(p[c]);
+                }
+                T* o = dst + (static_cast<size_t>(y) * W + x) * cn;
+                for (int c = 0; c < cn; ++c) o[c] = sat<T>(acc[c] / wsum);
+            }
+    });
+}
+
+// ---------------------------------------------------------------- dispatch
+template <class F>
+int with_type(int dt, F f) {
+    switch (dt) {
+        case DT_U8: return f(static_cast<uint8_t*>(nullptr));
+        case DT_U16: return f(static_cast<uint16_t*>(nullptr));
+        case DT_I16: return f(static_cast<int16_t*>(nullptr));
+        case DT_I32: return f(static_cast<int32_t*>(nullptr));
+        case DT_F32: return f(static_cast<float*>(nullptr));
+        case DT_F64: return f(static_cast<double*>(nullptr));
+    }
+    throw OracleError{"ipf: unsupported pixel type"};
+}
+
+template <class F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const OracleError& e) {
+        g_error = e.msg;
+        return -1;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -2;
+    }
+}
+
+void check_image(int w, int h, int cn) {
+    check(w >= 0 && h >= 0, "Image: negative dimensions");
+    check(cn >= 1, "Image: channel count must be >= 1");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ipfo_last_error(void) { return g_error.c_str(); }
+void ipfo_set_num_threads(int n) { g_threads = n < 0 ? 0 : n; }
+int ipfo_get_num_threads(void) { return hw_threads(); }
+
+int ipfo_border_index(int p, int len, int mode) { return border_index(p, len, mode); }
+
+// Writes saturate_cast<dtype>(v) into *out (core.hpp:67-103, double source).
+int ipfo_saturate(double v, int dtype, void* out) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            *static_cast<T*>(out) = sat<T>(v);
+            return 0;
+        });
+    });
+}
+
+// gaussianKernel1D (filter.hpp:129-145). Returns the tap count, writes at most
+// cap weights; negative on error.
+int ipfo_gaussian_kernel_1d(double sigma, int ksize, double* out, int cap) {
+    return guarded([&] {
+        if (ksize <= 0) {
+            check(sigma > 0, "gaussianKernel1D: need sigma > 0 or ksize > 0");
+            ksize = 2 * std::max(1, static_cast<int>(3.0 * sigma + 0.5)) + 1;
+        }
+        check(ksize % 2 == 1, "gaussianKernel1D: ksize must be odd");
+        if (sigma <= 0) sigma = 0.3 * ((ksize - 1) * 0.5 - 1) + 0.8;
+        std::vector<double> k(static_cast<size_t>(ksize));
+        const int r = ksize / 2;
+        double sum = 0;
+        for (int i = 0; i < ksize; ++i) {
+            k[i] = std::exp(-0.5 * (i - r) * (i - r) / (sigma * sigma));
+            sum += k[i];
+        }
+        for (int i = 0; i < ksize; ++i) {
+            k[i] /= sum;
+            if (i < cap) out[i] = k[i];
+        }
+        return ksize;
+    });
+}
+
+int ipfo_sep_filter(const void* src, int w, int h, int cn, int tin, void* dst, int tout, const double* kx, int kw,
+                    const double* ky, int kh, int ax, int ay, int border, double delta, double bval) {
+    return guarded([&] {
+        check(kw > 0 && kh > 0 && kx && ky, "separable filter: empty kernel");
+        check_image(w, h, cn);
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                sep_filter<Ti, To>(static_cast<const Ti*>(src), w, h, cn, static_cast<To*>(dst), kx, kw, ky, kh, ax,
+                                   ay, border, delta, bval);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfo_box_filter(const void* src, int w, int h, int cn, int tin, void* dst, int tout, int kw, int kh, int normalize,
+                    int border, double bval) {
+    return guarded([&] {
+        check(kw > 0 && kh > 0, "boxFilter: kernel size must be positive");
+        check_image(w, h, cn);
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                box_filter<Ti, To>(static_cast<const Ti*>(src), w, h, cn, static_cast<To*>(dst), kw, kh,
+                                   normalize != 0, border, bval);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfo_filter2d(const void* src, int w, int h, int cn, int tin, void* dst, int tout, const float* K, int kw, int kh,
+                  int ax, int ay, int border, double delta, double bval) {
+    return guarded([&] {
+        check(K != nullptr && kw > 0 && kh > 0, "filter2D: kernel must be a non-empty single-channel image");
+        if (ax < 0) ax = kw / 2;
+        if (ay < 0) ay = kh / 2;
+        check(ax < kw && ay < kh, "filter2D: anchor outside kernel");
+        check_image(w, h, cn);
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                filter2d<Ti, To>(static_cast<const Ti*>(src), w, h, cn, static_cast<To*>(dst), K, kw, kh, ax, ay,
+                                 border, delta, bval);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfo_median(const void* src, int w, int h, int cn, int dtype, void* dst, int ksize, int border) {
+    return guarded([&] {
+        check(ksize >= 1 && ksize % 2 == 1, "medianBlur: ksize must be odd and positive");
+        check(border != B_CONSTANT, "medianBlur: Constant border is not supported");
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            median<T>(static_cast<const T*>(src), w, h, cn, static_cast<T*>(dst), ksize, border);
+            return 0;
+        });
+    });
+}
+
+int ipfo_bilateral(const void* src, int w, int h, int cn, int dtype, void* dst, int d, double sigma_color,
+                   double sigma_space, int border) {
+    return guarded([&] {
+        check(sigma_color > 0 && sigma_space > 0, "bilateralFilter: sigmas must be positive");
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            bilateral<T>(static_cast<const T*>(src), w, h, cn, static_cast<T*>(dst), d, sigma_color, sigma_space,
+                         border);
+            return 0;
+        });
+    });
+}
+
+}  // extern "C"

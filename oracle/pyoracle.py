@@ -1,0 +1,172 @@
+"""TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
+
+numpy front end for the CPU oracle (oracle/libipf_oracle.so, the restatement
+of ipf's filtering path) and, where it was built, the reference-built checker
+(oracle/_ref/libipf_ref.so). Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py use this module.
+
+    o = Oracle("oracle")      # the restatement
+    r = Oracle("ref")         # compiled from the reference headers
+    out = o.gaussian_blur(img, 2.0)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "libipf_oracle.so"
+REF_SO = HERE / "_ref" / "libipf_ref.so"
+
+U8, U16, I16, I32, F32, F64 = range(6)
+CONSTANT, REPLICATE, REFLECT, REFLECT101, WRAP = range(5)
+BORDERS = (CONSTANT, REPLICATE, REFLECT, REFLECT101, WRAP)
+
+NP2DT = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.int16): I16,
+         np.dtype(np.int32): I32, np.dtype(np.float32): F32, np.dtype(np.float64): F64}
+DT2NP = {v: k for k, v in NP2DT.items()}
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def build(ref: bool = True) -> None:
+    """Compiles the oracle (and the reference checker when the reference tree exists)."""
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-s", "-C", str(HERE), target], check=True)
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+_D, _I, _V = C.c_double, C.c_int, C.c_void_p
+_PD = C.POINTER(C.c_double)
+_PF = C.POINTER(C.c_float)
+
+
+class Oracle:
+    def __init__(self, which: str = "oracle"):
+        if which == "oracle":
+            path, pre = ORACLE_SO, "ipfo_"
+        elif which == "ref":
+            path, pre = REF_SO, "ipfr_"
+        else:
+            raise ValueError(which)
+        if not path.exists():
+            raise OracleError(f"{path} not built (make -C oracle)")
+        self.which = which
+        self.lib = C.CDLL(str(path))
+        self.p = pre
+        sig = {
+            "last_error": (C.c_char_p, []),
+            "set_num_threads": (None, [_I]),
+            "get_num_threads": (_I, []),
+            "border_index": (_I, [_I, _I, _I]),
+            "saturate": (_I, [_D, _I, _V]),
+            "gaussian_kernel_1d": (_I, [_D, _I, _PD, _I]),
+            "sep_filter": (_I, [_V, _I, _I, _I, _I, _V, _I, _PD, _I, _PD, _I, _I, _I, _I, _D, _D]),
+            "box_filter": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _I, _I, _I, _D]),
+            "filter2d": (_I, [_V, _I, _I, _I, _I, _V, _I, _PF, _I, _I, _I, _I, _I, _D, _D]),
+            "median": (_I, [_V, _I, _I, _I, _I, _V, _I, _I]),
+            "bilateral": (_I, [_V, _I, _I, _I, _I, _V, _I, _D, _D, _I]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(self.lib, pre + name)
+            f.restype, f.argtypes = res, args
+        if which == "ref":
+            self.lib.ipfr_last_seconds.restype = C.c_double
+            self.lib.ipfr_last_seconds.argtypes = []
+
+    def _f(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OracleError(self._f("last_error")().decode())
+
+    def last_seconds(self) -> float:
+        """Reference checker only: duration of the last ipf call, excluding copies."""
+        return self.lib.ipfr_last_seconds()
+
+    def set_num_threads(self, n: int) -> None:
+        self._f("set_num_threads")(n)
+
+    def get_num_threads(self) -> int:
+        return self._f("get_num_threads")()
+
+    def border_index(self, p, n, mode):
+        return self._f("border_index")(p, n, mode)
+
+    def saturate(self, v: float, dtype) -> object:
+        dt = NP2DT[np.dtype(dtype)]
+        out = np.zeros(1, DT2NP[dt])
+        self._check(self._f("saturate")(float(v), dt, out.ctypes.data))
+        return out[0]
+
+    def gaussian_kernel_1d(self, sigma, ksize=0):
+        n = self._f("gaussian_kernel_1d")(float(sigma), int(ksize), None, 0)
+        if n < 0:
+            self._check(n)
+        out = np.empty(n, np.float64)
+        self._f("gaussian_kernel_1d")(float(sigma), int(ksize), out.ctypes.data_as(_PD), n)
+        return out
+
+    @staticmethod
+    def _shape(a):
+        a = np.ascontiguousarray(a)
+        if a.ndim == 2:
+            return a, a.shape[0], a.shape[1], 1
+        return a, a.shape[0], a.shape[1], a.shape[2]
+
+    def sep_filter(self, src, kx, ky, anchor=None, border=REFLECT101, delta=0.0, bval=0.0, out_dtype=None):
+        src, h, w, cn = self._shape(src)
+        kx = np.ascontiguousarray(kx, np.float64)
+        ky = np.ascontiguousarray(ky, np.float64)
+        ax, ay = anchor if anchor is not None else (len(kx) // 2, len(ky) // 2)
+        out = np.empty(src.shape, out_dtype or src.dtype)
+        self._check(self._f("sep_filter")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                          NP2DT[out.dtype], kx.ctypes.data_as(_PD), len(kx),
+                                          ky.ctypes.data_as(_PD), len(ky), ax, ay, border, delta, bval))
+        return out
+
+    def gaussian_blur(self, src, sigma_x, sigma_y=0.0, ksize=0, border=REFLECT101, out_dtype=None):
+        if sigma_y <= 0:
+            sigma_y = sigma_x
+        kx = self.gaussian_kernel_1d(sigma_x, ksize)
+        ky = self.gaussian_kernel_1d(sigma_y, ksize)
+        return self.sep_filter(src, kx, ky, None, border, 0.0, 0.0, out_dtype)
+
+    def box_filter(self, src, kw, kh, normalize=True, border=REFLECT101, bval=0.0, out_dtype=None):
+        src, h, w, cn = self._shape(src)
+        out = np.empty(src.shape, out_dtype or src.dtype)
+        self._check(self._f("box_filter")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                          NP2DT[out.dtype], kw, kh, int(bool(normalize)), border, bval))
+        return out
+
+    def filter2d(self, src, kernel, border=REFLECT101, delta=0.0, anchor=(-1, -1), bval=0.0, out_dtype=None):
+        src, h, w, cn = self._shape(src)
+        k = np.ascontiguousarray(kernel, np.float32)
+        kh, kw = k.shape
+        out = np.empty(src.shape, out_dtype or src.dtype)
+        self._check(self._f("filter2d")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                        NP2DT[out.dtype], k.ctypes.data_as(_PF), kw, kh, anchor[0], anchor[1],
+                                        border, delta, bval))
+        return out
+
+    def median(self, src, ksize, border=REPLICATE):
+        src, h, w, cn = self._shape(src)
+        out = np.empty_like(src)
+        self._check(self._f("median")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, ksize, border))
+        return out
+
+    def bilateral(self, src, d, sigma_color, sigma_space, border=REFLECT101):
+        src, h, w, cn = self._shape(src)
+        out = np.empty_like(src)
+        self._check(self._f("bilateral")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, d,
+                                         sigma_color, sigma_space, border))
+        return out

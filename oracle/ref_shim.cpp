@@ -1,0 +1,169 @@
+// TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
+//
+// C-ABI wrapper around the reference ipf headers, compiled by oracle/Makefile
+// from the reference source tree (include/ipf/filter.hpp, core.hpp,
+// parallel.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
+// restatement bit-for-bit against the real reference, (2) write the golden
+// fixtures under tests/golden/, and (3) serve as the "reference" CPU baseline.
+// Signatures match oracle/ipf_oracle.cpp's ipfo_* functions one for one.
+#include <ipf/filter.hpp>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+
+namespace {
+thread_local std::string g_error;
+thread_local double g_seconds = 0.0;
+
+template <class F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+template <class F>
+int with_type(int dt, F f) {
+    switch (dt) {
+        case 0: return f(static_cast<std::uint8_t*>(nullptr));
+        case 1: return f(static_cast<std::uint16_t*>(nullptr));
+        case 2: return f(static_cast<std::int16_t*>(nullptr));
+        case 3: return f(static_cast<std::int32_t*>(nullptr));
+        case 4: return f(static_cast<float*>(nullptr));
+        case 5: return f(static_cast<double*>(nullptr));
+    }
+    g_error = "ipf: unsupported pixel type";
+    return -1;
+}
+
+template <class T>
+ipf::Image<T> wrap(const void* p, int w, int h, int cn) {
+    return ipf::Image<T>::fromData(w, h, cn, static_cast<const T*>(p));
+}
+
+template <class T>
+void unwrap(const ipf::Image<T>& img, void* dst) {
+    if (!img.empty()) std::memcpy(dst, img.data(), img.sampleCount() * sizeof(T));
+}
+
+template <class F>
+auto timed(F f) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto r = f();
+    g_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return r;
+}
+
+ipf::BorderMode bm(int b) { return static_cast<ipf::BorderMode>(b); }
+}  // namespace
+
+extern "C" {
+
+const char* ipfr_last_error(void) { return g_error.c_str(); }
+double ipfr_last_seconds(void) { return g_seconds; }
+void ipfr_set_num_threads(int n) { ipf::setNumThreads(n); }
+int ipfr_get_num_threads(void) { return ipf::getNumThreads(); }
+int ipfr_border_index(int p, int len, int mode) { return ipf::borderIndex(p, len, bm(mode)); }
+
+int ipfr_saturate(double v, int dtype, void* out) {
+    return with_type(dtype, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        *static_cast<T*>(out) = ipf::saturate_cast<T>(v);
+        return 0;
+    });
+}
+
+int ipfr_gaussian_kernel_1d(double sigma, int ksize, double* out, int cap) {
+    return guarded([&] {
+        const std::vector<double> k = ipf::gaussianKernel1D(sigma, ksize);
+        for (int i = 0; i < static_cast<int>(k.size()) && i < cap; ++i) out[i] = k[i];
+        return static_cast<int>(k.size());
+    });
+}
+
+int ipfr_sep_filter(const void* src, int w, int h, int cn, int tin, void* dst, int tout, const double* kx, int kw,
+                    const double* ky, int kh, int ax, int ay, int border, double delta, double bval) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                const auto img = wrap<Ti>(src, w, h, cn);
+                const std::vector<double> vx(kx, kx + kw), vy(ky, ky + kh);
+                const auto out = timed([&] {
+                    return ipf::detail::sepFilterImpl<To>(img, vx, vy, ax, ay, bm(border), delta, bval);
+                });
+                unwrap(out, dst);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfr_box_filter(const void* src, int w, int h, int cn, int tin, void* dst, int tout, int kw, int kh, int normalize,
+                    int border, double bval) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                const auto img = wrap<Ti>(src, w, h, cn);
+                const auto out =
+                    timed([&] { return ipf::boxFilter<To>(img, kw, kh, normalize != 0, bm(border), bval); });
+                unwrap(out, dst);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfr_filter2d(const void* src, int w, int h, int cn, int tin, void* dst, int tout, const float* K, int kw, int kh,
+                  int ax, int ay, int border, double delta, double bval) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                const auto img = wrap<Ti>(src, w, h, cn);
+                const ipf::Kernel k = ipf::Kernel::fromData(kw, kh, 1, K);
+                const auto out = timed([&] {
+                    return ipf::filter2D<To>(img, k, bm(border), delta, ipf::Point(ax, ay), bval);
+                });
+                unwrap(out, dst);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfr_median(const void* src, int w, int h, int cn, int dtype, void* dst, int ksize, int border) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            const auto out = timed([&] { return ipf::medianBlur(img, ksize, bm(border)); });
+            unwrap(out, dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_bilateral(const void* src, int w, int h, int cn, int dtype, void* dst, int d, double sigma_color,
+                   double sigma_space, int border) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            const auto out =
+                timed([&] { return ipf::bilateralFilter(img, d, sigma_color, sigma_space, bm(border)); });
+            unwrap(out, dst);
+            return 0;
+        });
+    });
+}
+
+}  // extern "C"
