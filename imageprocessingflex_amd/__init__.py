@@ -142,6 +142,17 @@ def _shape(a: np.ndarray):
     raise IpfError("ipf: image arrays must have shape (H, W) or (H, W, C)")
 
 
+def _rows_contiguous(a: np.ndarray) -> bool:
+    """Each row is packed (a view may skip bytes between rows)."""
+    if a.ndim not in (2, 3) or a.size == 0:
+        return a.flags.c_contiguous
+    it = a.dtype.itemsize
+    pix = it * (a.shape[2] if a.ndim == 3 else 1)
+    if a.ndim == 3 and a.strides[2] != it:
+        return False
+    return a.strides[1] == pix and a.strides[0] >= pix * a.shape[1]
+
+
 def _desc(a: np.ndarray) -> _lib.IpfgImage:
     if a.dtype not in _NP2DT:
         raise IpfError(f"ipfg: unsupported pixel type {a.dtype}")
@@ -155,7 +166,7 @@ def _io(src, out_dtype):
         dst = DeviceImage.like(src, out_dtype)
         return src.descriptor(), dst.descriptor(), dst
     src = np.asarray(src)
-    if not src.flags.c_contiguous:
+    if not _rows_contiguous(src):
         src = np.ascontiguousarray(src)
     dst = np.empty(src.shape, np.dtype(out_dtype) if out_dtype is not None else src.dtype)
     return _desc(src), _desc(dst), (dst, src)

@@ -6,9 +6,11 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <list>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.hpp"
@@ -159,12 +161,105 @@ private:
 };
 PinnedRing g_ring;
 
+// ------------------------------------------------------------------ param cache
+// Per-call parameter blocks (weights, tap lists, range tables) are cached on
+// the device keyed by their bytes, so a repeated call enqueues its kernels
+// only. An entry is uploaded once on the calling stream (other streams wait
+// on its ready event) and is freed on eviction only after its last user's
+// kernels completed.
+class ParamCache {
+public:
+    struct Entry {
+        void* dev = nullptr;
+        size_t bytes = 0;
+        hipEvent_t ready = nullptr;
+        hipEvent_t last_use = nullptr;
+        int users = 0;
+        bool used = false;
+        std::list<const std::string*>::iterator pos;
+    };
+
+    Entry* acquire(const std::string& key, hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = map_.find(key);
+        if (it != map_.end()) {
+            Entry& e = it->second;
+            lru_.erase(e.pos);
+            lru_.push_front(&it->first);
+            e.pos = lru_.begin();
+            ++e.users;
+            HIPCALL(hipStreamWaitEvent(s, e.ready, 0));
+            return &e;
+        }
+        evict(key.size());
+        Entry e;
+        e.bytes = key.size();
+        HIPCALL(hipMalloc(&e.dev, e.bytes ? e.bytes : 16));
+        HIPCALL(hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
+        HIPCALL(hipEventCreateWithFlags(&e.last_use, hipEventDisableTiming));
+        if (e.bytes && !g_ring.copy(e.dev, key.data(), e.bytes, s))
+            HIPCALL(hipMemcpyAsync(e.dev, key.data(), e.bytes, hipMemcpyHostToDevice, s));
+        HIPCALL(hipEventRecord(e.ready, s));
+        e.users = 1;
+        auto ins = map_.emplace(key, e).first;
+        lru_.push_front(&ins->first);
+        ins->second.pos = lru_.begin();
+        total_ += e.bytes;
+        return &ins->second;
+    }
+
+    // The calling stream's kernels that read the entry have been enqueued.
+    void release(Entry* e, hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        (void)hipEventRecord(e->last_use, s);
+        e->used = true;
+        --e->users;
+    }
+
+private:
+    static constexpr size_t kMaxBytes = 256u << 20;
+    static constexpr size_t kMaxEntries = 4096;
+
+    void evict(size_t incoming) {
+        while (!lru_.empty() && (total_ + incoming > kMaxBytes || map_.size() >= kMaxEntries)) {
+            auto victim = std::prev(lru_.end());
+            bool freed = false;
+            for (auto it = victim;; --it) {  // least recently used entry nobody holds
+                auto mit = map_.find(**it);
+                Entry& e = mit->second;
+                if (e.users == 0) {
+                    if (e.used) HIPCALL(hipEventSynchronize(e.last_use));
+                    HIPCALL(hipEventSynchronize(e.ready));
+                    HIPCALL(hipFree(e.dev));
+                    HIPCALL(hipEventDestroy(e.ready));
+                    HIPCALL(hipEventDestroy(e.last_use));
+                    total_ -= e.bytes;
+                    lru_.erase(it);
+                    map_.erase(mit);
+                    freed = true;
+                    break;
+                }
+                if (it == lru_.begin()) break;
+            }
+            if (!freed) return;
+        }
+    }
+
+    std::mutex mu_;
+    std::unordered_map<std::string, Entry> map_;
+    std::list<const std::string*> lru_;
+    size_t total_ = 0;
+};
+ParamCache g_params;
+
 // ------------------------------------------------------------------ per call
-// Device memory for one call, allocated and freed in stream order.
+// Device memory for one call, allocated and freed in stream order, plus the
+// cached parameter blocks it reads.
 class CallScratch {
 public:
     explicit CallScratch(hipStream_t s) : s_(s) {}
     ~CallScratch() {
+        for (auto* e : held_) g_params.release(e, s_);
         for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
     }
     void* alloc(size_t bytes) {
@@ -173,10 +268,11 @@ public:
         ptrs_.push_back(p);
         return p;
     }
-    void* upload(const void* data, size_t bytes) {
-        void* d = alloc(bytes);
-        if (bytes && !g_ring.copy(d, data, bytes, s_)) HIPCALL(hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, s_));
-        return d;
+    // Device copy of a parameter block (cached across calls).
+    void* params(const std::vector<char>& blob) {
+        ParamCache::Entry* e = g_params.acquire(std::string(blob.data(), blob.size()), s_);
+        held_.push_back(e);
+        return e->dev;
     }
     FixList fixlist(unsigned long long capacity) {
         FixList f;
@@ -190,6 +286,7 @@ public:
 private:
     hipStream_t s_;
     std::vector<void*> ptrs_;
+    std::vector<ParamCache::Entry*> held_;
 };
 
 size_t packed_stride(const ipfg_image& im) {
@@ -369,7 +466,7 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         bf[kw + j] = static_cast<float>(ky[j]);
         sy += std::fabs(ky[j]);
     }
-    char* dev = static_cast<char*>(sc.upload(blob.data(), blob.size()));
+    char* dev = static_cast<char*>(sc.params(blob));
     a.kxd = reinterpret_cast<const double*>(dev);
     a.kyd = a.kxd + kw;
     a.kxf = reinterpret_cast<const float*>(a.kyd + kh);
@@ -469,7 +566,7 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
     std::memcpy(blob.data(), tw.data(), sizeof(double) * tw.size());
     std::memcpy(blob.data() + sizeof(double) * tw.size(), kernel, sizeof(float) * nk);
     std::memcpy(blob.data() + sizeof(double) * tw.size() + sizeof(float) * nk, taps.data(), sizeof(int) * taps.size());
-    char* dev = static_cast<char*>(sc.upload(blob.data(), blob.size()));
+    char* dev = static_cast<char*>(sc.params(blob));
     a.tapw = reinterpret_cast<const double*>(dev);
     a.kf = reinterpret_cast<const float*>(dev + sizeof(double) * tw.size());
     a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * tw.size() + sizeof(float) * nk);
@@ -575,7 +672,7 @@ void bilateral_impl(const ipfg_image* src, const ipfg_image* dst, int d, double 
     for (size_t i = 0; i < n; ++i) f[i] = static_cast<float>(sw[i]);
     for (size_t i = 0; i < nl; ++i) f[n + i] = static_cast<float>(lut[i]);
     std::memcpy(w + sizeof(double) * (n + nl) + sizeof(float) * (n + nl), taps.data(), sizeof(int) * taps.size());
-    char* dev = static_cast<char*>(sc.upload(blob.data(), blob.size()));
+    char* dev = static_cast<char*>(sc.params(blob));
     a.spaced = reinterpret_cast<const double*>(dev);
     a.lutd = nl ? a.spaced + n : nullptr;
     a.spacef = reinterpret_cast<const float*>(dev + sizeof(double) * (n + nl));

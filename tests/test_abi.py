@@ -1,0 +1,75 @@
+"""CPU checks of the C ABI: libipfg.so loads, exports every function that
+include/ipfg.h declares, and rejects bad arguments with the reference's
+messages before touching a device. No kernel runs here."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "ipfg.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(ipfg_\w+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for n in ("ipfg_sep_filter", "ipfg_gaussian_blur", "ipfg_box_filter", "ipfg_filter2d", "ipfg_median",
+              "ipfg_bilateral", "ipfg_init", "ipfg_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from imageprocessingflex_amd import _lib
+    lib = C.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding covers exactly the header
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+    _lib.load()
+
+
+def test_gaussian_kernel_host_matches_oracle(oracle):
+    import imageprocessingflex_amd as ipf
+    for s, k in ((0.5, 0), (1.0, 0), (2.0, 0), (8.0, 0), (0.0, 7), (1.3, 9)):
+        assert np.array_equal(ipf.gaussian_kernel_1d(s, k), oracle.gaussian_kernel_1d(s, k))
+    with pytest.raises(ipf.IpfError, match="ipf: gaussianKernel1D: ksize must be odd"):
+        ipf.gaussian_kernel_1d(1.0, 4)
+    with pytest.raises(ipf.IpfError, match="ipf: gaussianKernel1D: need sigma > 0 or ksize > 0"):
+        ipf.gaussian_kernel_1d(0.0, 0)
+
+
+def test_argument_errors_precede_device_work():
+    import imageprocessingflex_amd as ipf
+    img = np.zeros((4, 5), np.uint8)
+    cases = [
+        (lambda: ipf.median_blur(img, 3, ipf.BorderMode.CONSTANT), "ipf: medianBlur: Constant border is not supported"),
+        (lambda: ipf.median_blur(img, 4), "ipf: medianBlur: ksize must be odd and positive"),
+        (lambda: ipf.median_blur(img, -1), "ipf: medianBlur: ksize must be odd and positive"),
+        (lambda: ipf.bilateral_filter(img, 5, 0.0, 3.0), "ipf: bilateralFilter: sigmas must be positive"),
+        (lambda: ipf.bilateral_filter(img, 5, 3.0, -1.0), "ipf: bilateralFilter: sigmas must be positive"),
+        (lambda: ipf.box_filter(img, 0, 3), "ipf: boxFilter: kernel size must be positive"),
+        (lambda: ipf.sep_filter2d(img, [], [1.0]), "ipf: separable filter: empty kernel"),
+        (lambda: ipf.filter2d(img, np.ones((3, 3)), anchor=(3, 1)), "ipf: filter2D: anchor outside kernel"),
+        (lambda: ipf.filter2d(img, np.ones((0, 3))), "ipf: filter2D: kernel must be a non-empty single-channel image"),
+        (lambda: ipf.gaussian_blur(img, 0.0), "ipf: gaussianKernel1D: need sigma > 0 or ksize > 0"),
+        (lambda: ipf.gaussian_blur(img, 1.0, ksize=4), "ipf: gaussianKernel1D: ksize must be odd"),
+    ]
+    for fn, msg in cases:
+        with pytest.raises(ipf.IpfError) as e:
+            fn()
+        assert str(e.value) == msg
+
+
+def test_reference_check_order():
+    # medianBlur checks ksize before the border (filter.hpp:304-305)
+    import imageprocessingflex_amd as ipf
+    with pytest.raises(ipf.IpfError, match="ksize must be odd"):
+        ipf.median_blur(np.zeros((3, 3), np.uint8), 2, ipf.BorderMode.CONSTANT)

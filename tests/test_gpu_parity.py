@@ -1,0 +1,230 @@
+"""GPU parity: libipfg (gfx950 kernels through the C ABI) against the
+reference-generated fixtures and the pinned oracle.
+
+Bar: integer outputs and the EXACT precision mode are bit-identical to the
+reference; float outputs of the FP32 path stay within the bound computed by
+tests.fixtures.float_tolerance (written next to the formula)."""
+from __future__ import annotations
+
+import hashlib
+import json
+import threading
+
+import numpy as np
+import pytest
+
+from tests import fixtures
+
+pytestmark = pytest.mark.gpu
+
+CASES = fixtures.load_cases()
+IDS = [fixtures.case_id(c) for c in CASES]
+
+
+def _exact_expected(case, precision) -> bool:
+    c, _, want, _ = case
+    if np.issubdtype(want.dtype, np.integer):
+        return True
+    if c["op"] == "median":
+        return True
+    if precision == 1:
+        # EXACT reproduces the reference bit for bit, except the float
+        # bilateral filter whose range weights use the device exp().
+        return c["op"] != "bilateral"
+    return False
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["default", "exact"])
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_fixture(ipf, case, precision):
+    out = fixtures.run_gpu(ipf, case, precision)
+    want = case[2]
+    assert out.dtype == want.dtype and out.shape == want.shape
+    if _exact_expected(case, precision):
+        assert np.array_equal(out, want, equal_nan=True), (
+            f"{(out != want).sum()} samples differ, max {np.abs(out.astype(np.float64) - want).max()}")
+    else:
+        tol = fixtures.float_tolerance(case)
+        err = np.abs(out.astype(np.float64) - want.astype(np.float64))
+        assert np.all(err <= tol), f"max err {err.max()} vs tol {np.max(tol)}"
+
+
+# ----------------------------------------------------------------- edge shapes
+EDGE_SHAPES = [(0, 5, 1), (5, 0, 3), (1, 1, 1), (1, 9, 3), (13, 1, 1), (2, 2, 2), (65, 63, 1), (31, 130, 4),
+               (17, 67, 5), (129, 257, 3)]
+
+
+@pytest.mark.parametrize("shape", EDGE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_edge_shapes_vs_oracle(ipf, oracle, shape):
+    rng = np.random.default_rng(sum(shape) + 7)
+    u8 = rng.integers(0, 256, shape, dtype=np.uint8)
+    f = rng.random(shape).astype(np.float32)
+    u16 = rng.integers(0, 65536, shape, dtype=np.uint16)
+    i32 = rng.integers(-1000, 1000, shape).astype(np.int32)
+    f64 = rng.random(shape)
+    for b in range(5):
+        for a in (u8, u16, i32, f64):
+            assert np.array_equal(ipf.gaussian_blur(a, 1.7, border=b), oracle.gaussian_blur(a, 1.7, border=b))
+        g = ipf.gaussian_blur(f, 2.0, border=b)
+        assert np.abs(g - oracle.gaussian_blur(f, 2.0, border=b)).max(initial=0) <= 3e-6
+        for a in (u8, u16, i32):
+            assert np.array_equal(ipf.box_filter(a, 5, 7, True, b, 2.0), oracle.box_filter(a, 5, 7, True, b, 2.0))
+        K = rng.normal(size=(5, 3)).astype(np.float32)
+        assert np.array_equal(ipf.filter2d(u8, K, b, 1.0, (-1, -1), 9.0), oracle.filter2d(u8, K, b, 1.0, (-1, -1), 9.0))
+        assert np.array_equal(ipf.bilateral_filter(u8, 5, 30.0, 2.0, b), oracle.bilateral(u8, 5, 30.0, 2.0, b))
+        if b:
+            for k in (3, 5, 7):
+                for a in (u8, u16, i32, f, f64):
+                    assert np.array_equal(ipf.median_blur(a, k, b), oracle.median(a, k, b))
+
+
+def test_types_and_outputs_vs_oracle(ipf, oracle):
+    rng = np.random.default_rng(5)
+    shape = (45, 70, 3)
+    imgs = {
+        np.uint8: rng.integers(0, 256, shape, dtype=np.uint8),
+        np.uint16: rng.integers(0, 65536, shape, dtype=np.uint16),
+        np.int16: rng.integers(-32768, 32768, shape).astype(np.int16),
+        np.int32: rng.integers(-5000, 5000, shape).astype(np.int32),
+        np.float32: rng.random(shape).astype(np.float32),
+        np.float64: rng.random(shape),
+    }
+    kx, ky = rng.normal(size=5), rng.normal(size=9)
+    for ti, a in imgs.items():
+        for to in imgs:
+            got = ipf.sep_filter2d(a, kx, ky, 3, 0.5, 0.0, to)
+            want = oracle.sep_filter(a, kx, ky, None, 3, 0.5, 0.0, to)
+            if np.issubdtype(np.dtype(to), np.integer) or ti in (np.int32, np.float64) or to == np.float64:
+                assert np.array_equal(got, want), (ti, to)
+            else:
+                x = np.abs(a.astype(np.float64)).max()
+                tol = 19 * 2.0 ** -24 * np.abs(kx).sum() * np.abs(ky).sum() * x * 1.01 + 2.0 ** -24 * np.abs(want)
+                assert np.all(np.abs(got.astype(np.float64) - want) <= tol), (ti, to)
+            assert np.array_equal(ipf.sep_filter2d(a, kx, ky, 3, 0.5, 0.0, to, precision=1), want), (ti, to)
+        if ti != np.float64:
+            assert np.array_equal(ipf.median_blur(a, 5, 1), oracle.median(a, 5, 1))
+        if np.issubdtype(np.dtype(ti), np.integer):
+            assert np.array_equal(ipf.bilateral_filter(a, 5, 40.0, 3.0), oracle.bilateral(a, 5, 40.0, 3.0))
+
+
+def test_row_stride_views(ipf, oracle):
+    rng = np.random.default_rng(11)
+    big = rng.integers(0, 256, (50, 90, 3), dtype=np.uint8)
+    view = big[3:47, 5:77]  # rows with a stride larger than their width
+    want = oracle.gaussian_blur(np.ascontiguousarray(view), 2.0)
+    assert np.array_equal(ipf.gaussian_blur(view, 2.0), want)
+    assert np.array_equal(ipf.median_blur(view, 5), oracle.median(np.ascontiguousarray(view), 5))
+
+
+def test_device_resident_async_and_in_place(ipf, oracle):
+    rng = np.random.default_rng(12)
+    img = rng.integers(0, 256, (300, 410, 3), dtype=np.uint8)
+    s = ipf.Stream()
+    d = ipf.DeviceImage.from_array(img, s)
+    outs = [ipf.gaussian_blur(d, sig, stream=s) for sig in (0.7, 2.0, 5.0)]
+    med = ipf.median_blur(d, 5, stream=s)
+    s.synchronize()
+    for sig, o in zip((0.7, 2.0, 5.0), outs):
+        assert np.array_equal(o.download(s), oracle.gaussian_blur(img, sig))
+    assert np.array_equal(med.download(s), oracle.median(img, 5))
+    # dst aliasing src on the device reads from a private copy
+    from imageprocessingflex_amd import _lib
+    import ctypes as C
+    desc = d.descriptor()
+    kx = np.array([0.25, 0.5, 0.25])
+    _lib.check(_lib.load().ipfg_sep_filter(C.byref(desc), C.byref(desc), kx.ctypes.data_as(C.POINTER(C.c_double)), 3,
+                                           kx.ctypes.data_as(C.POINTER(C.c_double)), 3, 1, 1, 3, 0.0, 0.0, 0,
+                                           s.handle))
+    s.synchronize()
+    assert np.array_equal(d.download(s), oracle.sep_filter(img, kx, kx))
+
+
+def test_concurrent_host_threads(ipf, oracle):
+    rng = np.random.default_rng(13)
+    imgs = [rng.integers(0, 256, (120 + 7 * i, 200 - 5 * i, 1 + i % 4), dtype=np.uint8) for i in range(8)]
+    want = [oracle.bilateral(im, 9, 40.0, 4.0) for im in imgs]
+    got = [None] * 8
+    errs = []
+
+    def work(i):
+        try:
+            for _ in range(3):
+                got[i] = ipf.bilateral_filter(imgs[i], 9, 40.0, 4.0)
+        except Exception as e:  # pragma: no cover
+            errs.append(e)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(8)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_certification_overflow_falls_back_to_exact(ipf, oracle):
+    # kernel [0.5, 0.5]: every odd pair sum sits exactly on a rounding tie,
+    # flagging far more samples than the fix-up list holds.
+    rng = np.random.default_rng(14)
+    img = rng.integers(0, 256, (257, 301), dtype=np.uint8)
+    k = np.array([0.5, 0.5])
+    got = ipf.sep_filter2d(img, k, k, 1)
+    assert np.array_equal(got, oracle.sep_filter(img, k, k, border=1))
+    assert ipf.last_fixups() > img.size // 64
+
+
+# ------------------------------------------------------------- full-size runs
+FULL = json.loads((fixtures.GOLDEN / "fullsize.json").read_text())
+BNAME = {0: "constant", 1: "replicate", 2: "reflect", 3: "reflect101", 4: "wrap"}
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_full_c1_gaussian_bit_exact(ipf):
+    from imageprocessingflex_amd import workloads as wl
+    out = ipf.gaussian_blur(wl.c1_image(), 2.0)
+    assert _sha(out) == FULL["c1_gauss2_reflect101_u8"]
+    print("c1 fix-ups", ipf.last_fixups())
+
+
+def test_full_c3_median_and_bilateral_bit_exact(ipf):
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.polygon_image()
+    d = ipf.DeviceImage.from_array(img)
+    for k in (3, 5):
+        for b in (1, 2, 3, 4):
+            assert _sha(ipf.median_blur(d, k, b).download()) == FULL[f"c3_median{k}_{BNAME[b]}"], (k, b)
+    out = ipf.bilateral_filter(d, 9, 75.0, 75.0).download()
+    assert _sha(out) == FULL["c3_bilateral9_75_75_reflect101"]
+
+
+def test_full_c4_filter2d_u8_bit_exact(ipf):
+    from imageprocessingflex_amd import workloads as wl
+    u8, _ = wl.c4_images()
+    k, _ = wl.c4_kernel()
+    assert _sha(ipf.filter2d(u8, k)) == FULL["c4_filter2d31_reflect101_u8"]
+    print("c4 fix-ups", ipf.last_fixups())
+
+
+def test_full_c2_exact_mode_bit_exact_and_fast_within_bound(ipf):
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.sinusoid_image()
+    d = ipf.DeviceImage.from_array(img)
+    U = 2.0 ** -24
+    for s in (1.0, 3.0, 8.0):
+        ks = 2 * max(1, int(3 * s + 0.5)) + 1
+        for b in range(5):
+            exact = ipf.gaussian_blur(d, s, border=b, precision=1).download()
+            assert _sha(exact) == FULL[f"c2_gauss{s:g}_{BNAME[b]}_f32"], (s, b)
+            fast = ipf.gaussian_blur(d, s, border=b).download()
+            # |fp32 - ref| <= (2 ks + 5) u X + u |ref|, X = 1 (image in [0, 1], kernel sums 1)
+            tol = (2 * ks + 5) * U * 1.01 + U * np.abs(exact)
+            assert np.all(np.abs(fast.astype(np.float64) - exact) <= tol), (s, b)
+    for b in range(5):
+        exact = ipf.blur(d, 15, b, precision=1).download()
+        assert _sha(exact) == FULL[f"c2_box15_{BNAME[b]}_f32"], b
+        fast = ipf.blur(d, 15, b).download()
+        assert np.all(np.abs(fast.astype(np.float64) - exact) <= 1e-12 * 225 + U * np.abs(exact)), b

@@ -1,0 +1,302 @@
+"""Pins the CPU oracle (oracle/ipf_oracle.cpp) before anything is checked
+against it: reference-generated fixtures and known answers, the reference's own
+unit-test expectations, brute-force definitions and invariants, and
+sensitivity checks that plant mistakes in the oracle's inputs."""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+from scipy import ndimage as ndi
+
+from tests import fixtures
+
+CASES = fixtures.load_cases()
+SCIPY_MODE = {0: "constant", 1: "nearest", 2: "reflect", 3: "mirror", 4: "wrap"}  # crossval mode map
+
+
+# ------------------------------------------------------------- golden fixtures
+@pytest.mark.parametrize("case", CASES, ids=[fixtures.case_id(c) for c in CASES])
+def test_oracle_matches_reference_fixture(oracle, case):
+    out = fixtures.run_oracle(oracle, case)
+    assert out.dtype == case[2].dtype and out.shape == case[2].shape
+    assert np.array_equal(out, case[2], equal_nan=True)
+
+
+def test_oracle_known_answers(oracle):
+    kat = json.loads((fixtures.GOLDEN / "kat.json").read_text())
+    for v, dt, want in kat["saturate"]:
+        assert int(oracle.saturate(v, np.dtype(dt))) == want, (v, dt)
+    for p, n, m, want in kat["border"]:
+        assert oracle.border_index(p, n, m) == want, (p, n, m)
+    for s, k, want in kat["gauss"]:
+        assert np.array_equal(oracle.gaussian_kernel_1d(s, k), np.array(want)), (s, k)
+
+
+# ------------------------------------------------ the reference's own test KATs
+def test_border_sequences_opencv(oracle):
+    # test_core.cpp:24-50: len 4, p = -3..4
+    exp = {2: [2, 1, 0, 0, 1, 2, 3, 3], 3: [3, 2, 1, 0, 1, 2, 3, 2], 4: [1, 2, 3, 0, 1, 2, 3, 0],
+           1: [0, 0, 0, 0, 1, 2, 3, 3]}
+    for m, seq in exp.items():
+        assert [oracle.border_index(p, 4, m) for p in range(-3, 5)] == seq
+    assert oracle.border_index(-1, 4, 0) == -1 and oracle.border_index(4, 4, 0) == -1
+    for p in range(-50, 50):
+        for m in (1, 2, 3, 4):
+            for n in range(1, 6):
+                assert 0 <= oracle.border_index(p, n, m) < n
+
+
+def test_border_closed_form(oracle):
+    # reflect101 of p into [0, n): period 2n-2 triangle wave; reflect: period 2n
+    for n in range(2, 9):
+        for p in range(-40, 40):
+            q = p % (2 * n - 2)
+            assert oracle.border_index(p, n, 3) == (q if q < n else 2 * n - 2 - q)
+            q = p % (2 * n)
+            assert oracle.border_index(p, n, 2) == (q if q < n else 2 * n - 1 - q)
+            assert oracle.border_index(p, n, 4) == p % n
+            assert oracle.border_index(p, n, 1) == min(max(p, 0), n - 1)
+
+
+def test_saturate_cast_kats(oracle):
+    # test_core.cpp:7-22
+    assert oracle.saturate(300, np.uint8) == 255
+    assert oracle.saturate(-5, np.uint8) == 0
+    assert oracle.saturate(127.5, np.uint8) == 128
+    assert oracle.saturate(127.49, np.uint8) == 127
+    assert oracle.saturate(-0.4, np.uint8) == 0
+    assert oracle.saturate(1e10, np.uint8) == 255
+    assert oracle.saturate(float("nan"), np.uint8) == 0
+    assert oracle.saturate(-40000, np.int16) == -32768
+    assert oracle.saturate(-2.5, np.int16) == -3
+    assert oracle.saturate(70000, np.uint16) == 65535
+    assert oracle.saturate(4e9, np.int32) == 2147483647
+    assert oracle.saturate(0.25, np.float32) == np.float32(0.25)
+
+
+def test_gaussian_kernel_closed_form(oracle):
+    # test_filter.cpp:9-16 and the closed form exp(-x^2 / 2 s^2) / sum
+    g = oracle.gaussian_kernel_1d(1.0)
+    assert len(g) == 7 and abs(g.sum() - 1) < 1e-12
+    assert abs(g[3] / g[4] - np.exp(0.5)) < 1e-12
+    assert len(oracle.gaussian_kernel_1d(0, 5)) == 5
+    for s in (0.5, 2.0, 3.0, 8.0):
+        g = oracle.gaussian_kernel_1d(s)
+        r = len(g) // 2
+        assert r == max(1, int(3 * s + 0.5))  # scipy truncate=3 radius
+        w = np.exp(-0.5 * (np.arange(-r, r + 1) / s) ** 2)
+        np.testing.assert_allclose(g, w / w.sum(), rtol=1e-14)
+    # sigma <= 0 derives sigma from ksize (OpenCV rule)
+    g = oracle.gaussian_kernel_1d(0, 9)
+    s = 0.3 * ((9 - 1) * 0.5 - 1) + 0.8
+    w = np.exp(-0.5 * (np.arange(-4, 5) / s) ** 2)
+    np.testing.assert_allclose(g, w / w.sum(), rtol=1e-14)
+
+
+def test_impulse_response(oracle):
+    # test_filter.cpp:34-47: correlating an impulse reproduces the flipped kernel
+    imp = np.zeros((7, 7), np.float32)
+    imp[3, 3] = 1
+    k = np.arange(1, 10, dtype=np.float32).reshape(3, 3)
+    r = oracle.filter2d(imp, k, border=0)
+    assert r[2, 2] == 9 and r[4, 4] == 1 and r[2, 3] == 8
+    s = oracle.sep_filter(imp, [1, 2, 3], [4, 5, 6], border=0)
+    assert s[2, 2] == 18 and s[2, 4] == 6 and s[4, 2] == 12
+
+
+def test_constant_invariance(oracle):
+    # test_filter.cpp:49-63
+    c = np.full((15, 20, 3), 77, np.uint8)
+    assert np.array_equal(oracle.gaussian_blur(c, 2.0), c)
+    assert np.array_equal(oracle.box_filter(c, 5, 5), c)
+    assert np.array_equal(oracle.median(c, 5), c)
+    assert np.array_equal(oracle.median(c, 9), c)
+    assert np.array_equal(oracle.bilateral(c, 7, 30, 3), c)
+    for m in (1, 2, 3, 4):
+        assert np.array_equal(oracle.box_filter(c, 4, 6, True, m), c)
+
+
+def test_box_equals_filter2d(oracle):
+    # test_filter.cpp:65-75
+    img = np.random.default_rng(9).integers(0, 256, (17, 31, 2), dtype=np.uint8)
+    for m in range(5):
+        fast = oracle.box_filter(img, 5, 3, True, m, 3.0, np.float32)
+        ref = oracle.filter2d(img, np.full((3, 5), 1 / 15, np.float32), m, 0.0, (-1, -1), 3.0, np.float32)
+        assert np.abs(fast - ref).max() < 1e-3
+        un = oracle.box_filter(img, 4, 4, False, m, 0.0, np.float32)
+        ref2 = oracle.filter2d(img, np.ones((4, 4), np.float32), m, 0.0, (-1, -1), 0.0, np.float32)
+        assert np.abs(un - ref2).max() < 1e-2
+
+
+def test_median_and_bilateral_kats(oracle):
+    # test_filter.cpp:100-121
+    salt = np.full((9, 9), 100, np.uint8)
+    salt[4, 4] = 255
+    salt[6, 2] = 0
+    m3 = oracle.median(salt, 3)
+    assert m3[4, 4] == 100 and m3[6, 2] == 100
+    step = np.where(np.arange(20) < 10, 20, 220).astype(np.uint8)[None, :].repeat(10, 0)
+    bf = oracle.bilateral(step, 9, 20, 5)
+    assert bf[5, 9] == 20 and bf[5, 10] == 220
+    assert oracle.gaussian_blur(step, 2.0)[5, 9] > 60
+
+
+# ------------------------------------------------------- brute-force definitions
+def _img(seed, shape, kind="f32"):
+    rng = np.random.default_rng(seed)
+    return rng.random(shape).astype(np.float32) if kind == "f32" else rng.integers(0, 256, shape, dtype=np.uint8)
+
+
+@pytest.mark.parametrize("mode", range(5))
+def test_filter2d_vs_scipy_correlate(oracle, mode):
+    # run_crossval.py:483-491 (max 2e-3, mean 1e-4)
+    img = _img(61, (33, 27))
+    k = np.random.default_rng(62).normal(size=(3, 5)).astype(np.float32)
+    want = ndi.correlate(img.astype(np.float64), k.astype(np.float64), mode=SCIPY_MODE[mode], cval=7.0)
+    got = oracle.filter2d(img, k, mode, 0.0, (-1, -1), 7.0)
+    assert np.abs(got - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("mode", range(5))
+def test_sep_vs_scipy_correlate1d(oracle, mode):
+    img = _img(63, (33, 27))
+    rng = np.random.default_rng(64)
+    kx, ky = rng.normal(size=7), rng.normal(size=3)
+    sm = SCIPY_MODE[mode]
+    want = ndi.correlate1d(ndi.correlate1d(img.astype(np.float64), kx, axis=1, mode=sm), ky, axis=0, mode=sm)
+    got = oracle.sep_filter(img, kx, ky, border=mode)
+    assert np.abs(got - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_box_and_gaussian_vs_scipy(oracle, mode):
+    img = _img(65, (33, 27, 3))
+    sm = SCIPY_MODE[mode]
+    for c in range(3):
+        want = ndi.uniform_filter(img[..., c].astype(np.float64), size=(3, 7), mode=sm)
+        assert np.abs(oracle.box_filter(img, 7, 3, True, mode)[..., c] - want).max() < 2e-6
+        want = ndi.gaussian_filter(img[..., c].astype(np.float64), sigma=(0.8, 2.3), truncate=3.0, mode=sm)
+        assert np.abs(oracle.gaussian_blur(img, 2.3, 0.8, 0, mode)[..., c] - want).max() < 2e-6
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_median_vs_scipy(oracle, mode):
+    u8 = _img(66, (33, 27, 3), "u8")
+    for k in (3, 5, 7, 11):
+        got = oracle.median(u8, k, mode)
+        for c in range(3):
+            assert np.array_equal(got[..., c], ndi.median_filter(u8[..., c], size=k, mode=SCIPY_MODE[mode]))
+    f = _img(67, (33, 27))
+    assert np.array_equal(oracle.median(f, 5, mode), ndi.median_filter(f, size=5, mode=SCIPY_MODE[mode]))
+
+
+def test_bilateral_vs_numpy_definition(oracle):
+    # run_crossval.py:537-555: brute force with L1 colour distance, reflect101
+    u8 = _img(68, (33, 27, 3), "u8")
+    for d, sc, ss in ((5, 20.0, 3.0), (9, 50.0, 4.0)):
+        r = d // 2
+        a = u8.astype(np.float64)
+        pad = np.pad(a, ((r, r), (r, r), (0, 0)), mode="reflect")
+        acc = np.zeros_like(a)
+        wsum = np.zeros(a.shape[:2])
+        h, w = a.shape[:2]
+        for j in range(-r, r + 1):
+            for i in range(-r, r + 1):
+                if i * i + j * j > r * r:
+                    continue
+                sh = pad[r + j:r + j + h, r + i:r + i + w]
+                dist = np.abs(sh - a).sum(-1)
+                wt = np.exp(-(i * i + j * j) / (2 * ss * ss)) * np.exp(-dist ** 2 / (2 * sc * sc))
+                acc += wt[..., None] * sh
+                wsum += wt
+        want = np.floor(acc / wsum[..., None] + 0.5)
+        got = oracle.bilateral(u8, d, sc, ss, 3).astype(np.float64)
+        assert np.abs(got - want).max() <= 1 and (got != want).mean() < 0.01
+
+
+def test_thread_count_independence(oracle):
+    # test_parallel.cpp:52-86: results do not depend on the thread count
+    img = _img(69, (211, 301, 3), "u8")
+    ops = [lambda: oracle.gaussian_blur(img, 2.5, out_dtype=np.float32),
+           lambda: oracle.box_filter(img, 7, 5, out_dtype=np.float32),
+           lambda: oracle.median(img, 5), lambda: oracle.median(img, 3),
+           lambda: oracle.bilateral(img, 7, 30, 3)]
+    for op in ops:
+        oracle.set_num_threads(1)
+        one = op()
+        oracle.set_num_threads(8)
+        many = op()
+        oracle.set_num_threads(0)
+        assert np.array_equal(one, many)
+
+
+def test_argument_errors(oracle):
+    from oracle.pyoracle import OracleError
+    img = np.zeros((4, 4), np.uint8)
+    with pytest.raises(OracleError, match="ipf: medianBlur: Constant border is not supported"):
+        oracle.median(img, 3, 0)
+    with pytest.raises(OracleError, match="ipf: medianBlur: ksize must be odd and positive"):
+        oracle.median(img, 4, 1)
+    with pytest.raises(OracleError, match="ipf: bilateralFilter: sigmas must be positive"):
+        oracle.bilateral(img, 5, 0.0, 1.0)
+    with pytest.raises(OracleError, match="ipf: boxFilter: kernel size must be positive"):
+        oracle.box_filter(img, 0, 3)
+    with pytest.raises(OracleError, match="ipf: gaussianKernel1D: ksize must be odd"):
+        oracle.gaussian_kernel_1d(1.0, 4)
+    with pytest.raises(OracleError, match="ipf: filter2D: anchor outside kernel"):
+        oracle.filter2d(img, np.ones((3, 3), np.float32), anchor=(3, 0))
+
+
+# ------------------------------------------------- planted mistakes are caught
+def _first(op, pred=lambda c: True):
+    return next(c for c in CASES if c[0]["op"] == op and pred(c[0]["params"]))
+
+
+def test_fixture_checks_catch_planted_mistakes(oracle):
+    # A transposed separable kernel, a shifted anchor, a wrong border mode and a
+    # dropped tap must each change a fixture result.
+    c, inp, want, ex = _first("sep", lambda p: p["border"] == 3 and "out" not in p and "anchor" not in p)
+    p = c["params"]
+    assert not np.array_equal(oracle.sep_filter(inp, ex["ky"], ex["kx"], None, 3, p["delta"], p["bval"]), want)
+    assert not np.array_equal(oracle.sep_filter(inp, ex["kx"], ex["ky"], (2, 1), 3, p["delta"], p["bval"]), want)
+    assert not np.array_equal(oracle.sep_filter(inp, ex["kx"], ex["ky"], None, 2, p["delta"], p["bval"]), want)
+    c, inp, want, ex = _first("filter2d", lambda p: p["border"] == 3 and "anchor" not in p)
+    k = ex["k"].copy()
+    k[0, 0] = 0
+    assert not np.array_equal(oracle.filter2d(inp, k, 3, 0.0, (-1, -1), c["params"]["bval"]), want)
+    c, inp, want, ex = _first("median", lambda p: p["k"] == 5)
+    assert not np.array_equal(oracle.median(inp, 3, c["params"]["border"]), want)
+    c, inp, want, ex = _first("bilateral", lambda p: p["d"] == 9)
+    p = c["params"]
+    assert not np.array_equal(oracle.bilateral(inp, 7, p["sc"], p["ss"], p["border"]), want)
+
+
+# --------------------------------------------------- live against the reference
+@pytest.mark.parametrize("seed", range(3))
+def test_oracle_equals_reference_live(oracle, ref, seed):
+    rng = np.random.default_rng(1000 + seed)
+    for shape in ((1, 1, 1), (2, 9, 3), (19, 1, 1), (23, 17, 4), (8, 40, 2)):
+        u8 = rng.integers(0, 256, shape, dtype=np.uint8)
+        f = rng.random(shape).astype(np.float32)
+        for b in range(5):
+            kx, ky = rng.normal(size=int(rng.integers(1, 12))), rng.normal(size=int(rng.integers(1, 12)))
+            ax, ay = int(rng.integers(0, len(kx))), int(rng.integers(0, len(ky)))
+            for a, od in ((u8, None), (u8, np.int16), (f, None), (u8, np.float32)):
+                assert np.array_equal(oracle.sep_filter(a, kx, ky, (ax, ay), b, 0.5, 2.0, od),
+                                      ref.sep_filter(a, kx, ky, (ax, ay), b, 0.5, 2.0, od))
+            kw, kh = int(rng.integers(1, 9)), int(rng.integers(1, 9))
+            for a in (u8, f):
+                assert np.array_equal(oracle.box_filter(a, kw, kh, bool(seed % 2), b, 1.5),
+                                      ref.box_filter(a, kw, kh, bool(seed % 2), b, 1.5))
+            K = rng.normal(size=(int(rng.integers(1, 8)), int(rng.integers(1, 8)))).astype(np.float32)
+            K[K < -1] = 0
+            for a in (u8, f):
+                assert np.array_equal(oracle.filter2d(a, K, b, 0.25, (-1, -1), 3.0),
+                                      ref.filter2d(a, K, b, 0.25, (-1, -1), 3.0))
+            assert np.array_equal(oracle.bilateral(u8, 7, 40.0, 3.0, b), ref.bilateral(u8, 7, 40.0, 3.0, b))
+            if b:
+                for k in (3, 5, 9):
+                    assert np.array_equal(oracle.median(u8, k, b), ref.median(u8, k, b))
+                    assert np.array_equal(oracle.median(f, k, b), ref.median(f, k, b))
