@@ -22,35 +22,51 @@ constexpr int TX = 64;
 
 __host__ __device__ inline int bil_xp(int r) { return (TX + 2 * r) | 1; }
 
-__host__ inline size_t bil_lds_bytes(int TH, int r, int lut_len) {
-    return sizeof(uint32_t) * static_cast<size_t>(TH + 2 * r) * bil_xp(r) + sizeof(float) * lut_len;
+__host__ inline size_t bil_lds_bytes(int TH, int r, int lut_len, size_t acc_size) {
+    return sizeof(uint32_t) * static_cast<size_t>(TH + 2 * r) * bil_xp(r) + acc_size * lut_len + 16;
 }
 
-template <int CN, bool CONST_BORDER>
-__global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(const uint8_t* __restrict__ src, size_t sstride, int W,
-                                                              int H, uint8_t* __restrict__ dst, size_t dstride,
-                                                              const int* __restrict__ taps,
-                                                              const float* __restrict__ spacef, int ntaps,
-                                                              const float* __restrict__ lutg, int lut_len, int r,
-                                                              int border, int TH, int certify, double bound,
-                                                              FixList fix) {
-    extern __shared__ uint32_t bsm[];
+__device__ inline double bil_exact_channel(const DevImage& src, int x, int y, int c, const int* taps,
+                                           const double* spaced, int ntaps, const double* lutd, int lut_len, int r,
+                                           int border, double cc);
+
+struct BilArgs {
+    DevImage src, dst;
+    const int* taps;
+    const void* space;    // ntaps space weights of type Acc
+    const void* lut;      // lut_len range weights of type Acc
+    const double* spaced; // FP64 tables for exact recomputation
+    const double* lutd;
+    int ntaps, lut_len, r, border, TH;
+    double cc;
+    double bound;         // certification bound on |result - reference|
+    FixList fix;
+};
+
+template <int CN, bool CONST_BORDER, class Acc, bool CERT>
+__global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
+    extern __shared__ __align__(16) uint32_t bsm[];
+    const int r = a.r, TH = a.TH;
+    const int W = a.src.w, H = a.src.h;
     const int XP = bil_xp(r);
     const int HR = TH + 2 * r;
     uint32_t* tile = bsm;
-    float* lut = reinterpret_cast<float*>(bsm + HR * XP);
+    Acc* lut = reinterpret_cast<Acc*>(bsm + ((HR * XP + 3) / 4) * 4);
+    const Acc* __restrict__ lutg = static_cast<const Acc*>(a.lut);
+    const Acc* __restrict__ space = static_cast<const Acc*>(a.space);
+    const uint8_t* __restrict__ src = static_cast<const uint8_t*>(a.src.data);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TH;
-    for (int i = tid; i < lut_len; i += kBlock) lut[i] = lutg[i];
+    for (int i = tid; i < a.lut_len; i += kBlock) lut[i] = lutg[i];
     const int XW = TX + 2 * r;
     for (int rr = wave; rr < HR; rr += kBlock / 64) {
         const int gy = y0 - r + rr;
-        const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, border);
+        const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, a.border);
         uint32_t* trow = tile + rr * XP;
-        const uint8_t* srow = src + static_cast<size_t>(sy < 0 ? 0 : sy) * sstride;
+        const uint8_t* srow = src + static_cast<size_t>(sy < 0 ? 0 : sy) * a.src.stride;
         for (int px = lane; px < XW; px += 64) {
             const int gx = x0 - r + px;
-            const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, border);
+            const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, a.border);
             uint32_t v = 0;
             if (sy >= 0 && sx >= 0) {
                 const uint8_t* p = srow + static_cast<size_t>(sx) * CN;
@@ -67,12 +83,12 @@ __global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(const uint8_t* __r
         if (x >= W || y >= H) continue;
         const uint32_t* cen = tile + (ly + r) * XP + (lx + r);
         const uint32_t p0 = *cen;
-        float wsum = 0.f;
-        float acc[CN];
+        Acc wsum = Acc(0);
+        Acc acc[CN];
 #pragma unroll
-        for (int c = 0; c < CN; ++c) acc[c] = 0.f;
-        for (int k = 0; k < ntaps; ++k) {
-            const int dx = taps[2 * k], dy = taps[2 * k + 1];
+        for (int c = 0; c < CN; ++c) acc[c] = Acc(0);
+        for (int k = 0; k < a.ntaps; ++k) {
+            const int dx = a.taps[2 * k], dy = a.taps[2 * k + 1];
             if (CONST_BORDER) {
                 if (static_cast<unsigned>(x + dx) >= static_cast<unsigned>(W) ||
                     static_cast<unsigned>(y + dy) >= static_cast<unsigned>(H))
@@ -80,20 +96,34 @@ __global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(const uint8_t* __r
             }
             const uint32_t p = cen[dy * XP + dx];
             const uint32_t dist = __builtin_amdgcn_sad_u8(p, p0, 0u);
-            const float w = spacef[k] * lut[dist];
+            const Acc w = space[k] * lut[dist];
             wsum += w;
 #pragma unroll
-            for (int c = 0; c < CN; ++c) acc[c] = fmaf(w, static_cast<float>((p >> (8 * c)) & 255u), acc[c]);
+            for (int c = 0; c < CN; ++c) {
+                const Acc pc = static_cast<Acc>((p >> (8 * c)) & 255u);
+                if constexpr (sizeof(Acc) == 8) acc[c] = fma(w, pc, acc[c]);
+                else acc[c] = fmaf(w, pc, acc[c]);
+            }
         }
-        uint8_t* o = dst + static_cast<size_t>(y) * dstride + static_cast<size_t>(x) * CN;
+        uint8_t* o = static_cast<uint8_t*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride + static_cast<size_t>(x) * CN;
         bool doubt = false;
 #pragma unroll
         for (int c = 0; c < CN; ++c) {
-            const float v = acc[c] / wsum;
+            const double v = static_cast<double>(acc[c] / wsum);
             o[c] = saturate_from_double<uint8_t>(v);
-            doubt |= needs_exact(v, bound, 0.0, 255.0);
+            if (CERT) doubt |= needs_exact(v, a.bound, 0.0, 255.0);
         }
-        if (certify && doubt) fix_push(fix, static_cast<unsigned long long>(y) * W + x);
+        if (CERT && doubt) fix_push(a.fix, static_cast<unsigned long long>(y) * W + x);
+    }
+    if (CERT) {
+        fix_epilogue(a.fix, static_cast<unsigned long long>(W) * H, [&](unsigned long long pix) {
+            const int y = static_cast<int>(pix / W);
+            const int x = static_cast<int>(pix - static_cast<unsigned long long>(y) * W);
+            uint8_t* o = static_cast<uint8_t*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride + static_cast<size_t>(x) * CN;
+            for (int c = 0; c < CN; ++c)
+                o[c] = saturate_from_double<uint8_t>(
+                    bil_exact_channel(a.src, x, y, c, a.taps, a.spaced, a.ntaps, a.lutd, a.lut_len, r, a.border, a.cc));
+        });
     }
 }
 
@@ -131,18 +161,11 @@ __global__ __launch_bounds__(kBlock) void bilateral_exact_kernel(DevImage src, D
                                                                  const int* __restrict__ taps,
                                                                  const double* __restrict__ spaced, int ntaps,
                                                                  const double* __restrict__ lutd, int lut_len, int r,
-                                                                 int border, double cc, FixList fix, int from_list) {
+                                                                 int border, double cc) {
     const size_t es = dtype_size(dst.dt);
-    unsigned long long N = static_cast<unsigned long long>(dst.w) * dst.h;
-    bool all = true;
-    if (from_list) {
-        const unsigned long long n = *fix.count;
-        all = n > fix.capacity;
-        if (!all) N = n;
-    }
-    for (unsigned long long k = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; k < N;
-         k += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
-        const unsigned long long pix = all ? k : fix.index[k];
+    const unsigned long long N = static_cast<unsigned long long>(dst.w) * dst.h;
+    for (unsigned long long pix = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; pix < N;
+         pix += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
         const int y = static_cast<int>(pix / dst.w);
         const int x = static_cast<int>(pix - static_cast<unsigned long long>(y) * dst.w);
         for (int c = 0; c < dst.cn; ++c) {
@@ -158,23 +181,45 @@ int grid_for(size_t n) {
     return static_cast<int>(b < 16384 ? (b ? b : 1) : 16384);
 }
 
-template <int CN, bool CB>
-hipError_t bil_fast_launch(const BilateralLaunch& a, int TH, size_t lds, hipStream_t s) {
-    const dim3 grid(static_cast<unsigned>((a.src.w + TX - 1) / TX), static_cast<unsigned>((a.src.h + TH - 1) / TH));
-    hipLaunchKernelGGL((bilateral_u8_kernel<CN, CB>), grid, dim3(kBlock), lds, s,
-                       static_cast<const uint8_t*>(a.src.data), a.src.stride, a.src.w, a.src.h,
-                       static_cast<uint8_t*>(a.dst.data), a.dst.stride, a.taps, a.spacef, a.ntaps, a.lutf, a.lut_len,
-                       a.r, a.border, TH, a.mode == MODE_CERTIFY ? 1 : 0, a.bound, a.fix);
+template <int CN, bool CB, class Acc, bool CERT>
+hipError_t bil_fast_launch(const BilateralLaunch& l, hipStream_t s) {
+    int TH = 32;
+    while (TH > 4 && bil_lds_bytes(TH, l.r, l.lut_len, sizeof(Acc)) > 64 * 1024) TH /= 2;
+    const size_t lds = bil_lds_bytes(TH, l.r, l.lut_len, sizeof(Acc));
+    BilArgs a{};
+    a.src = l.src;
+    a.dst = l.dst;
+    a.taps = l.taps;
+    a.space = sizeof(Acc) == 8 ? static_cast<const void*>(l.spaced) : static_cast<const void*>(l.spacef);
+    a.lut = sizeof(Acc) == 8 ? static_cast<const void*>(l.lutd) : static_cast<const void*>(l.lutf);
+    a.spaced = l.spaced;
+    a.lutd = l.lutd;
+    a.ntaps = l.ntaps;
+    a.lut_len = l.lut_len;
+    a.r = l.r;
+    a.border = l.border;
+    a.TH = TH;
+    a.cc = l.cc;
+    a.bound = l.bound;
+    a.fix = l.fix;
+    const dim3 grid(static_cast<unsigned>((l.src.w + TX - 1) / TX), static_cast<unsigned>((l.src.h + TH - 1) / TH));
+    hipLaunchKernelGGL((bilateral_u8_kernel<CN, CB, Acc, CERT>), grid, dim3(kBlock), lds, s, a);
     return hipGetLastError();
 }
 
+template <int CN, bool CB>
+hipError_t bil_fast_mode(const BilateralLaunch& a, hipStream_t s) {
+    if (a.mode == MODE_CERTIFY) return bil_fast_launch<CN, CB, double, true>(a, s);
+    return bil_fast_launch<CN, CB, float, false>(a, s);
+}
+
 template <bool CB>
-hipError_t bil_fast_cn(const BilateralLaunch& a, int TH, size_t lds, hipStream_t s) {
+hipError_t bil_fast_cn(const BilateralLaunch& a, hipStream_t s) {
     switch (a.src.cn) {
-        case 1: return bil_fast_launch<1, CB>(a, TH, lds, s);
-        case 2: return bil_fast_launch<2, CB>(a, TH, lds, s);
-        case 3: return bil_fast_launch<3, CB>(a, TH, lds, s);
-        case 4: return bil_fast_launch<4, CB>(a, TH, lds, s);
+        case 1: return bil_fast_mode<1, CB>(a, s);
+        case 2: return bil_fast_mode<2, CB>(a, s);
+        case 3: return bil_fast_mode<3, CB>(a, s);
+        case 4: return bil_fast_mode<4, CB>(a, s);
     }
     return hipErrorInvalidValue;
 }
@@ -183,29 +228,17 @@ hipError_t bil_fast_cn(const BilateralLaunch& a, int TH, size_t lds, hipStream_t
 
 // True when the uint8 fast kernel can run this call (tile fits LDS).
 bool bilateral_fast_ok(int dt, int cn, int r, int lut_len) {
-    return dt == IPFG_U8 && cn >= 1 && cn <= 4 && lut_len > 0 && bil_lds_bytes(4, r, lut_len) <= 64 * 1024;
+    return dt == IPFG_U8 && cn >= 1 && cn <= 4 && lut_len > 0 && bil_lds_bytes(4, r, lut_len, 8) <= 64 * 1024;
 }
 
 hipError_t launch_bilateral(const BilateralLaunch& a, hipStream_t s) {
     const size_t npix = static_cast<size_t>(a.src.w) * a.src.h;
     if (a.mode == MODE_EXACT) {
         hipLaunchKernelGGL(bilateral_exact_kernel, dim3(grid_for(npix)), dim3(kBlock), 0, s, a.src, a.dst, a.taps,
-                           a.spaced, a.ntaps, a.lutd, a.lut_len, a.r, a.border, a.cc, a.fix, 0);
+                           a.spaced, a.ntaps, a.lutd, a.lut_len, a.r, a.border, a.cc);
         return hipGetLastError();
     }
-    int TH = 32;
-    while (TH > 4 && bil_lds_bytes(TH, a.r, a.lut_len) > 64 * 1024) TH /= 2;
-    const size_t lds = bil_lds_bytes(TH, a.r, a.lut_len);
-    hipError_t e;
-    if (a.mode == MODE_CERTIFY) {
-        e = launch_zero_counter(a.fix.count, s);
-        if (e != hipSuccess) return e;
-    }
-    e = a.border == IPFG_BORDER_CONSTANT ? bil_fast_cn<true>(a, TH, lds, s) : bil_fast_cn<false>(a, TH, lds, s);
-    if (e != hipSuccess || a.mode != MODE_CERTIFY) return e;
-    hipLaunchKernelGGL(bilateral_exact_kernel, dim3(kFixBlocks), dim3(kBlock), 0, s, a.src, a.dst, a.taps, a.spaced,
-                       a.ntaps, a.lutd, a.lut_len, a.r, a.border, a.cc, a.fix, 1);
-    return hipGetLastError();
+    return a.border == IPFG_BORDER_CONSTANT ? bil_fast_cn<true>(a, s) : bil_fast_cn<false>(a, s);
 }
 
 }  // namespace ipfg

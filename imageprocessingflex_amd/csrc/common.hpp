@@ -131,26 +131,82 @@ __device__ inline double load_sample(const void* base, size_t byte_off, int dt) 
 // reference computes in FP64) determines saturate_cast<int type>(v) exactly.
 // The integer conversion is constant between consecutive half-integers and
 // outside [lo + 0.5, hi - 0.5]; anything else must be recomputed exactly.
-__device__ inline bool needs_exact(float r, double bound, double lo, double hi) {
-    const double rd = static_cast<double>(r);
-    if (!(rd == rd)) return true;  // NaN: let the exact path decide
-    const double a = rd - bound, b = rd + bound;
-    if (b < lo + 0.5 || a > hi - 0.5) return false;  // saturated either way
-    return floor(a + 0.5) != floor(b + 0.5);
+__device__ inline bool needs_exact(double r, double bound, double lo, double hi) {
+    if (!(r == r) || !(bound < 1e300)) return true;  // NaN or no usable bound: recompute
+    if (r + bound < lo + 0.5 || r - bound > hi - 0.5) return false;  // saturated either way
+    const double t = r + 0.5;
+    const double fr = t - floor(t);  // position between the jumps at integer t
+    return fr <= bound || fr >= 1.0 - bound;
 }
 
-// Device-side fix-up list: samples flagged by a certified kernel. When more
-// than `capacity` samples are flagged, the fix-up kernel recomputes the whole
-// image exactly.
+// Same question for a float32 output: static_cast<float>(v) is constant
+// between the midpoints of r's float neighbours. The midpoint on r's side of
+// f = (float)r is half a float spacing away: ulp(f) / 2 above, and below
+// either ulp(f) / 2 or, for a power of two, ulp(f) / 4.
+__device__ inline bool needs_exact_f32(double r, double bound) {
+    const float f = static_cast<float>(r);
+    const unsigned bits = __float_as_uint(f);
+    const int e = static_cast<int>((bits >> 23) & 0xffu);
+    if (e == 0xff || !(bound < 1e300)) return true;  // inf or NaN
+    const double side = r - static_cast<double>(f);
+    const bool toward_zero = (side < 0.0) == (f > 0.0f);
+    const bool halved = toward_zero && (bits & 0x7fffffu) == 0u && e > 1;
+    // 2^(max(e, 1) - 151) is half a spacing; one more halving on the short side
+    const int ee = (e > 1 ? e : 1) - (halved ? 1 : 0);
+    const double half = __longlong_as_double(static_cast<long long>(ee + 872) << 52);
+    return fabs(side) + bound >= half;
+}
+
+// Device-side fix-up list of a certified launch. The counter and the
+// finished-workgroup ticket belong to the stream (zero between launches); the
+// last workgroup of the launch recomputes the listed samples exactly and
+// resets both. When more than `capacity` samples are flagged the last
+// workgroup recomputes every sample.
 struct FixList {
-    unsigned long long* count;  // one counter, zeroed before the producing kernel
+    unsigned long long* count;
+    unsigned int* done;
+    unsigned long long* report;  // receives the launch's flag count
     unsigned long long* index;
     unsigned long long capacity;
 };
 
+// Flags one sample. The index is written with an agent-scope atomic store
+// (coherent across the XCDs' L2 caches), so no device-wide fence is needed.
 __device__ inline void fix_push(const FixList& f, unsigned long long idx) {
     const unsigned long long slot = atomicAdd(f.count, 1ull);
-    if (slot < f.capacity) f.index[slot] = idx;
+    if (slot < f.capacity) __hip_atomic_store(f.index + slot, idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by every thread of every workgroup after its last flag. The last
+// workgroup to arrive calls fix(i) for each flagged index (or every index
+// below `total` on overflow) and leaves the stream's counters at zero.
+// Ordering: each wave waits for its own stores to complete before the
+// workgroup takes a ticket, and the last workgroup reads the list with
+// agent-scope atomic loads -- no L2 write-back or invalidate is issued.
+template <class Fix>
+__device__ inline void fix_epilogue(const FixList& f, unsigned long long total, Fix fix) {
+    __shared__ int last;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int nblk = gridDim.x * gridDim.y * gridDim.z;
+        last = atomicAdd(f.done, 1u) == nblk - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    const unsigned long long n = __hip_atomic_load(f.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) *f.report = n;
+    if (n <= f.capacity) {
+        for (unsigned long long k = threadIdx.x; k < n; k += blockDim.x)
+            fix(__hip_atomic_load(f.index + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    } else {
+        for (unsigned long long k = threadIdx.x; k < total; k += blockDim.x) fix(k);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(f.count, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(f.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // Grid used by fix-up kernels (grid-stride loops over the list or the image).

@@ -28,13 +28,33 @@ __host__ inline size_t f2d_lds_bytes(int cn, int TH, int kw, int kh) {
     return sizeof(float) * static_cast<size_t>(cn) * (TH + kh - 1) * f2d_xp(kw);
 }
 
-template <class Ti, bool HZ, bool CERT>
-__global__ __launch_bounds__(kBlock) void f2d_fast_kernel(const Ti* __restrict__ src, size_t sstride, int W, int H,
-                                                          int cn, void* __restrict__ dst, size_t dstride, int dt_out,
-                                                          const float* __restrict__ K, int kw, int kh, int ax, int ay,
-                                                          int border, float delta, float bval, int TH,
-                                                          double bscale, double lo, double hi, FixList fix) {
+__device__ inline double f2d_exact_sample(const DevImage& src, int x, int y, int c, const int* taps,
+                                          const double* tw, int ntaps, int ax, int ay, int border, double delta,
+                                          double bval);
+
+struct F2dArgs {
+    DevImage src, dst;
+    const void* K;       // kw * kh weights of type Acc, row-major
+    const int* taps;     // non-zero taps (dx, dy) for exact recomputation
+    const double* tapw;
+    int ntaps, kw, kh, ax, ay, border, TH;
+    double delta, bval;
+    double bcoef, bdelta, xbound;  // certification bound = bcoef * X + bdelta
+    FixList fix;
+};
+
+template <class Acc>
+__device__ __forceinline__ Acc f2d_fma(Acc a, Acc b, Acc c) {
+    if constexpr (sizeof(Acc) == 8) return fma(a, b, c);
+    else return fmaf(a, b, c);
+}
+
+template <class Ti, class Acc, bool HZ, bool CERT>
+__global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
     extern __shared__ float smem[];
+    __shared__ unsigned int tile_max_bits;
+    const int W = a.src.w, H = a.src.h, cn = a.src.cn;
+    const int kw = a.kw, kh = a.kh, TH = a.TH;
     const int XP = f2d_xp(kw);
     const int HR = TH + kh - 1;
     const int plane = HR * XP;
@@ -42,33 +62,45 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(const Ti* __restrict__
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TH;
     const int XW = TX + kw - 1;
     const int rowSamples = XW * cn;
-    const bool inside_x = (x0 - ax >= 0) && (x0 - ax + XW <= W);
+    const bool inside_x = (x0 - a.ax >= 0) && (x0 - a.ax + XW <= W);
+    const float bval = static_cast<float>(a.bval);
+    const bool measure = CERT && a.xbound < 0;
+    const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
+    const Acc* __restrict__ K = static_cast<const Acc*>(a.K);
+    if (tid == 0) tile_max_bits = 0u;
+    float my_max = 0.f;
+    __syncthreads();
     for (int r = wave; r < HR; r += kBlock / 64) {
-        const int gy = y0 - ay + r;
-        const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, border);
+        const int gy = y0 - a.ay + r;
+        const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, a.border);
         float* trow = smem + r * XP;
         if (sy < 0) {
             for (int s = lane; s < rowSamples; s += 64) {
                 const int px = s / cn, c = s - px * cn;
                 trow[c * plane + px] = bval;
             }
+            if (measure) my_max = fmaxf(my_max, fabsf(bval));
             continue;
         }
-        const Ti* srow = reinterpret_cast<const Ti*>(reinterpret_cast<const char*>(src) + static_cast<size_t>(sy) * sstride);
+        const Ti* srow = reinterpret_cast<const Ti*>(reinterpret_cast<const char*>(src) + static_cast<size_t>(sy) * a.src.stride);
         for (int s = lane; s < rowSamples; s += 64) {
             const int px = s / cn, c = s - px * cn;
             float v;
             if (inside_x) {
-                v = static_cast<float>(srow[static_cast<size_t>(x0 - ax) * cn + s]);
+                v = static_cast<float>(srow[static_cast<size_t>(x0 - a.ax) * cn + s]);
             } else {
-                const int gx = x0 - ax + px;
-                const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, border);
+                const int gx = x0 - a.ax + px;
+                const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, a.border);
                 v = sx < 0 ? bval : static_cast<float>(srow[static_cast<size_t>(sx) * cn + c]);
             }
+            if (measure) my_max = fmaxf(my_max, (v == v) ? fabsf(v) : INFINITY);
             trow[c * plane + px] = v;
         }
     }
+    if (measure) atomicMax(&tile_max_bits, __float_as_uint(my_max));
     __syncthreads();
+    double bound = 0.0;
+    if (CERT) bound = a.bcoef * (measure ? static_cast<double>(__uint_as_float(tile_max_bits)) : a.xbound) + a.bdelta;
     constexpr int G = TX / P;
     const int ntask = TH * G * cn;
     for (int t = tid; t < ntask; t += kBlock) {
@@ -76,29 +108,23 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(const Ti* __restrict__
         const int rest = t / G;
         const int r = rest % TH;
         const int c = rest / TH;
-        float acc[P], aab[P];
+        Acc acc[P];
 #pragma unroll
-        for (int m = 0; m < P; ++m) {
-            acc[m] = delta;
-            aab[m] = 0.f;
-        }
+        for (int m = 0; m < P; ++m) acc[m] = static_cast<Acc>(a.delta);
         for (int j = 0; j < kh; ++j) {
             const float* base = smem + c * plane + (r + j) * XP + g * P;
-            const float* krow = K + j * kw;
-            float win[P];
+            const Acc* krow = K + j * kw;
+            Acc win[P];
 #pragma unroll
             for (int m = 0; m < P; ++m) win[m] = base[m];
             int i0 = 0;
             for (; i0 + P <= kw; i0 += P) {
 #pragma unroll
                 for (int ii = 0; ii < P; ++ii) {
-                    const float k = krow[i0 + ii];
-                    if (!HZ || k != 0.f) {
+                    const Acc k = krow[i0 + ii];
+                    if (!HZ || k != Acc(0)) {
 #pragma unroll
-                        for (int m = 0; m < P; ++m) {
-                            acc[m] = fmaf(k, win[(ii + m) % P], acc[m]);
-                            if (CERT) aab[m] = fmaf(fabsf(k), fabsf(win[(ii + m) % P]), aab[m]);
-                        }
+                        for (int m = 0; m < P; ++m) acc[m] = f2d_fma(k, win[(ii + m) % P], acc[m]);
                     }
                     win[ii] = base[i0 + ii + P];
                 }
@@ -106,13 +132,10 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(const Ti* __restrict__
 #pragma unroll
             for (int ii = 0; ii < P; ++ii) {
                 if (i0 + ii < kw) {
-                    const float k = krow[i0 + ii];
-                    if (!HZ || k != 0.f) {
+                    const Acc k = krow[i0 + ii];
+                    if (!HZ || k != Acc(0)) {
 #pragma unroll
-                        for (int m = 0; m < P; ++m) {
-                            acc[m] = fmaf(k, win[(ii + m) % P], acc[m]);
-                            if (CERT) aab[m] = fmaf(fabsf(k), fabsf(win[(ii + m) % P]), aab[m]);
-                        }
+                        for (int m = 0; m < P; ++m) acc[m] = f2d_fma(k, win[(ii + m) % P], acc[m]);
                     }
                     win[ii] = base[i0 + ii + P];
                 }
@@ -120,26 +143,52 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(const Ti* __restrict__
         }
         const int y = y0 + r;
         if (y >= H) continue;
-        char* drow = static_cast<char*>(dst) + static_cast<size_t>(y) * dstride;
+        char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
 #pragma unroll
         for (int m = 0; m < P; ++m) {
             const int x = x0 + g * P + m;
             if (x >= W) break;
             const size_t q = static_cast<size_t>(x) * cn + c;
-            switch (dt_out) {
-                case IPFG_U8: reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(acc[m]); break;
-                case IPFG_U16: reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(acc[m]); break;
-                case IPFG_I16: reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(acc[m]); break;
-                case IPFG_I32: reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(acc[m]); break;
-                case IPFG_F32: reinterpret_cast<float*>(drow)[q] = acc[m]; break;
-                case IPFG_F64: reinterpret_cast<double*>(drow)[q] = acc[m]; break;
+            const double v = static_cast<double>(acc[m]);
+            bool doubt = false;
+            switch (a.dst.dt) {
+                case IPFG_U8:
+                    reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(v);
+                    if (CERT) doubt = needs_exact(v, bound, 0.0, 255.0);
+                    break;
+                case IPFG_U16:
+                    reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(v);
+                    if (CERT) doubt = needs_exact(v, bound, 0.0, 65535.0);
+                    break;
+                case IPFG_I16:
+                    reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(v);
+                    if (CERT) doubt = needs_exact(v, bound, -32768.0, 32767.0);
+                    break;
+                case IPFG_I32:
+                    reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(v);
+                    if (CERT) doubt = needs_exact(v, bound, -2147483648.0, 2147483647.0);
+                    break;
+                case IPFG_F32:
+                    reinterpret_cast<float*>(drow)[q] = static_cast<float>(v);
+                    if (CERT) doubt = needs_exact_f32(v, bound);
+                    break;
+                default:
+                    reinterpret_cast<double*>(drow)[q] = v;
+                    break;
             }
-            if (CERT) {
-                // |fp32 sum - FP64 reference| <= bscale * (sum of |w||v| + |delta|)
-                const double bound = bscale * (static_cast<double>(aab[m]) + fabs(static_cast<double>(delta)));
-                if (needs_exact(acc[m], bound, lo, hi)) fix_push(fix, static_cast<unsigned long long>(y) * W * cn + q);
-            }
+            if (CERT && doubt) fix_push(a.fix, static_cast<unsigned long long>(y) * W * cn + q);
         }
+    }
+    if (CERT) {
+        const size_t rowLen = static_cast<size_t>(W) * cn;
+        const size_t es = dtype_size(a.dst.dt);
+        fix_epilogue(a.fix, static_cast<unsigned long long>(H) * rowLen, [&](unsigned long long s) {
+            const int y = static_cast<int>(s / rowLen);
+            const size_t q = s - static_cast<size_t>(y) * rowLen;
+            const int x = static_cast<int>(q / cn), c = static_cast<int>(q - static_cast<size_t>(x) * cn);
+            const double v = f2d_exact_sample(a.src, x, y, c, a.taps, a.tapw, a.ntaps, a.ax, a.ay, a.border, a.delta, a.bval);
+            store_sample(a.dst.data, static_cast<size_t>(y) * a.dst.stride + q * es, a.dst.dt, v);
+        });
     }
 }
 
@@ -176,59 +225,52 @@ __global__ __launch_bounds__(kBlock) void f2d_exact_kernel(DevImage src, DevImag
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f2d_fix_kernel(DevImage src, DevImage dst, const int* __restrict__ taps,
-                                                         const double* __restrict__ tw, int ntaps, int ax, int ay,
-                                                         int border, double delta, double bval, FixList fix) {
-    const unsigned long long n = *fix.count;
-    const size_t rowLen = static_cast<size_t>(dst.w) * dst.cn;
-    const bool all = n > fix.capacity;
-    const unsigned long long N = all ? static_cast<unsigned long long>(dst.h) * rowLen : n;
-    const size_t es = dtype_size(dst.dt);
-    for (unsigned long long k = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; k < N;
-         k += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
-        const unsigned long long s = all ? k : fix.index[k];
-        const int y = static_cast<int>(s / rowLen);
-        const size_t q = s - static_cast<size_t>(y) * rowLen;
-        const int x = static_cast<int>(q / dst.cn), c = static_cast<int>(q - static_cast<size_t>(x) * dst.cn);
-        const double v = f2d_exact_sample(src, x, y, c, taps, tw, ntaps, ax, ay, border, delta, bval);
-        store_sample(dst.data, static_cast<size_t>(y) * dst.stride + q * es, dst.dt, v);
-    }
-}
-
 int grid_for(size_t n) {
     const size_t b = (n + kBlock - 1) / kBlock;
     return static_cast<int>(b < 16384 ? (b ? b : 1) : 16384);
 }
 
-template <class Ti, bool HZ, bool CERT>
-hipError_t f2d_fast_launch(const F2dLaunch& a, hipStream_t s) {
-    const int cn = a.src.cn;
-    // rows per tile: at least 4 tasks per thread of work, LDS within 64 KB
+template <class Ti, class Acc, bool HZ, bool CERT>
+hipError_t f2d_fast_launch(const F2dLaunch& l, hipStream_t s) {
+    const int cn = l.src.cn;
+    // rows per tile: halo overhead (TH + kh - 1) / TH <= 1.25 up to 64 rows; LDS within 64 KB
     int TH = 16;
-    while (TH < 64 && static_cast<long long>(TH + a.kh - 1) * 4 > static_cast<long long>(TH) * 5) TH *= 2;
-    while (TH > 1 && f2d_lds_bytes(cn, TH, a.kw, a.kh) > 64 * 1024) TH /= 2;
-    const size_t lds = f2d_lds_bytes(cn, TH, a.kw, a.kh);
+    while (TH < 64 && static_cast<long long>(TH + l.kh - 1) * 4 > static_cast<long long>(TH) * 5) TH *= 2;
+    while (TH > 1 && f2d_lds_bytes(cn, TH, l.kw, l.kh) > 64 * 1024) TH /= 2;
+    const size_t lds = f2d_lds_bytes(cn, TH, l.kw, l.kh);
     if (lds > 64 * 1024) return hipErrorInvalidConfiguration;
-    const dim3 grid(static_cast<unsigned>((a.src.w + TX - 1) / TX), static_cast<unsigned>((a.src.h + TH - 1) / TH));
-    // per-sample bound factor: (taps + 4) unit roundoffs, 1% and FP64 slack
-    const double bscale = 1.01 * (a.ntaps + 5) / 16777216.0 + 1e-9;
-    hipLaunchKernelGGL((f2d_fast_kernel<Ti, HZ, CERT>), grid, dim3(kBlock), lds, s,
-                       static_cast<const Ti*>(a.src.data), a.src.stride, a.src.w, a.src.h, cn, a.dst.data,
-                       a.dst.stride, a.dst.dt, a.kf, a.kw, a.kh, a.ax, a.ay, a.border, static_cast<float>(a.delta),
-                       static_cast<float>(a.bval), TH, bscale, dtype_lo(a.dst.dt), dtype_hi(a.dst.dt), a.fix);
+    F2dArgs a{};
+    a.src = l.src;
+    a.dst = l.dst;
+    a.K = sizeof(Acc) == 8 ? static_cast<const void*>(l.kd) : static_cast<const void*>(l.kf);
+    a.taps = l.taps;
+    a.tapw = l.tapw;
+    a.ntaps = l.ntaps;
+    a.kw = l.kw;
+    a.kh = l.kh;
+    a.ax = l.ax;
+    a.ay = l.ay;
+    a.border = l.border;
+    a.TH = TH;
+    a.delta = l.delta;
+    a.bval = l.bval;
+    a.bcoef = l.bcoef;
+    a.bdelta = l.bdelta;
+    a.xbound = l.xbound;
+    a.fix = l.fix;
+    const dim3 grid(static_cast<unsigned>((l.src.w + TX - 1) / TX), static_cast<unsigned>((l.src.h + TH - 1) / TH));
+    hipLaunchKernelGGL((f2d_fast_kernel<Ti, Acc, HZ, CERT>), grid, dim3(kBlock), lds, s, a);
     return hipGetLastError();
 }
 
 template <class Ti>
 hipError_t f2d_fast_dispatch(const F2dLaunch& a, hipStream_t s) {
-    const bool cert = a.mode == MODE_CERTIFY;
-    if (a.has_zero) return cert ? f2d_fast_launch<Ti, true, true>(a, s) : f2d_fast_launch<Ti, true, false>(a, s);
-    return cert ? f2d_fast_launch<Ti, false, true>(a, s) : f2d_fast_launch<Ti, false, false>(a, s);
+    if (a.mode == MODE_CERTIFY)
+        return a.has_zero ? f2d_fast_launch<Ti, double, true, true>(a, s) : f2d_fast_launch<Ti, double, false, true>(a, s);
+    return a.has_zero ? f2d_fast_launch<Ti, float, true, false>(a, s) : f2d_fast_launch<Ti, float, false, false>(a, s);
 }
 
 }  // namespace
-
-hipError_t launch_filter2d_mfma(const F2dLaunch& a, hipStream_t s);
 
 hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s) {
     const size_t total = static_cast<size_t>(a.src.w) * a.src.h * a.src.cn;
@@ -237,22 +279,13 @@ hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s) {
                            a.tapw, a.ntaps, a.ax, a.ay, a.border, a.delta, a.bval);
         return hipGetLastError();
     }
-    hipError_t e;
-    if (a.mode == MODE_CERTIFY) {
-        e = launch_zero_counter(a.fix.count, s);
-        if (e != hipSuccess) return e;
-    }
     switch (a.src.dt) {
-        case IPFG_U8: e = f2d_fast_dispatch<uint8_t>(a, s); break;
-        case IPFG_U16: e = f2d_fast_dispatch<uint16_t>(a, s); break;
-        case IPFG_I16: e = f2d_fast_dispatch<int16_t>(a, s); break;
-        case IPFG_F32: e = f2d_fast_dispatch<float>(a, s); break;
-        default: return hipErrorInvalidValue;
+        case IPFG_U8: return f2d_fast_dispatch<uint8_t>(a, s);
+        case IPFG_U16: return f2d_fast_dispatch<uint16_t>(a, s);
+        case IPFG_I16: return f2d_fast_dispatch<int16_t>(a, s);
+        case IPFG_F32: return f2d_fast_dispatch<float>(a, s);
     }
-    if (e != hipSuccess || a.mode != MODE_CERTIFY) return e;
-    hipLaunchKernelGGL(f2d_fix_kernel, dim3(kFixBlocks), dim3(kBlock), 0, s, a.src, a.dst, a.taps, a.tapw, a.ntaps,
-                       a.ax, a.ay, a.border, a.delta, a.bval, a.fix);
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 }  // namespace ipfg

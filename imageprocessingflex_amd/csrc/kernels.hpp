@@ -10,18 +10,22 @@ enum Mode { MODE_FAST = 0, MODE_CERTIFY = 1, MODE_EXACT = 2 };
 // ---------------------------------------------------------------- separable
 struct SepLaunch {
     DevImage src, dst;
-    const float* kxf;   // kw FP32 weights (device)
-    const float* kyf;   // kh FP32 weights (device)
-    const double* kxd;  // kw FP64 weights (device)
-    const double* kyd;  // kh FP64 weights (device)
+    const float* kxf;   // kw FP32 weights + 16 zeros (device)
+    const float* kyf;   // kh FP32 weights + 16 zeros (device)
+    const double* kxd;  // kw FP64 weights + 16 zeros (device)
+    const double* kyd;  // kh FP64 weights + 16 zeros (device)
     int kw, kh, ax, ay, border;
     double delta, bval;
     int mode;
-    double bound;  // certification bound (MODE_CERTIFY)
+    // MODE_CERTIFY: |FP64-FMA result - reference| <= bcoef * X + bdelta where
+    // X bounds |input| (xbound, or measured per tile when xbound < 0)
+    double bcoef, bdelta, xbound;
     FixList fix;
     double* tmp;  // MODE_EXACT: (H + kh - 1) * W * cn doubles
 };
 hipError_t launch_sep(const SepLaunch& a, hipStream_t s);
+// The fast separable kernel supports this channel count and kernel size.
+bool sep_fast_fits(int cn, int kw, int kh);
 size_t sep_exact_tmp_bytes(int w, int h, int cn, int kh);
 
 // ---------------------------------------------------------------- box
@@ -39,6 +43,7 @@ hipError_t launch_box(const BoxLaunch& a, hipStream_t s);
 struct F2dLaunch {
     DevImage src, dst;
     const float* kf;      // kw*kh FP32 weights, row-major (device)
+    const double* kd;     // kw*kh FP64 weights, row-major (device)
     const int* taps;      // non-zero taps: ntaps (dx, dy) pairs (device)
     const double* tapw;   // non-zero tap weights as double (device)
     int ntaps;
@@ -46,7 +51,7 @@ struct F2dLaunch {
     int has_zero;         // the kernel has zero weights (they must not take part)
     double delta, bval;
     int mode;
-    double bound;
+    double bcoef, bdelta, xbound;  // MODE_CERTIFY bound: bcoef * X + bdelta
     FixList fix;
     int precision;        // IPFG_PREC_* for the MFMA variants
     const void* mfma_w;   // packed split weights for the MFMA path (device)
@@ -78,7 +83,7 @@ struct BilateralLaunch {
 hipError_t launch_bilateral(const BilateralLaunch& a, hipStream_t s);
 bool bilateral_fast_ok(int dt, int cn, int r, int lut_len);
 
-// Zeroes the fix-up counter with a kernel (usable inside graph capture).
-hipError_t launch_zero_counter(unsigned long long* count, hipStream_t s);
+// Zeroes a stream's fix-up counter and ticket with a kernel.
+hipError_t launch_zero_counter(unsigned long long* count, unsigned int* done, hipStream_t s);
 
 }  // namespace ipfg

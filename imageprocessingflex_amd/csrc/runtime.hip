@@ -252,6 +252,41 @@ private:
 };
 ParamCache g_params;
 
+// ------------------------------------------------------------------ fix-up state
+// Each stream owns the counter, workgroup ticket and report word of its
+// certified launches. They are zero between launches: zeroed by a kernel
+// when the stream is first seen, reset by the last workgroup of each launch.
+struct FixState {
+    unsigned long long* count = nullptr;
+    unsigned int* done = nullptr;
+    unsigned long long* report = nullptr;
+};
+std::mutex g_fix_mu;
+std::unordered_map<hipStream_t, FixState> g_fix;
+
+FixState fix_state(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_fix_mu);
+    auto it = g_fix.find(s);
+    if (it != g_fix.end()) return it->second;
+    void* p = nullptr;
+    HIPCALL(hipMalloc(&p, 32));
+    FixState f;
+    f.count = static_cast<unsigned long long*>(p);
+    f.report = f.count + 1;
+    f.done = reinterpret_cast<unsigned int*>(f.count + 2);
+    HIPCALL(launch_zero_counter(f.count, f.done, s));
+    g_fix.emplace(s, f);
+    return f;
+}
+
+void forget_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_fix_mu);
+    auto it = g_fix.find(s);
+    if (it == g_fix.end()) return;
+    (void)hipFree(it->second.count);
+    g_fix.erase(it);
+}
+
 // ------------------------------------------------------------------ per call
 // Device memory for one call, allocated and freed in stream order, plus the
 // cached parameter blocks it reads.
@@ -275,8 +310,11 @@ public:
         return e->dev;
     }
     FixList fixlist(unsigned long long capacity) {
+        const FixState st = fix_state(s_);
         FixList f;
-        f.count = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long)));
+        f.count = st.count;
+        f.done = st.done;
+        f.report = st.report;
         f.capacity = capacity;
         f.index = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long) * (capacity ? capacity : 1)));
         return f;
@@ -372,41 +410,42 @@ private:
     bool sync_ = false;
 };
 
-// Fix-up list capacity: 1/64 of the samples (at least 64K entries); beyond it
-// the fix-up kernel recomputes the whole image in FP64.
-unsigned long long fix_capacity(const DevImage& d) {
-    const unsigned long long n = static_cast<unsigned long long>(d.w) * d.h * d.cn;
-    const unsigned long long c = n / 64;
+// Fix-up list capacity: 1/8 of the samples (at least 64K entries); beyond it
+// the last workgroup recomputes every sample in FP64.
+unsigned long long fix_capacity(unsigned long long n) {
+    const unsigned long long c = n / 8;
     return c < 65536 ? 65536 : c;
 }
 
-void record_fixups(const FixList& f, hipStream_t s, bool synchronous) {
+void record_fixups(const FixList& f, hipStream_t s) {
     unsigned long long* h = fix_host_slot();
-    HIPCALL(hipMemcpyAsync(h, f.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCALL(hipMemcpyAsync(h, f.report, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     t_ctx.fix_valid = true;
-    (void)synchronous;
 }
 
-constexpr double kU32 = 1.0 / 16777216.0;  // unit roundoff of FP32 (2^-24)
+constexpr double kU32 = 1.0 / 16777216.0;          // unit roundoff of FP32 (2^-24)
+constexpr double kU64 = 1.0 / 9007199254740992.0;  // unit roundoff of FP64 (2^-53)
 
-// Largest magnitude an input sample (or the Constant border value) can have.
+// Largest magnitude an input sample (or the Constant border value) can have;
+// -1 for float images (the kernels measure each tile).
 double input_bound(int dt, double bval, int border) {
-    double x = dt == IPFG_U8 ? 255.0 : dt == IPFG_U16 ? 65535.0 : dt == IPFG_I16 ? 32768.0 : 0.0;
+    if (dt == IPFG_F32 || dt == IPFG_F64) return -1.0;
+    double x = dt == IPFG_U8 ? 255.0 : dt == IPFG_U16 ? 65535.0 : dt == IPFG_I16 ? 32768.0 : 2147483648.0;
     if (border == IPFG_BORDER_CONSTANT) x = std::max(x, std::fabs(bval));
     return x;
 }
 
-// Chooses the kernel family. The FP32 fast path reads u8/u16/i16/f32 images
-// exactly; certification is used for u8 inputs with integer outputs.
+// Kernel family of a linear filter call. DEFAULT accumulates in FP64 and
+// certifies every integer or float32 output (bit-identical to the
+// reference); FP32 trades that for FP32 arithmetic; EXACT and the cases the
+// fast kernels do not read (int32/double images, double outputs) run the
+// reference's operation order.
 int resolve_mode(int precision, int tin, int tout) {
     if (precision == IPFG_PREC_EXACT) return MODE_EXACT;
     const bool fast_in = tin == IPFG_U8 || tin == IPFG_U16 || tin == IPFG_I16 || tin == IPFG_F32;
-    if (!fast_in || tout == IPFG_F64) return MODE_EXACT;
-    if (dtype_is_int(tout)) {
-        if (precision == IPFG_PREC_FP32) return MODE_FAST;
-        return tin == IPFG_U8 ? MODE_CERTIFY : MODE_EXACT;
-    }
-    return MODE_FAST;  // float output
+    if (!fast_in) return MODE_EXACT;
+    if (precision == IPFG_PREC_FP32) return MODE_FAST;
+    return tout == IPFG_F64 ? MODE_EXACT : MODE_CERTIFY;
 }
 
 // gaussianKernel1D (filter.hpp:129-145), host FP64 with the platform exp.
@@ -451,10 +490,14 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
     a.delta = delta;
     a.bval = bval;
     a.mode = resolve_mode(precision, src->dtype, dst->dtype);
-    // Parameter block: kx, ky as double, then as float.
-    std::vector<char> blob(sizeof(double) * (kw + kh) + sizeof(float) * (kw + kh));
+    if (a.mode != MODE_EXACT && !sep_fast_fits(src->channels, kw, kh)) a.mode = MODE_EXACT;
+    // Parameter block: kx and ky as double, then as float; every array is
+    // followed by kWeightPad zeros (the kernels prefetch weights ahead).
+    constexpr int kWeightPad = 16;
+    const int nx = kw + kWeightPad, ny = kh + kWeightPad;
+    std::vector<char> blob((sizeof(double) + sizeof(float)) * (nx + ny), 0);
     double* bd = reinterpret_cast<double*>(blob.data());
-    float* bf = reinterpret_cast<float*>(bd + kw + kh);
+    float* bf = reinterpret_cast<float*>(bd + nx + ny);
     double sx = 0, sy = 0;
     for (int i = 0; i < kw; ++i) {
         bd[i] = kx[i];
@@ -462,25 +505,30 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         sx += std::fabs(kx[i]);
     }
     for (int j = 0; j < kh; ++j) {
-        bd[kw + j] = ky[j];
-        bf[kw + j] = static_cast<float>(ky[j]);
+        bd[nx + j] = ky[j];
+        bf[nx + j] = static_cast<float>(ky[j]);
         sy += std::fabs(ky[j]);
     }
     char* dev = static_cast<char*>(sc.params(blob));
     a.kxd = reinterpret_cast<const double*>(dev);
-    a.kyd = a.kxd + kw;
-    a.kxf = reinterpret_cast<const float*>(a.kyd + kh);
-    a.kyf = a.kxf + kw;
+    a.kyd = a.kxd + nx;
+    a.kxf = reinterpret_cast<const float*>(a.kxd + nx + ny);
+    a.kyf = a.kxf + nx;
     if (a.mode == MODE_CERTIFY) {
-        const double X = input_bound(src->dtype, bval, border);
-        const double S = sx * sy * X;
-        a.bound = 1.01 * ((kw + kh + 5) * kU32 * S + (kh + 2) * kU32 * std::fabs(delta)) +
-                  1e-9 * (1.0 + S + std::fabs(delta));
-        a.fix = sc.fixlist(fix_capacity(a.dst));
+        // Both the FP64 FMA chains and the reference's multiply/add chains are
+        // within (kw + kh + 1) u * sum|ky| sum|kx| X (+ (kh + 1) u |delta|) of
+        // the exact value; a Constant value staged as float adds its own
+        // rounding. Factors are padded.
+        a.bcoef = 2.02 * (kw + kh + 4) * kU64 * sx * sy;
+        a.bdelta = 2.02 * (kh + 2) * kU64 * std::fabs(delta) + 1e-300;
+        if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
+            a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sx * sy;
+        a.xbound = input_bound(src->dtype, bval, border);
+        a.fix = sc.fixlist(fix_capacity(static_cast<unsigned long long>(a.dst.w) * a.dst.h * a.dst.cn));
     }
     if (a.mode == MODE_EXACT) a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
     HIPCALL(launch_sep(a, s));
-    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s, io.synchronous());
+    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
     io.finish();
 }
@@ -562,20 +610,36 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
     a.ntaps = static_cast<int>(tw.size());
     a.has_zero = a.ntaps < kw * kh;
     const size_t nk = static_cast<size_t>(kw) * kh;
-    std::vector<char> blob(sizeof(double) * tw.size() + sizeof(float) * nk + sizeof(int) * taps.size());
-    std::memcpy(blob.data(), tw.data(), sizeof(double) * tw.size());
-    std::memcpy(blob.data() + sizeof(double) * tw.size(), kernel, sizeof(float) * nk);
-    std::memcpy(blob.data() + sizeof(double) * tw.size() + sizeof(float) * nk, taps.data(), sizeof(int) * taps.size());
+    // Parameter block: non-zero tap weights (double), dense weights as double
+    // and float, then the non-zero tap offsets.
+    std::vector<char> blob(sizeof(double) * (tw.size() + nk) + sizeof(float) * nk + sizeof(int) * taps.size());
+    char* w = blob.data();
+    std::memcpy(w, tw.data(), sizeof(double) * tw.size());
+    double* kd = reinterpret_cast<double*>(w + sizeof(double) * tw.size());
+    double sk = 0.0;
+    for (size_t i = 0; i < nk; ++i) {
+        kd[i] = static_cast<double>(kernel[i]);
+        sk += std::fabs(kd[i]);
+    }
+    std::memcpy(w + sizeof(double) * (tw.size() + nk), kernel, sizeof(float) * nk);
+    std::memcpy(w + sizeof(double) * (tw.size() + nk) + sizeof(float) * nk, taps.data(), sizeof(int) * taps.size());
     char* dev = static_cast<char*>(sc.params(blob));
     a.tapw = reinterpret_cast<const double*>(dev);
-    a.kf = reinterpret_cast<const float*>(dev + sizeof(double) * tw.size());
-    a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * tw.size() + sizeof(float) * nk);
+    a.kd = reinterpret_cast<const double*>(dev + sizeof(double) * tw.size());
+    a.kf = reinterpret_cast<const float*>(dev + sizeof(double) * (tw.size() + nk));
+    a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * (tw.size() + nk) + sizeof(float) * nk);
     if (a.mode == MODE_CERTIFY) {
-        const unsigned long long n = static_cast<unsigned long long>(a.dst.w) * a.dst.h * a.dst.cn;
-        a.fix = sc.fixlist(n / 8 < 65536 ? 65536 : n / 8);
+        // FP64 FMA chain and the reference's multiply/add chain each within
+        // (taps + 1) u (sum|k| X + |delta|) of the exact sum.
+        a.bcoef = 2.02 * (a.ntaps + 3) * kU64 * sk;
+        a.bdelta = 2.02 * (a.ntaps + 3) * kU64 * std::fabs(delta) + 1e-300;
+        if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
+            a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sk;
+        a.xbound = input_bound(src->dtype, bval, border);
+        a.fix = sc.fixlist(fix_capacity(static_cast<unsigned long long>(a.dst.w) * a.dst.h * a.dst.cn));
     }
     HIPCALL(launch_filter2d(a, s));
-    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s, io.synchronous());
+    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
     io.finish();
 }
@@ -679,15 +743,15 @@ void bilateral_impl(const ipfg_image* src, const ipfg_image* dst, int d, double 
     a.lutf = nl ? a.spacef + n : nullptr;
     a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * (n + nl) + sizeof(float) * (n + nl));
     if (a.mode == MODE_CERTIFY) {
-        // |fp32 - FP64 reference| <= X * (2 ntaps + 10) u: weights (3 roundings),
-        // two sums of ntaps positive terms and one division.
+        // The FP64 FMA evaluation and the reference differ only in rounding:
+        // per side, one product per weight, two sums of ntaps positive terms
+        // and a division, i.e. (2 ntaps + 4) u relative to a result <= X.
         const double X = 255.0;
-        a.bound = 1.01 * (2.0 * a.ntaps + 10.0) * kU32 * X + 1e-9 * X;
-        const unsigned long long np = static_cast<unsigned long long>(a.dst.w) * a.dst.h;
-        a.fix = sc.fixlist(np / 16 < 65536 ? 65536 : np / 16);
+        a.bound = 2.02 * (2.0 * a.ntaps + 6.0) * kU64 * X + 1e-300;
+        a.fix = sc.fixlist(fix_capacity(static_cast<unsigned long long>(a.dst.w) * a.dst.h));
     }
     HIPCALL(launch_bilateral(a, s));
-    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s, io.synchronous());
+    if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
     io.finish();
 }
@@ -774,7 +838,10 @@ int ipfg_stream_create(ipfg_stream* stream) {
 int ipfg_stream_destroy(ipfg_stream stream) {
     return guarded([&] {
         ensure_init();
-        if (stream) HIPCALL(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+        if (!stream) return;
+        HIPCALL(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+        forget_stream(static_cast<hipStream_t>(stream));
+        HIPCALL(hipStreamDestroy(static_cast<hipStream_t>(stream)));
     });
 }
 int ipfg_stream_synchronize(ipfg_stream stream) {

@@ -1,0 +1,661 @@
+// Separable-filter kernel templates shared by the translation units that
+// instantiate them (sep_u8.hip, sep_f32.hip, sep_generic.hip); see sep.hip for
+// the algorithm.
+#pragma once
+
+#include "kernels.hpp"
+#include "stage.hpp"
+
+namespace ipfg {
+namespace sepk {
+
+constexpr int P = 4;    // horizontal outputs per thread
+constexpr int M = 4;    // vertical outputs per thread
+constexpr int CH = 16;  // rows per step: staged input rows = h rows = output rows
+
+template <int CN>
+constexpr int tile_x() {
+    return CN == 1 ? 64 : 32;
+}
+
+struct SepTile {
+    int XS;  // staged row length in samples: (TX + kw - 1) * cn
+    int XP;  // staged row pitch (floats), multiple of 4
+    int HP;  // ring row pitch (elements)
+    int R;   // ring rows (power of two >= (K0 + 1) * CH)
+    int K0;  // steps before the first output rows: ceil((kh - 1) / CH)
+};
+
+__host__ __device__ inline SepTile sep_tile(int TX, int cn, int kw, int kh) {
+    SepTile t;
+    t.XS = (TX + kw - 1) * cn;
+    t.XP = ((t.XS + cn + P + 3) / 4) * 4;
+    t.HP = TX * cn;  // rows are written and read along the row: no padding needed
+    t.K0 = (kh - 1 + CH - 1) / CH;
+    int r = CH;
+    while (r < (t.K0 + 1) * CH) r *= 2;
+    t.R = r;
+    return t;
+}
+
+// Bytes of the ring, rounded up so the staged rows behind it start 16-byte aligned.
+__host__ __device__ inline size_t sep_ring_bytes(const SepTile& t, size_t acc_size) {
+    return ((static_cast<size_t>(t.R) * t.HP * acc_size + 15) / 16) * 16;
+}
+
+__host__ inline size_t sep_lds_bytes(int TX, int cn, int kw, int kh, size_t acc_size) {
+    const SepTile t = sep_tile(TX, cn, kw, kh);
+    return sep_ring_bytes(t, acc_size) + sizeof(float) * static_cast<size_t>(CH) * t.XP;
+}
+
+template <class Acc>
+__device__ __forceinline__ Acc fma_acc(Acc a, Acc b, Acc c) {
+    if constexpr (sizeof(Acc) == 8) return fma(a, b, c);
+    else return fmaf(a, b, c);
+}
+
+// Loads 4 consecutive weights (the weight arrays carry 16 zeros of padding).
+template <class Acc>
+__device__ __forceinline__ void load4(const Acc* __restrict__ k, int i, Acc (&out)[4]) {
+    out[0] = k[i];
+    out[1] = k[i + 1];
+    out[2] = k[i + 2];
+    out[3] = k[i + 3];
+}
+
+// One output sample in the reference's FP64 operation order
+// (filter.hpp:57-86: padded row -> tmp from 0.0 over i, acc from delta over j).
+__device__ inline double sep_exact_sample(const DevImage& src, int x, int y, int c, const double* kx, int kw,
+                                          const double* ky, int kh, int ax, int ay, int border, double delta,
+                                          double bval) {
+    const size_t es = dtype_size(src.dt);
+    const bool inside_x = x - ax >= 0 && x - ax + kw <= src.w;
+    double acc = delta;
+    for (int j = 0; j < kh; ++j) {
+        const int sy = border_index(y + j - ay, src.h, border);
+        double t = 0.0;
+        if (sy >= 0 && inside_x) {
+            const char* row = static_cast<const char*>(src.data) + static_cast<size_t>(sy) * src.stride +
+                              (static_cast<size_t>(x - ax) * src.cn + c) * es;
+            for (int i = 0; i < kw; ++i) t = t + kx[i] * load_sample(row, static_cast<size_t>(i) * src.cn * es, src.dt);
+        } else {
+            for (int i = 0; i < kw; ++i) {
+                const int sx = border_index(x + i - ax, src.w, border);
+                const double v = (sy < 0 || sx < 0)
+                                     ? bval
+                                     : load_sample(src.data, static_cast<size_t>(sy) * src.stride +
+                                                                 (static_cast<size_t>(sx) * src.cn + c) * es, src.dt);
+                t = t + kx[i] * v;
+            }
+        }
+        acc = acc + ky[j] * t;
+    }
+    return acc;
+}
+
+struct SepArgs {
+    DevImage src, dst;
+    const void* kx;      // kw weights of type Acc, zero-padded
+    const void* ky;      // kh weights of type Acc, zero-padded
+    const double* kxd;   // FP64 weights for exact recomputation
+    const double* kyd;
+    int kw, kh, ax, ay, border;
+    double delta, bval;
+    double bcoef;        // certification bound = bcoef * X + bdelta, X = max |input| of the tile
+    double bdelta;
+    double xbound;       // fixed X for integer images (< 0: measure the tile)
+    int SH;
+    FixList fix;
+};
+
+template <class Ti, class Acc, int CN, int SJ, bool CERT>
+__global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* __restrict__ kx,
+                                                          const Acc* __restrict__ ky) {
+    extern __shared__ __align__(16) unsigned char sep_smem[];
+    constexpr int TX = tile_x<CN>();
+    constexpr int G = TX / P;
+    const int W = a.src.w, H = a.src.h;
+    const int cn = CN > 0 ? CN : a.src.cn;
+    const int kw = a.kw, kh = a.kh;
+    const SepTile T = sep_tile(TX, cn, kw, kh);
+    Acc* ring = reinterpret_cast<Acc*>(sep_smem);
+    float* in_buf = reinterpret_cast<float*>(sep_smem + sep_ring_bytes(T, sizeof(Acc)));
+    __shared__ unsigned int tile_max_bits;  // max |staged value| as float bits
+    const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * a.SH;
+    const int rows_out = min(a.SH, H - y0);
+    const int nh = rows_out + kh - 1;
+    const int nhstep = (nh + CH - 1) / CH;
+    const int nstep = T.K0 + (rows_out + CH - 1) / CH;
+    const float bvalf = static_cast<float>(a.bval);
+    const Ti bv = static_cast<Ti>(bvalf);
+    const bool patch_bval = a.border == IPFG_BORDER_CONSTANT && static_cast<float>(bv) != bvalf;
+    const bool measure = CERT && a.xbound < 0;
+    if (tid == 0) tile_max_bits = 0u;
+    float my_max = 0.f;
+    Stager<Ti, SJ> st;
+    auto conv = [&](Ti v) {
+        const float f = static_cast<float>(v);
+        if (measure) my_max = fmaxf(my_max, (f == f) ? fabsf(f) : INFINITY);
+        return f;
+    };
+    const int ncol = TX * cn;
+    const int gx0 = x0 - a.ax, gy0 = y0 - a.ay;
+
+    st.load(src, a.src.stride, W, H, cn, a.border, bv, gx0, gy0, min(CH, nh), T.XS, tid);
+    for (int k = 0; k < nstep; ++k) {
+        const int r0 = k * CH;
+        const int nr = min(CH, nh - r0);
+        if (nr > 0) st.store(in_buf, T.XP, nr, T.XS, tid, conv);
+        if (nr > 0 && patch_bval) {  // a Constant value that Ti cannot hold
+            for (int e = tid; e < nr * T.XS; e += kBlock) {
+                const int r = e / T.XS, s = e - r * T.XS;
+                const int gy = gy0 + r0 + r, gx = gx0 + s / cn;
+                if (gy < 0 || gy >= H || gx < 0 || gx >= W) in_buf[r * T.XP + s] = bvalf;
+            }
+        }
+        if (measure) {
+            if (patch_bval) my_max = fmaxf(my_max, fabsf(bvalf));
+            atomicMax(&tile_max_bits, __float_as_uint(my_max));
+        }
+        __syncthreads();
+        if (k + 1 < nhstep)
+            st.load(src, a.src.stride, W, H, cn, a.border, bv, gx0, gy0 + r0 + CH, min(CH, nh - r0 - CH), T.XS, tid);
+        // ---- horizontal taps (reference order: i ascending from 0) into the ring
+        if (nr > 0) {
+            for (int t = tid; t < CH * G * cn; t += kBlock) {
+                const int cg = t % (G * cn);
+                const int r = t / (G * cn);
+                if (r >= nr) continue;
+                const int g = cg / cn, c = cg - g * cn;
+                const float* base = in_buf + r * T.XP + g * P * cn + c;
+                Acc acc[P], win[P];
+                if (CN == 1) {
+                    const float4 f = *reinterpret_cast<const float4*>(base);
+                    win[0] = f.x;
+                    win[1] = f.y;
+                    win[2] = f.z;
+                    win[3] = f.w;
+                } else {
+#pragma unroll
+                    for (int m = 0; m < P; ++m) win[m] = static_cast<Acc>(base[m * cn]);
+                }
+#pragma unroll
+                for (int m = 0; m < P; ++m) acc[m] = Acc(0);
+                Acc kc[4];
+                load4(kx, 0, kc);
+                int i0 = 0;
+                for (; i0 + P <= kw; i0 += P) {
+                    Acc kn[4];
+                    load4(kx, i0 + P, kn);
+                    Acc nx[P];
+                    if (CN == 1) {
+                        const float4 f = *reinterpret_cast<const float4*>(base + i0 + P);
+                        nx[0] = f.x;
+                        nx[1] = f.y;
+                        nx[2] = f.z;
+                        nx[3] = f.w;
+                    } else {
+#pragma unroll
+                        for (int ii = 0; ii < P; ++ii) nx[ii] = base[(i0 + ii + P) * cn];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < P; ++ii) {
+#pragma unroll
+                        for (int m = 0; m < P; ++m) acc[m] = fma_acc(kc[ii], win[(ii + m) % P], acc[m]);
+                        win[ii] = nx[ii];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < 4; ++ii) kc[ii] = kn[ii];
+                }
+#pragma unroll
+                for (int ii = 0; ii < P; ++ii) {
+                    if (i0 + ii < kw) {
+#pragma unroll
+                        for (int m = 0; m < P; ++m) acc[m] = fma_acc(kc[ii], win[(ii + m) % P], acc[m]);
+                        win[ii] = base[(i0 + ii + P) * cn];
+                    }
+                }
+                Acc* hrow = ring + ((r0 + r) & (T.R - 1)) * T.HP + c;
+                if (CN == 1) {  // P consecutive outputs: 16-byte stores, no bank conflicts
+                    if constexpr (sizeof(Acc) == 8) {
+                        reinterpret_cast<double2*>(hrow + g * P)[0] = make_double2(acc[0], acc[1]);
+                        reinterpret_cast<double2*>(hrow + g * P)[1] = make_double2(acc[2], acc[3]);
+                    } else {
+                        *reinterpret_cast<float4*>(hrow + g * P) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < P; ++m) hrow[(g * P + m) * cn] = acc[m];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- vertical taps for output rows [(k - K0) * CH, (k - K0 + 1) * CH)
+        const int ob = (k - T.K0) * CH;
+        if (ob < 0) continue;
+        double bound = 0.0;
+        if (CERT) bound = a.bcoef * (measure ? static_cast<double>(__uint_as_float(tile_max_bits)) : a.xbound) + a.bdelta;
+        for (int t = tid; t < (CH / M) * ncol; t += kBlock) {
+            const int col = t % ncol;
+            const int v = t / ncol;
+            const int yr = ob + v * M;  // first output row of this task (strip-relative)
+            if (yr >= rows_out) continue;
+            const Acc* colp = ring + col;
+            const int msk = T.R - 1;
+            Acc acc[M], win[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                acc[m] = static_cast<Acc>(a.delta);
+                win[m] = colp[((yr + m) & msk) * T.HP];
+            }
+            Acc kc[M];
+            load4(ky, 0, kc);
+            int j0 = 0;
+            for (; j0 + M <= kh; j0 += M) {
+                Acc kn[M];
+                load4(ky, j0 + M, kn);
+                Acc nx[M];
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) nx[jj] = colp[((yr + j0 + jj + M) & msk) * T.HP];
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
+                    win[jj] = nx[jj];
+                }
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) kc[jj] = kn[jj];
+            }
+#pragma unroll
+            for (int jj = 0; jj < M; ++jj) {
+                if (j0 + jj < kh) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
+                    win[jj] = colp[((yr + j0 + jj + M) & msk) * T.HP];
+                }
+            }
+            const int px = col / cn, c = col - px * cn;
+            const int x = x0 + px;
+            if (x >= W) continue;
+            const size_t q = static_cast<size_t>(x) * cn + c;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (yr + m >= rows_out) break;
+                const int y = y0 + yr + m;
+                char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
+                const double r = static_cast<double>(acc[m]);
+                bool doubt = false;
+                switch (a.dst.dt) {
+                    case IPFG_U8:
+                        reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(r);
+                        if (CERT) doubt = needs_exact(r, bound, 0.0, 255.0);
+                        break;
+                    case IPFG_U16:
+                        reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(r);
+                        if (CERT) doubt = needs_exact(r, bound, 0.0, 65535.0);
+                        break;
+                    case IPFG_I16:
+                        reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(r);
+                        if (CERT) doubt = needs_exact(r, bound, -32768.0, 32767.0);
+                        break;
+                    case IPFG_I32:
+                        reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(r);
+                        if (CERT) doubt = needs_exact(r, bound, -2147483648.0, 2147483647.0);
+                        break;
+                    case IPFG_F32:
+                        reinterpret_cast<float*>(drow)[q] = static_cast<float>(r);
+                        if (CERT) doubt = needs_exact_f32(r, bound);
+                        break;
+                    default:
+                        reinterpret_cast<double*>(drow)[q] = r;
+                        break;
+                }
+                if (CERT && doubt) fix_push(a.fix, static_cast<unsigned long long>(y) * W * cn + q);
+            }
+        }
+    }
+    if (CERT) {
+        const size_t rowLen = static_cast<size_t>(W) * cn;
+        const size_t es = dtype_size(a.dst.dt);
+        fix_epilogue(a.fix, static_cast<unsigned long long>(H) * rowLen, [&](unsigned long long s) {
+            const int y = static_cast<int>(s / rowLen);
+            const size_t q = s - static_cast<size_t>(y) * rowLen;
+            const int x = static_cast<int>(q / cn), c = static_cast<int>(q - static_cast<size_t>(x) * cn);
+            const double v = sep_exact_sample(a.src, x, y, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border, a.delta, a.bval);
+            store_sample(a.dst.data, static_cast<size_t>(y) * a.dst.stride + q * es, a.dst.dt, v);
+        });
+    }
+}
+
+// ------------------------------------------------------------------ planar kernel
+// Channel counts 1, 3 and 4: staged rows are de-interleaved into one LDS plane
+// per channel, so every plane is a contiguous row of pixels and the
+// horizontal pass reads its windows with 16-byte LDS loads and writes four
+// results with one 16-byte store. Ring rows are addressed once per group of
+// M rows (groups never straddle the ring's wrap) with immediate offsets.
+constexpr int PTX = 64;  // tile width in pixels (per plane)
+
+struct PlaneTile {
+    int XS;     // staged samples per row: (PTX + kw - 1) * cn
+    int XPP;    // staged plane row pitch (floats), multiple of 4, >= PTX + kw - 1 + P
+    int R;      // ring rows per plane (power of two)
+    int K0;     // steps before the first output rows
+    int RPL;    // ring plane size (elements): R * PTX + pad
+};
+
+__host__ __device__ inline PlaneTile plane_tile(int cn, int kw, int kh) {
+    PlaneTile t;
+    t.XS = (PTX + kw - 1) * cn;
+    t.XPP = ((PTX + kw - 1 + P + 3) / 4) * 4;
+    t.K0 = (kh - 1 + CH - 1) / CH;
+    int r = CH;
+    while (r < (t.K0 + 1) * CH) r *= 2;
+    t.R = r;
+    t.RPL = r * PTX + 4;
+    return t;
+}
+
+__host__ inline size_t plane_lds_bytes(int cn, int kw, int kh, size_t acc_size) {
+    const PlaneTile t = plane_tile(cn, kw, kh);
+    const size_t ring = ((static_cast<size_t>(cn) * t.RPL * acc_size + 15) / 16) * 16;
+    return ring + sizeof(float) * static_cast<size_t>(cn) * CH * t.XPP;
+}
+
+template <class To>
+__device__ __forceinline__ void put(char* p, double r) {
+    if constexpr (std::is_same<To, float>::value) *reinterpret_cast<float*>(p) = static_cast<float>(r);
+    else if constexpr (std::is_same<To, double>::value) *reinterpret_cast<double*>(p) = r;
+    else *reinterpret_cast<To*>(p) = saturate_from_double<To>(r);
+}
+
+template <class To>
+__device__ __forceinline__ bool doubtful(double r, double bound) {
+    if constexpr (std::is_same<To, float>::value) return needs_exact_f32(r, bound);
+    else if constexpr (std::is_same<To, double>::value) return false;
+    else
+        return needs_exact(r, bound, static_cast<double>(std::numeric_limits<To>::lowest()),
+                           static_cast<double>(std::numeric_limits<To>::max()));
+}
+
+template <class Ti, class To, class Acc, int CN, int SJ, bool CERT>
+__global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc* __restrict__ kx,
+                                                           const Acc* __restrict__ ky) {
+    extern __shared__ __align__(16) unsigned char sep_smem[];
+    constexpr int G = PTX / P;
+    const int W = a.src.w, H = a.src.h;
+    const int kw = a.kw, kh = a.kh;
+    const PlaneTile T = plane_tile(CN, kw, kh);
+    Acc* ring = reinterpret_cast<Acc*>(sep_smem);
+    float* in_buf = reinterpret_cast<float*>(
+        sep_smem + ((static_cast<size_t>(CN) * T.RPL * sizeof(Acc) + 15) / 16) * 16);
+    __shared__ unsigned int tile_max_bits;
+    const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * a.SH;
+    const int rows_out = min(a.SH, H - y0);
+    const int nh = rows_out + kh - 1;
+    const int nhstep = (nh + CH - 1) / CH;
+    const int nstep = T.K0 + (rows_out + CH - 1) / CH;
+    const float bvalf = static_cast<float>(a.bval);
+    const Ti bv = static_cast<Ti>(bvalf);
+    const bool patch_bval = a.border == IPFG_BORDER_CONSTANT && static_cast<float>(bv) != bvalf;
+    const bool measure = CERT && a.xbound < 0;
+    if (tid == 0) tile_max_bits = 0u;
+    float my_max = 0.f;
+    Stager<Ti, SJ> st;
+    const int gx0 = x0 - a.ax, gy0 = y0 - a.ay;
+    const int plane_in = CH * T.XPP;
+    const int msk = T.R - 1;
+
+    st.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0, min(CH, nh), T.XS, tid);
+    for (int k = 0; k < nstep; ++k) {
+        const int r0 = k * CH;
+        const int nr = min(CH, nh - r0);
+        if (nr > 0) {
+            // de-interleave: sample s of staged row r -> plane s % CN, pixel s / CN
+            const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = wave + 4 * i;
+                if (r >= nr) continue;
+#pragma unroll
+                for (int j = 0; j < SJ; ++j) {
+                    const int s = lane + 64 * j;
+                    if (s < T.XS) {
+                        const int px = s / CN, c = s - px * CN;
+                        const float f = static_cast<float>(st.val[i][j]);
+                        if (measure) my_max = fmaxf(my_max, (f == f) ? fabsf(f) : INFINITY);
+                        in_buf[c * plane_in + r * T.XPP + px] = f;
+                    }
+                }
+            }
+            if (patch_bval) {  // a Constant value that Ti cannot hold
+                for (int e = tid; e < nr * T.XS; e += kBlock) {
+                    const int r = e / T.XS, s = e - r * T.XS;
+                    const int px = s / CN, c = s - px * CN;
+                    const int gy = gy0 + r0 + r, gx = gx0 + px;
+                    if (gy < 0 || gy >= H || gx < 0 || gx >= W) in_buf[c * plane_in + r * T.XPP + px] = bvalf;
+                }
+                if (measure) my_max = fmaxf(my_max, fabsf(bvalf));
+            }
+            if (measure) atomicMax(&tile_max_bits, __float_as_uint(my_max));
+        }
+        __syncthreads();
+        if (k + 1 < nhstep)
+            st.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0 + r0 + CH, min(CH, nh - r0 - CH), T.XS, tid);
+        // ---- horizontal taps (reference order: i ascending from 0) into the ring
+        if (nr > 0) {
+            for (int t = tid; t < CH * G * CN; t += kBlock) {
+                const int g = t % G;
+                const int rest = t / G;
+                const int c = rest % CN;
+                const int r = rest / CN;
+                if (r >= nr) continue;
+                const float* base = in_buf + c * plane_in + r * T.XPP + g * P;
+                Acc acc[P], win[P];
+                {
+                    const float4 f = *reinterpret_cast<const float4*>(base);
+                    win[0] = f.x;
+                    win[1] = f.y;
+                    win[2] = f.z;
+                    win[3] = f.w;
+                }
+#pragma unroll
+                for (int m = 0; m < P; ++m) acc[m] = Acc(0);
+                Acc kc[4];
+                load4(kx, 0, kc);
+                int i0 = 0;
+                for (; i0 + P <= kw; i0 += P) {
+                    Acc kn[4];
+                    load4(kx, i0 + P, kn);
+                    const float4 f = *reinterpret_cast<const float4*>(base + i0 + P);
+                    const Acc nx[P] = {static_cast<Acc>(f.x), static_cast<Acc>(f.y), static_cast<Acc>(f.z),
+                                       static_cast<Acc>(f.w)};
+#pragma unroll
+                    for (int ii = 0; ii < P; ++ii) {
+#pragma unroll
+                        for (int m = 0; m < P; ++m) acc[m] = fma_acc(kc[ii], win[(ii + m) % P], acc[m]);
+                        win[ii] = nx[ii];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < 4; ++ii) kc[ii] = kn[ii];
+                }
+                if (i0 < kw) {
+                    const float4 f = *reinterpret_cast<const float4*>(base + i0 + P);
+                    const Acc nx[P] = {static_cast<Acc>(f.x), static_cast<Acc>(f.y), static_cast<Acc>(f.z),
+                                       static_cast<Acc>(f.w)};
+#pragma unroll
+                    for (int ii = 0; ii < P; ++ii) {
+                        if (i0 + ii < kw) {
+#pragma unroll
+                            for (int m = 0; m < P; ++m) acc[m] = fma_acc(kc[ii], win[(ii + m) % P], acc[m]);
+                            win[ii] = nx[ii];
+                        }
+                    }
+                }
+                Acc* hrow = ring + c * T.RPL + ((r0 + r) & msk) * PTX + g * P;
+                if constexpr (sizeof(Acc) == 8) {
+                    reinterpret_cast<double2*>(hrow)[0] = make_double2(acc[0], acc[1]);
+                    reinterpret_cast<double2*>(hrow)[1] = make_double2(acc[2], acc[3]);
+                } else {
+                    *reinterpret_cast<float4*>(hrow) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- vertical taps for output rows [(k - K0) * CH, (k - K0 + 1) * CH)
+        const int ob = (k - T.K0) * CH;
+        if (ob < 0) continue;
+        double bound = 0.0;
+        if (CERT) bound = a.bcoef * (measure ? static_cast<double>(__uint_as_float(tile_max_bits)) : a.xbound) + a.bdelta;
+        for (int t = tid; t < (CH / M) * CN * PTX; t += kBlock) {
+            const int x = t % PTX;
+            const int rest = t / PTX;
+            const int c = rest % CN;
+            const int v = rest / CN;
+            const int yr = ob + v * M;  // first output row of this task (strip-relative), multiple of M
+            if (yr >= rows_out) continue;
+            const Acc* plane = ring + c * T.RPL + x;
+            Acc acc[M], win[M];
+            {
+                const Acc* g0 = plane + (yr & msk) * PTX;  // rows yr .. yr + M - 1 do not wrap
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    acc[m] = static_cast<Acc>(a.delta);
+                    win[m] = g0[m * PTX];
+                }
+            }
+            Acc kc[M];
+            load4(ky, 0, kc);
+            int j0 = 0;
+            for (; j0 + M <= kh; j0 += M) {
+                Acc kn[M];
+                load4(ky, j0 + M, kn);
+                const Acc* gp = plane + ((yr + j0 + M) & msk) * PTX;
+                Acc nx[M];
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) nx[jj] = gp[jj * PTX];
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
+                    win[jj] = nx[jj];
+                }
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) kc[jj] = kn[jj];
+            }
+            if (j0 < kh) {
+                const Acc* gp = plane + ((yr + j0 + M) & msk) * PTX;
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) {
+                    if (j0 + jj < kh) {
+#pragma unroll
+                        for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
+                        win[jj] = gp[jj * PTX];
+                    }
+                }
+            }
+            const int xg = x0 + x;
+            if (xg >= W) continue;
+            const size_t q = static_cast<size_t>(xg) * CN + c;
+            char* op = static_cast<char*>(a.dst.data) + static_cast<size_t>(y0 + yr) * a.dst.stride + q * sizeof(To);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (yr + m >= rows_out) break;
+                const double rv = static_cast<double>(acc[m]);
+                put<To>(op, rv);
+                if (CERT && doubtful<To>(rv, bound))
+                    fix_push(a.fix, static_cast<unsigned long long>(y0 + yr + m) * W * CN + q);
+                op += a.dst.stride;
+            }
+        }
+    }
+    if (CERT) {
+        const size_t rowLen = static_cast<size_t>(W) * CN;
+        fix_epilogue(a.fix, static_cast<unsigned long long>(H) * rowLen, [&](unsigned long long s) {
+            const int y = static_cast<int>(s / rowLen);
+            const size_t q = s - static_cast<size_t>(y) * rowLen;
+            const int x = static_cast<int>(q / CN), c = static_cast<int>(q - static_cast<size_t>(x) * CN);
+            const double v = sep_exact_sample(a.src, x, y, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border, a.delta, a.bval);
+            put<To>(static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride + q * sizeof(To), v);
+        });
+    }
+}
+
+template <class Ti, class Acc, int CN, int SJ, bool CERT>
+hipError_t sep_fast_launch_v(const SepArgs& a, size_t lds, hipStream_t s) {
+    constexpr int TX = tile_x<CN>();
+    auto kern = sep_fast_kernel<Ti, Acc, CN, SJ, CERT>;
+    if (lds > 64 * 1024) {
+        static size_t raised = 0;  // largest dynamic LDS granted to this instantiation
+        if (lds > raised) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+            raised = lds;
+        }
+    }
+    const dim3 grid(static_cast<unsigned>((a.src.w + TX - 1) / TX), static_cast<unsigned>((a.src.h + a.SH - 1) / a.SH));
+    hipLaunchKernelGGL(kern, grid, dim3(kBlock), lds, s, a, static_cast<const Acc*>(a.kx),
+                       static_cast<const Acc*>(a.ky));
+    return hipGetLastError();
+}
+
+template <class Ti, class Acc, int CN, bool CERT>
+hipError_t sep_fast_launch(SepArgs a, hipStream_t s) {
+    constexpr int TX = tile_x<CN>();
+    const int cn = a.src.cn;
+    const size_t lds = sep_lds_bytes(TX, cn, a.kw, a.kh, sizeof(Acc));
+    const int xs = (TX + a.kw - 1) * cn;  // staged samples per row
+    if (lds > 156 * 1024 || xs > 64 * 8) return hipErrorInvalidConfiguration;
+    // Strip height: long strips amortise the kh - 1 history rows, but the
+    // grid must still cover the 256 CUs several times over.
+    const long long gx = (a.src.w + TX - 1) / TX;
+    int SH = 512;
+    while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
+    a.SH = SH;
+    if (xs <= 128) return sep_fast_launch_v<Ti, Acc, CN, 2, CERT>(a, lds, s);
+    return sep_fast_launch_v<Ti, Acc, CN, 8, CERT>(a, lds, s);
+}
+
+template <class Ti, class To, class Acc, int CN, int SJ, bool CERT>
+hipError_t plane_launch_v(const SepArgs& a, size_t lds, hipStream_t s) {
+    auto kern = sep_plane_kernel<Ti, To, Acc, CN, SJ, CERT>;
+    if (lds > 64 * 1024) {
+        static size_t raised = 0;
+        if (lds > raised) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+            raised = lds;
+        }
+    }
+    const dim3 grid(static_cast<unsigned>((a.src.w + PTX - 1) / PTX), static_cast<unsigned>((a.src.h + a.SH - 1) / a.SH));
+    hipLaunchKernelGGL(kern, grid, dim3(kBlock), lds, s, a, static_cast<const Acc*>(a.kx),
+                       static_cast<const Acc*>(a.ky));
+    return hipGetLastError();
+}
+
+template <class Ti, class To, class Acc, int CN, bool CERT>
+hipError_t plane_launch(SepArgs a, hipStream_t s) {
+    const size_t lds = plane_lds_bytes(CN, a.kw, a.kh, sizeof(Acc));
+    const int xs = (PTX + a.kw - 1) * CN;
+    if (lds > 156 * 1024 || xs > 64 * 8) return hipErrorInvalidConfiguration;
+    const long long gx = (a.src.w + PTX - 1) / PTX;
+    int SH = 512;
+    while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
+    a.SH = SH;
+    if (xs <= 128) return plane_launch_v<Ti, To, Acc, CN, 2, CERT>(a, lds, s);
+    return plane_launch_v<Ti, To, Acc, CN, 8, CERT>(a, lds, s);
+}
+
+
+// Instantiation groups (one per translation unit).
+hipError_t plane_u8(const SepArgs& a, bool cert, hipStream_t s);
+hipError_t plane_f32(const SepArgs& a, bool cert, hipStream_t s);
+hipError_t generic_any(const SepArgs& a, bool cert, hipStream_t s);
+
+}  // namespace sepk
+}  // namespace ipfg
