@@ -26,6 +26,55 @@ __host__ inline size_t bil_lds_bytes(int TH, int r, int lut_len, size_t acc_size
     return sizeof(uint32_t) * static_cast<size_t>(TH + 2 * r) * bil_xp(r) + acc_size * lut_len + 16;
 }
 
+// One pixel, every channel, in the reference's FP64 order (filter.hpp:366-383:
+// for each tap, w = space * range(L1 distance); wsum += w; acc[c] += w * p[c];
+// out = acc / wsum; Constant border skips taps), computed by a whole wave:
+// each lane forms the weights and products of 64 taps at a time and the sums
+// run over broadcast values. All 64 lanes must call it; all return the results.
+template <int CN>
+__device__ inline void bil_exact_wave(const DevImage& src, int x, int y, const int* __restrict__ taps,
+                                      const double* __restrict__ spaced, int ntaps, const double* __restrict__ lutd,
+                                      int lut_len, int border, double cc, double (&out)[CN]) {
+    const int lane = lane_id();
+    const uint8_t* p0 = static_cast<const uint8_t*>(src.data) + static_cast<size_t>(y) * src.stride + static_cast<size_t>(x) * CN;
+    double wsum = 0.0, acc[CN];
+#pragma unroll
+    for (int c = 0; c < CN; ++c) acc[c] = 0.0;
+    for (int k0 = 0; k0 < ntaps; k0 += 64) {
+        const int k = k0 + lane;
+        bool valid = false;
+        double w = 0.0, wp[CN];
+#pragma unroll
+        for (int c = 0; c < CN; ++c) wp[c] = 0.0;
+        if (k < ntaps) {
+            const int sx = border_index(x + taps[2 * k], src.w, border);
+            const int sy = border_index(y + taps[2 * k + 1], src.h, border);
+            if (sx >= 0 && sy >= 0) {
+                valid = true;
+                const uint8_t* p = static_cast<const uint8_t*>(src.data) + static_cast<size_t>(sy) * src.stride +
+                                   static_cast<size_t>(sx) * CN;
+                double dist = 0.0;
+#pragma unroll
+                for (int c = 0; c < CN; ++c) dist = dist + fabs(static_cast<double>(p[c]) - static_cast<double>(p0[c]));
+                const double rw = dist < static_cast<double>(lut_len) ? lutd[static_cast<size_t>(dist)] : exp(dist * dist * cc);
+                w = spaced[k] * rw;
+#pragma unroll
+                for (int c = 0; c < CN; ++c) wp[c] = w * static_cast<double>(p[c]);
+            }
+        }
+        const unsigned long long vb = __ballot(valid);
+        const int n = min(64, ntaps - k0);
+        for (int kk = 0; kk < n; ++kk) {
+            if (!((vb >> kk) & 1ull)) continue;
+            wsum = wsum + bcast_d(w, kk);
+#pragma unroll
+            for (int c = 0; c < CN; ++c) acc[c] = acc[c] + bcast_d(wp[c], kk);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CN; ++c) out[c] = acc[c] / wsum;
+}
+
 __device__ inline double bil_exact_channel(const DevImage& src, int x, int y, int c, const int* taps,
                                            const double* spaced, int ntaps, const double* lutd, int lut_len, int r,
                                            int border, double cc);
@@ -80,14 +129,14 @@ __global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
     for (int t = tid; t < TX * TH; t += kBlock) {
         const int lx = t % TX, ly = t / TX;
         const int x = x0 + lx, y = y0 + ly;
-        if (x >= W || y >= H) continue;
+        const bool act = x < W && y < H;
         const uint32_t* cen = tile + (ly + r) * XP + (lx + r);
         const uint32_t p0 = *cen;
         Acc wsum = Acc(0);
         Acc acc[CN];
 #pragma unroll
         for (int c = 0; c < CN; ++c) acc[c] = Acc(0);
-        for (int k = 0; k < a.ntaps; ++k) {
+        for (int k = 0; k < (act ? a.ntaps : 0); ++k) {
             const int dx = a.taps[2 * k], dy = a.taps[2 * k + 1];
             if (CONST_BORDER) {
                 if (static_cast<unsigned>(x + dx) >= static_cast<unsigned>(W) ||
@@ -105,26 +154,33 @@ __global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
                 else acc[c] = fmaf(w, pc, acc[c]);
             }
         }
-        uint8_t* o = static_cast<uint8_t*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride + static_cast<size_t>(x) * CN;
-        bool doubt = false;
-#pragma unroll
-        for (int c = 0; c < CN; ++c) {
-            const double v = static_cast<double>(acc[c] / wsum);
-            o[c] = saturate_from_double<uint8_t>(v);
-            if (CERT) doubt |= needs_exact(v, a.bound, 0.0, 255.0);
-        }
-        if (CERT && doubt) fix_push(a.fix, static_cast<unsigned long long>(y) * W + x);
-    }
-    if (CERT) {
-        fix_epilogue(a.fix, static_cast<unsigned long long>(W) * H, [&](unsigned long long pix) {
-            const int y = static_cast<int>(pix / W);
-            const int x = static_cast<int>(pix - static_cast<unsigned long long>(y) * W);
+        unsigned int dmask = 0u;
+        if (act) {
             uint8_t* o = static_cast<uint8_t*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride + static_cast<size_t>(x) * CN;
-            for (int c = 0; c < CN; ++c)
-                o[c] = saturate_from_double<uint8_t>(
-                    bil_exact_channel(a.src, x, y, c, a.taps, a.spaced, a.ntaps, a.lutd, a.lut_len, r, a.border, a.cc));
-        });
+            bool doubt = false;
+#pragma unroll
+            for (int c = 0; c < CN; ++c) {
+                const double v = static_cast<double>(acc[c] / wsum);
+                o[c] = saturate_from_double<uint8_t>(v);
+                if (CERT) doubt |= needs_exact(v, a.bound, 0.0, 255.0);
+            }
+            dmask = doubt ? 1u : 0u;
+        }
+        if (CERT) {
+            wave_fix(a.fix, dmask, [&](int l, int) {
+                const int xl = bcast_i(x, l), yl = bcast_i(y, l);
+                double e[CN];
+                bil_exact_wave<CN>(a.src, xl, yl, a.taps, a.spaced, a.ntaps, a.lutd, a.lut_len, a.border, a.cc, e);
+                if (lane_id() == l) {
+                    uint8_t* o = static_cast<uint8_t*>(a.dst.data) + static_cast<size_t>(yl) * a.dst.stride +
+                                 static_cast<size_t>(xl) * CN;
+#pragma unroll
+                    for (int c = 0; c < CN; ++c) o[c] = saturate_from_double<uint8_t>(e[c]);
+                }
+            });
+        }
     }
+    if (CERT) fix_finish(a.fix);
 }
 
 // One pixel, one channel, in the reference's FP64 order. The range weight of

@@ -157,34 +157,74 @@ __device__ inline bool needs_exact_f32(double r, double bound) {
     return fabs(side) + bound >= half;
 }
 
-// Device-side fix-up list of a certified launch. The counter and the
-// finished-workgroup ticket belong to the stream (zero between launches); the
-// last workgroup of the launch recomputes the listed samples exactly and
-// resets both. When more than `capacity` samples are flagged the last
-// workgroup recomputes every sample.
+// ------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane_id() { return static_cast<int>(__lane_id()); }
+
+// Value of v in lane l (l wave-uniform).
+__device__ __forceinline__ int bcast_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double bcast_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), l));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), l));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Maximum of a non-negative float (or +inf) over the wave, added to *m (an
+// LDS word holding float bits) by one lane. Every lane of the wave calls it.
+__device__ __forceinline__ void wave_max_into(unsigned int* m, float v) {
+    unsigned int b = __float_as_uint(v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = static_cast<unsigned int>(__shfl_xor(static_cast<int>(b), off));
+        b = o > b ? o : b;
+    }
+    if (lane_id() == 0) atomicMax(m, b);
+}
+
+// ------------------------------------------------------------------ fix-ups
+// Certified launches recompute doubtful samples in place: after a wave has
+// written a group of outputs, wave_fix() walks the wave's doubtful ones and,
+// for each, runs the exact recomputation on all 64 lanes at once (the
+// reference's sequential sums are split into lane-parallel products or rows
+// and one broadcast chain). Nothing is left for a later pass, so a launch has
+// no serial tail however many samples are flagged.
+//
+// Counting: each wave adds its recomputed samples to a counter that belongs
+// to the stream (zero between launches); the last workgroup to finish copies
+// it to `report` and resets it and the finished-workgroup ticket.
 struct FixList {
     unsigned long long* count;
     unsigned int* done;
-    unsigned long long* report;  // receives the launch's flag count
-    unsigned long long* index;
-    unsigned long long capacity;
+    unsigned long long* report;  // receives the launch's count
 };
 
-// Flags one sample. The index is written with an agent-scope atomic store
-// (coherent across the XCDs' L2 caches), so no device-wide fence is needed.
-__device__ inline void fix_push(const FixList& f, unsigned long long idx) {
-    const unsigned long long slot = atomicAdd(f.count, 1ull);
-    if (slot < f.capacity) __hip_atomic_store(f.index + slot, idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// `mask` bit m set: this lane's output m is doubtful. All 64 lanes of the
+// wave must call this together; fix(l, m) is then called by the whole wave
+// (l and m uniform) for each doubtful output m of lane l, in lane order.
+template <class Fix>
+__device__ __forceinline__ void wave_fix(const FixList& f, unsigned int mask, Fix fix) {
+    unsigned long long bal = __ballot(mask != 0u);
+    if (bal == 0ull) return;
+    unsigned int total = 0;
+    while (bal) {
+        const int l = __builtin_ctzll(bal);
+        bal &= bal - 1ull;
+        unsigned int mm = static_cast<unsigned int>(bcast_i(static_cast<int>(mask), l));
+        total += __builtin_popcount(mm);
+        while (mm) {
+            const int m = __builtin_ctz(mm);
+            mm &= mm - 1u;
+            fix(l, m);
+        }
+    }
+    if (lane_id() == 0) atomicAdd(f.count, static_cast<unsigned long long>(total));
 }
 
-// Called by every thread of every workgroup after its last flag. The last
-// workgroup to arrive calls fix(i) for each flagged index (or every index
-// below `total` on overflow) and leaves the stream's counters at zero.
-// Ordering: each wave waits for its own stores to complete before the
-// workgroup takes a ticket, and the last workgroup reads the list with
-// agent-scope atomic loads -- no L2 write-back or invalidate is issued.
-template <class Fix>
-__device__ inline void fix_epilogue(const FixList& f, unsigned long long total, Fix fix) {
+// Called by every thread of every workgroup of a certified launch after its
+// last wave_fix(). Each wave waits for its own memory operations before the
+// workgroup takes a ticket; the last workgroup reads the counter with an
+// agent-scope atomic load (no L2 write-back or invalidate is issued).
+__device__ inline void fix_finish(const FixList& f) {
     __shared__ int last;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -193,24 +233,11 @@ __device__ inline void fix_epilogue(const FixList& f, unsigned long long total, 
         last = atomicAdd(f.done, 1u) == nblk - 1;
     }
     __syncthreads();
-    if (!last) return;
-    const unsigned long long n = __hip_atomic_load(f.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (threadIdx.x == 0) *f.report = n;
-    if (n <= f.capacity) {
-        for (unsigned long long k = threadIdx.x; k < n; k += blockDim.x)
-            fix(__hip_atomic_load(f.index + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    } else {
-        for (unsigned long long k = threadIdx.x; k < total; k += blockDim.x) fix(k);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(f.count, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(f.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (!last || threadIdx.x != 0) return;
+    *f.report = __hip_atomic_load(f.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f.count, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-
-// Grid used by fix-up kernels (grid-stride loops over the list or the image).
-constexpr int kFixBlocks = 1024;
 
 // ------------------------------------------------------------------ image
 struct DevImage {

@@ -28,9 +28,34 @@ __host__ inline size_t f2d_lds_bytes(int cn, int TH, int kw, int kh) {
     return sizeof(float) * static_cast<size_t>(cn) * (TH + kh - 1) * f2d_xp(kw);
 }
 
-__device__ inline double f2d_exact_sample(const DevImage& src, int x, int y, int c, const int* taps,
-                                          const double* tw, int ntaps, int ax, int ay, int border, double delta,
-                                          double bval);
+// One output sample in the reference's order (filter.hpp:185-190: s = delta,
+// then s += w * v over the non-zero taps in row-major order), computed by a
+// whole wave: each lane forms the products of 64 taps at a time and the sum
+// runs over broadcast values. All 64 lanes must call it; all return the result.
+template <class Ti>
+__device__ inline double f2d_exact_wave(const DevImage& src, int x, int y, int c, const int* __restrict__ taps,
+                                        const double* __restrict__ tw, int ntaps, int ax, int ay, int border,
+                                        double delta, double bval) {
+    const int lane = lane_id();
+    double s = delta;
+    for (int t0 = 0; t0 < ntaps; t0 += 64) {
+        const int t = t0 + lane;
+        double p = 0.0;
+        if (t < ntaps) {
+            const int sx = border_index(x + taps[2 * t] - ax, src.w, border);
+            const int sy = border_index(y + taps[2 * t + 1] - ay, src.h, border);
+            const double v = (sx < 0 || sy < 0)
+                                 ? bval
+                                 : static_cast<double>(*reinterpret_cast<const Ti*>(
+                                       static_cast<const char*>(src.data) + static_cast<size_t>(sy) * src.stride +
+                                       (static_cast<size_t>(sx) * src.cn + c) * sizeof(Ti)));
+            p = tw[t] * v;
+        }
+        const int n = min(64, ntaps - t0);
+        for (int k = 0; k < n; ++k) s = s + bcast_d(p, k);
+    }
+    return s;
+}
 
 struct F2dArgs {
     DevImage src, dst;
@@ -97,13 +122,15 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
             trow[c * plane + px] = v;
         }
     }
-    if (measure) atomicMax(&tile_max_bits, __float_as_uint(my_max));
+    if (measure) wave_max_into(&tile_max_bits, my_max);
     __syncthreads();
     double bound = 0.0;
     if (CERT) bound = a.bcoef * (measure ? static_cast<double>(__uint_as_float(tile_max_bits)) : a.xbound) + a.bdelta;
     constexpr int G = TX / P;
     const int ntask = TH * G * cn;
-    for (int t = tid; t < ntask; t += kBlock) {
+    const int ntask64 = (ntask + 63) & ~63;  // whole waves stay together for the fix-ups
+    for (int t = tid; t < ntask64; t += kBlock) {
+        const bool act = t < ntask;
         const int g = t % G;
         const int rest = t / G;
         const int r = rest % TH;
@@ -111,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
         Acc acc[P];
 #pragma unroll
         for (int m = 0; m < P; ++m) acc[m] = static_cast<Acc>(a.delta);
-        for (int j = 0; j < kh; ++j) {
+        for (int j = 0; j < (act ? kh : 0); ++j) {
             const float* base = smem + c * plane + (r + j) * XP + g * P;
             const Acc* krow = K + j * kw;
             Acc win[P];
@@ -142,54 +169,57 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
             }
         }
         const int y = y0 + r;
-        if (y >= H) continue;
-        char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
+        unsigned int dmask = 0u;
+        if (act && y < H) {
+            char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
 #pragma unroll
-        for (int m = 0; m < P; ++m) {
-            const int x = x0 + g * P + m;
-            if (x >= W) break;
-            const size_t q = static_cast<size_t>(x) * cn + c;
-            const double v = static_cast<double>(acc[m]);
-            bool doubt = false;
-            switch (a.dst.dt) {
-                case IPFG_U8:
-                    reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(v);
-                    if (CERT) doubt = needs_exact(v, bound, 0.0, 255.0);
-                    break;
-                case IPFG_U16:
-                    reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(v);
-                    if (CERT) doubt = needs_exact(v, bound, 0.0, 65535.0);
-                    break;
-                case IPFG_I16:
-                    reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(v);
-                    if (CERT) doubt = needs_exact(v, bound, -32768.0, 32767.0);
-                    break;
-                case IPFG_I32:
-                    reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(v);
-                    if (CERT) doubt = needs_exact(v, bound, -2147483648.0, 2147483647.0);
-                    break;
-                case IPFG_F32:
-                    reinterpret_cast<float*>(drow)[q] = static_cast<float>(v);
-                    if (CERT) doubt = needs_exact_f32(v, bound);
-                    break;
-                default:
-                    reinterpret_cast<double*>(drow)[q] = v;
-                    break;
+            for (int m = 0; m < P; ++m) {
+                const int x = x0 + g * P + m;
+                if (x >= W) break;
+                const size_t q = static_cast<size_t>(x) * cn + c;
+                const double v = static_cast<double>(acc[m]);
+                bool doubt = false;
+                switch (a.dst.dt) {
+                    case IPFG_U8:
+                        reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(v);
+                        if (CERT) doubt = needs_exact(v, bound, 0.0, 255.0);
+                        break;
+                    case IPFG_U16:
+                        reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(v);
+                        if (CERT) doubt = needs_exact(v, bound, 0.0, 65535.0);
+                        break;
+                    case IPFG_I16:
+                        reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(v);
+                        if (CERT) doubt = needs_exact(v, bound, -32768.0, 32767.0);
+                        break;
+                    case IPFG_I32:
+                        reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(v);
+                        if (CERT) doubt = needs_exact(v, bound, -2147483648.0, 2147483647.0);
+                        break;
+                    case IPFG_F32:
+                        reinterpret_cast<float*>(drow)[q] = static_cast<float>(v);
+                        if (CERT) doubt = needs_exact_f32(v, bound);
+                        break;
+                    default:
+                        reinterpret_cast<double*>(drow)[q] = v;
+                        break;
+                }
+                if (CERT && doubt) dmask |= 1u << m;
             }
-            if (CERT && doubt) fix_push(a.fix, static_cast<unsigned long long>(y) * W * cn + q);
+        }
+        if (CERT) {
+            const size_t es = dtype_size(a.dst.dt);
+            wave_fix(a.fix, dmask, [&](int l, int m) {
+                const int xl = x0 + bcast_i(g, l) * P + m, yl = bcast_i(y, l), cl = bcast_i(c, l);
+                const double e = f2d_exact_wave<Ti>(a.src, xl, yl, cl, a.taps, a.tapw, a.ntaps, a.ax, a.ay, a.border,
+                                                    a.delta, a.bval);
+                if (lane_id() == l)
+                    store_sample(a.dst.data, static_cast<size_t>(yl) * a.dst.stride + (static_cast<size_t>(xl) * cn + cl) * es,
+                                 a.dst.dt, e);
+            });
         }
     }
-    if (CERT) {
-        const size_t rowLen = static_cast<size_t>(W) * cn;
-        const size_t es = dtype_size(a.dst.dt);
-        fix_epilogue(a.fix, static_cast<unsigned long long>(H) * rowLen, [&](unsigned long long s) {
-            const int y = static_cast<int>(s / rowLen);
-            const size_t q = s - static_cast<size_t>(y) * rowLen;
-            const int x = static_cast<int>(q / cn), c = static_cast<int>(q - static_cast<size_t>(x) * cn);
-            const double v = f2d_exact_sample(a.src, x, y, c, a.taps, a.tapw, a.ntaps, a.ax, a.ay, a.border, a.delta, a.bval);
-            store_sample(a.dst.data, static_cast<size_t>(y) * a.dst.stride + q * es, a.dst.dt, v);
-        });
-    }
+    if (CERT) fix_finish(a.fix);
 }
 
 // ------------------------------------------------------------------ exact

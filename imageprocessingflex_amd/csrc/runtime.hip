@@ -98,7 +98,8 @@ void ensure_init() {
 // fix-up count of the last certified call.
 struct ThreadCtx {
     hipStream_t stream = nullptr;
-    unsigned long long* fix_host = nullptr;  // pinned
+    unsigned long long* fix_host = nullptr;  // pinned, mapped
+    unsigned long long* fix_dev = nullptr;   // the same word's device address
     bool fix_valid = false;
 };
 thread_local ThreadCtx t_ctx;
@@ -109,14 +110,18 @@ hipStream_t pick_stream(ipfg_stream user) {
     return t_ctx.stream;
 }
 
+// Device address of this thread's report word.
 unsigned long long* fix_host_slot() {
     if (!t_ctx.fix_host) {
         void* p = nullptr;
-        HIPCALL(hipHostMalloc(&p, sizeof(unsigned long long), hipHostMallocDefault));
+        HIPCALL(hipHostMalloc(&p, sizeof(unsigned long long), hipHostMallocMapped));
         t_ctx.fix_host = static_cast<unsigned long long*>(p);
         *t_ctx.fix_host = 0;
+        void* d = nullptr;
+        HIPCALL(hipHostGetDevicePointer(&d, p, 0));
+        t_ctx.fix_dev = static_cast<unsigned long long*>(d);
     }
-    return t_ctx.fix_host;
+    return t_ctx.fix_dev;
 }
 
 // ------------------------------------------------------------------ pinned ring
@@ -253,13 +258,12 @@ private:
 ParamCache g_params;
 
 // ------------------------------------------------------------------ fix-up state
-// Each stream owns the counter, workgroup ticket and report word of its
-// certified launches. They are zero between launches: zeroed by a kernel
+// Each stream owns the counter and workgroup ticket of its certified
+// launches. They are zero between launches: zeroed by a kernel
 // when the stream is first seen, reset by the last workgroup of each launch.
 struct FixState {
     unsigned long long* count = nullptr;
     unsigned int* done = nullptr;
-    unsigned long long* report = nullptr;
 };
 std::mutex g_fix_mu;
 std::unordered_map<hipStream_t, FixState> g_fix;
@@ -272,8 +276,7 @@ FixState fix_state(hipStream_t s) {
     HIPCALL(hipMalloc(&p, 32));
     FixState f;
     f.count = static_cast<unsigned long long*>(p);
-    f.report = f.count + 1;
-    f.done = reinterpret_cast<unsigned int*>(f.count + 2);
+    f.done = reinterpret_cast<unsigned int*>(f.count + 1);
     HIPCALL(launch_zero_counter(f.count, f.done, s));
     g_fix.emplace(s, f);
     return f;
@@ -309,14 +312,14 @@ public:
         held_.push_back(e);
         return e->dev;
     }
-    FixList fixlist(unsigned long long capacity) {
+    // The stream's counter and ticket; the count lands in this thread's
+    // pinned report word (written by the launch's last workgroup).
+    FixList fixlist() {
         const FixState st = fix_state(s_);
         FixList f;
         f.count = st.count;
         f.done = st.done;
-        f.report = st.report;
-        f.capacity = capacity;
-        f.index = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long) * (capacity ? capacity : 1)));
+        f.report = fix_host_slot();
         return f;
     }
     hipStream_t stream() const { return s_; }
@@ -410,18 +413,7 @@ private:
     bool sync_ = false;
 };
 
-// Fix-up list capacity: 1/8 of the samples (at least 64K entries); beyond it
-// the last workgroup recomputes every sample in FP64.
-unsigned long long fix_capacity(unsigned long long n) {
-    const unsigned long long c = n / 8;
-    return c < 65536 ? 65536 : c;
-}
-
-void record_fixups(const FixList& f, hipStream_t s) {
-    unsigned long long* h = fix_host_slot();
-    HIPCALL(hipMemcpyAsync(h, f.report, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    t_ctx.fix_valid = true;
-}
+void record_fixups(const FixList&, hipStream_t) { t_ctx.fix_valid = true; }
 
 constexpr double kU32 = 1.0 / 16777216.0;          // unit roundoff of FP32 (2^-24)
 constexpr double kU64 = 1.0 / 9007199254740992.0;  // unit roundoff of FP64 (2^-53)
@@ -524,7 +516,7 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
             a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sx * sy;
         a.xbound = input_bound(src->dtype, bval, border);
-        a.fix = sc.fixlist(fix_capacity(static_cast<unsigned long long>(a.dst.w) * a.dst.h * a.dst.cn));
+        a.fix = sc.fixlist();
     }
     if (a.mode == MODE_EXACT) a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
     HIPCALL(launch_sep(a, s));
@@ -636,7 +628,7 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
         if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
             a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sk;
         a.xbound = input_bound(src->dtype, bval, border);
-        a.fix = sc.fixlist(fix_capacity(static_cast<unsigned long long>(a.dst.w) * a.dst.h * a.dst.cn));
+        a.fix = sc.fixlist();
     }
     HIPCALL(launch_filter2d(a, s));
     if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
@@ -748,7 +740,7 @@ void bilateral_impl(const ipfg_image* src, const ipfg_image* dst, int d, double 
         // and a division, i.e. (2 ntaps + 4) u relative to a result <= X.
         const double X = 255.0;
         a.bound = 2.02 * (2.0 * a.ntaps + 6.0) * kU64 * X + 1e-300;
-        a.fix = sc.fixlist(fix_capacity(static_cast<unsigned long long>(a.dst.w) * a.dst.h));
+        a.fix = sc.fixlist();
     }
     HIPCALL(launch_bilateral(a, s));
     if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);

@@ -31,7 +31,6 @@ namespace ipfg {
 namespace {
 
 using sepk::SepArgs;
-using sepk::sep_exact_sample;
 
 // ------------------------------------------------------------------ exact
 __global__ __launch_bounds__(kBlock) void sep_exact_h(DevImage src, double* __restrict__ tmp,

@@ -63,32 +63,45 @@ __device__ __forceinline__ void load4(const Acc* __restrict__ k, int i, Acc (&ou
     out[3] = k[i + 3];
 }
 
-// One output sample in the reference's FP64 operation order
-// (filter.hpp:57-86: padded row -> tmp from 0.0 over i, acc from delta over j).
-__device__ inline double sep_exact_sample(const DevImage& src, int x, int y, int c, const double* kx, int kw,
-                                          const double* ky, int kh, int ax, int ay, int border, double delta,
-                                          double bval) {
-    const size_t es = dtype_size(src.dt);
-    const bool inside_x = x - ax >= 0 && x - ax + kw <= src.w;
+// One output sample in the reference's FP64 operation order (filter.hpp:57-86:
+// tmp = 0.0 + kx[0] * v[0] + kx[1] * v[1] + ... along the padded row, then
+// acc = delta + ky[0] * tmp[0] + ... down the rows), computed by a whole wave:
+// lane j forms the tmp value of row j (64 rows per pass) and the vertical sum
+// runs over broadcast values. All 64 lanes must call it; all return the result.
+template <class Ti>
+__device__ inline double sep_exact_wave(const DevImage& src, int x, int y, int c, const double* __restrict__ kx,
+                                        int kw, const double* __restrict__ ky, int kh, int ax, int ay, int border,
+                                        double delta, double bval) {
+    const int cn = src.cn;
+    const int xl = x - ax;
+    const bool inside_x = xl >= 0 && xl + kw <= src.w;
+    const int lane = lane_id();
     double acc = delta;
-    for (int j = 0; j < kh; ++j) {
-        const int sy = border_index(y + j - ay, src.h, border);
+    for (int j0 = 0; j0 < kh; j0 += 64) {
+        const int j = j0 + lane;
         double t = 0.0;
-        if (sy >= 0 && inside_x) {
-            const char* row = static_cast<const char*>(src.data) + static_cast<size_t>(sy) * src.stride +
-                              (static_cast<size_t>(x - ax) * src.cn + c) * es;
-            for (int i = 0; i < kw; ++i) t = t + kx[i] * load_sample(row, static_cast<size_t>(i) * src.cn * es, src.dt);
-        } else {
-            for (int i = 0; i < kw; ++i) {
-                const int sx = border_index(x + i - ax, src.w, border);
-                const double v = (sy < 0 || sx < 0)
-                                     ? bval
-                                     : load_sample(src.data, static_cast<size_t>(sy) * src.stride +
-                                                                 (static_cast<size_t>(sx) * src.cn + c) * es, src.dt);
-                t = t + kx[i] * v;
+        if (j < kh) {
+            const int sy = border_index(y + j - ay, src.h, border);
+            if (sy < 0) {
+                for (int i = 0; i < kw; ++i) t = t + kx[i] * bval;
+            } else {
+                const Ti* row = reinterpret_cast<const Ti*>(static_cast<const char*>(src.data) +
+                                                            static_cast<size_t>(sy) * src.stride) + c;
+                if (inside_x) {
+                    const Ti* p = row + static_cast<size_t>(xl) * cn;
+#pragma unroll 8
+                    for (int i = 0; i < kw; ++i) t = t + kx[i] * static_cast<double>(p[static_cast<size_t>(i) * cn]);
+                } else {
+                    for (int i = 0; i < kw; ++i) {
+                        const int sx = border_index(xl + i, src.w, border);
+                        const double v = sx < 0 ? bval : static_cast<double>(row[static_cast<size_t>(sx) * cn]);
+                        t = t + kx[i] * v;
+                    }
+                }
             }
         }
-        acc = acc + ky[j] * t;
+        const int jn = min(64, kh - j0);
+        for (int jj = 0; jj < jn; ++jj) acc = acc + ky[j0 + jj] * bcast_d(t, jj);
     }
     return acc;
 }
@@ -157,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
         }
         if (measure) {
             if (patch_bval) my_max = fmaxf(my_max, fabsf(bvalf));
-            atomicMax(&tile_max_bits, __float_as_uint(my_max));
+            wave_max_into(&tile_max_bits, my_max);
         }
         __syncthreads();
         if (k + 1 < nhstep)
@@ -241,92 +254,97 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
             const int col = t % ncol;
             const int v = t / ncol;
             const int yr = ob + v * M;  // first output row of this task (strip-relative)
-            if (yr >= rows_out) continue;
-            const Acc* colp = ring + col;
-            const int msk = T.R - 1;
-            Acc acc[M], win[M];
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                acc[m] = static_cast<Acc>(a.delta);
-                win[m] = colp[((yr + m) & msk) * T.HP];
-            }
-            Acc kc[M];
-            load4(ky, 0, kc);
-            int j0 = 0;
-            for (; j0 + M <= kh; j0 += M) {
-                Acc kn[M];
-                load4(ky, j0 + M, kn);
-                Acc nx[M];
-#pragma unroll
-                for (int jj = 0; jj < M; ++jj) nx[jj] = colp[((yr + j0 + jj + M) & msk) * T.HP];
-#pragma unroll
-                for (int jj = 0; jj < M; ++jj) {
-#pragma unroll
-                    for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
-                    win[jj] = nx[jj];
-                }
-#pragma unroll
-                for (int jj = 0; jj < M; ++jj) kc[jj] = kn[jj];
-            }
-#pragma unroll
-            for (int jj = 0; jj < M; ++jj) {
-                if (j0 + jj < kh) {
-#pragma unroll
-                    for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
-                    win[jj] = colp[((yr + j0 + jj + M) & msk) * T.HP];
-                }
-            }
             const int px = col / cn, c = col - px * cn;
             const int x = x0 + px;
-            if (x >= W) continue;
             const size_t q = static_cast<size_t>(x) * cn + c;
+            unsigned int dmask = 0u;  // doubtful outputs of this task
+            if (yr < rows_out) {
+                const Acc* colp = ring + col;
+                const int msk = T.R - 1;
+                Acc acc[M], win[M];
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                if (yr + m >= rows_out) break;
-                const int y = y0 + yr + m;
-                char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
-                const double r = static_cast<double>(acc[m]);
-                bool doubt = false;
-                switch (a.dst.dt) {
-                    case IPFG_U8:
-                        reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(r);
-                        if (CERT) doubt = needs_exact(r, bound, 0.0, 255.0);
-                        break;
-                    case IPFG_U16:
-                        reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(r);
-                        if (CERT) doubt = needs_exact(r, bound, 0.0, 65535.0);
-                        break;
-                    case IPFG_I16:
-                        reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(r);
-                        if (CERT) doubt = needs_exact(r, bound, -32768.0, 32767.0);
-                        break;
-                    case IPFG_I32:
-                        reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(r);
-                        if (CERT) doubt = needs_exact(r, bound, -2147483648.0, 2147483647.0);
-                        break;
-                    case IPFG_F32:
-                        reinterpret_cast<float*>(drow)[q] = static_cast<float>(r);
-                        if (CERT) doubt = needs_exact_f32(r, bound);
-                        break;
-                    default:
-                        reinterpret_cast<double*>(drow)[q] = r;
-                        break;
+                for (int m = 0; m < M; ++m) {
+                    acc[m] = static_cast<Acc>(a.delta);
+                    win[m] = colp[((yr + m) & msk) * T.HP];
                 }
-                if (CERT && doubt) fix_push(a.fix, static_cast<unsigned long long>(y) * W * cn + q);
+                Acc kc[M];
+                load4(ky, 0, kc);
+                int j0 = 0;
+                for (; j0 + M <= kh; j0 += M) {
+                    Acc kn[M];
+                    load4(ky, j0 + M, kn);
+                    Acc nx[M];
+#pragma unroll
+                    for (int jj = 0; jj < M; ++jj) nx[jj] = colp[((yr + j0 + jj + M) & msk) * T.HP];
+#pragma unroll
+                    for (int jj = 0; jj < M; ++jj) {
+#pragma unroll
+                        for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
+                        win[jj] = nx[jj];
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < M; ++jj) kc[jj] = kn[jj];
+                }
+#pragma unroll
+                for (int jj = 0; jj < M; ++jj) {
+                    if (j0 + jj < kh) {
+#pragma unroll
+                        for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
+                        win[jj] = colp[((yr + j0 + jj + M) & msk) * T.HP];
+                    }
+                }
+                if (x < W) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        if (yr + m >= rows_out) break;
+                        const int y = y0 + yr + m;
+                        char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
+                        const double r = static_cast<double>(acc[m]);
+                        bool doubt = false;
+                        switch (a.dst.dt) {
+                            case IPFG_U8:
+                                reinterpret_cast<uint8_t*>(drow)[q] = saturate_from_double<uint8_t>(r);
+                                if (CERT) doubt = needs_exact(r, bound, 0.0, 255.0);
+                                break;
+                            case IPFG_U16:
+                                reinterpret_cast<uint16_t*>(drow)[q] = saturate_from_double<uint16_t>(r);
+                                if (CERT) doubt = needs_exact(r, bound, 0.0, 65535.0);
+                                break;
+                            case IPFG_I16:
+                                reinterpret_cast<int16_t*>(drow)[q] = saturate_from_double<int16_t>(r);
+                                if (CERT) doubt = needs_exact(r, bound, -32768.0, 32767.0);
+                                break;
+                            case IPFG_I32:
+                                reinterpret_cast<int32_t*>(drow)[q] = saturate_from_double<int32_t>(r);
+                                if (CERT) doubt = needs_exact(r, bound, -2147483648.0, 2147483647.0);
+                                break;
+                            case IPFG_F32:
+                                reinterpret_cast<float*>(drow)[q] = static_cast<float>(r);
+                                if (CERT) doubt = needs_exact_f32(r, bound);
+                                break;
+                            default:
+                                reinterpret_cast<double*>(drow)[q] = r;
+                                break;
+                        }
+                        if (CERT && doubt) dmask |= 1u << m;
+                    }
+                }
+            }
+            if (CERT) {
+                const size_t es = dtype_size(a.dst.dt);
+                wave_fix(a.fix, dmask, [&](int l, int m) {
+                    const int xl = bcast_i(x, l), cl = bcast_i(c, l);
+                    const int yl = y0 + bcast_i(yr, l) + m;
+                    const double e = sep_exact_wave<Ti>(a.src, xl, yl, cl, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border,
+                                                        a.delta, a.bval);
+                    if (lane_id() == l)
+                        store_sample(a.dst.data, static_cast<size_t>(yl) * a.dst.stride + (static_cast<size_t>(xl) * cn + cl) * es,
+                                     a.dst.dt, e);
+                });
             }
         }
     }
-    if (CERT) {
-        const size_t rowLen = static_cast<size_t>(W) * cn;
-        const size_t es = dtype_size(a.dst.dt);
-        fix_epilogue(a.fix, static_cast<unsigned long long>(H) * rowLen, [&](unsigned long long s) {
-            const int y = static_cast<int>(s / rowLen);
-            const size_t q = s - static_cast<size_t>(y) * rowLen;
-            const int x = static_cast<int>(q / cn), c = static_cast<int>(q - static_cast<size_t>(x) * cn);
-            const double v = sep_exact_sample(a.src, x, y, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border, a.delta, a.bval);
-            store_sample(a.dst.data, static_cast<size_t>(y) * a.dst.stride + q * es, a.dst.dt, v);
-        });
-    }
+    if (CERT) fix_finish(a.fix);
 }
 
 // ------------------------------------------------------------------ planar kernel
@@ -440,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                 }
                 if (measure) my_max = fmaxf(my_max, fabsf(bvalf));
             }
-            if (measure) atomicMax(&tile_max_bits, __float_as_uint(my_max));
+            if (measure) wave_max_into(&tile_max_bits, my_max);
         }
         __syncthreads();
         if (k + 1 < nhstep)
@@ -558,30 +576,35 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                 }
             }
             const int xg = x0 + x;
-            if (xg >= W) continue;
-            const size_t q = static_cast<size_t>(xg) * CN + c;
-            char* op = static_cast<char*>(a.dst.data) + static_cast<size_t>(y0 + yr) * a.dst.stride + q * sizeof(To);
+            unsigned int dmask = 0u;  // doubtful outputs of this task
+            if (xg < W) {
+                char* op = static_cast<char*>(a.dst.data) + static_cast<size_t>(y0 + yr) * a.dst.stride +
+                           (static_cast<size_t>(xg) * CN + c) * sizeof(To);
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                if (yr + m >= rows_out) break;
-                const double rv = static_cast<double>(acc[m]);
-                put<To>(op, rv);
-                if (CERT && doubtful<To>(rv, bound))
-                    fix_push(a.fix, static_cast<unsigned long long>(y0 + yr + m) * W * CN + q);
-                op += a.dst.stride;
+                for (int m = 0; m < M; ++m) {
+                    if (yr + m >= rows_out) break;
+                    const double rv = static_cast<double>(acc[m]);
+                    put<To>(op, rv);
+                    if (CERT && doubtful<To>(rv, bound)) dmask |= 1u << m;
+                    op += a.dst.stride;
+                }
+            }
+            // yr and c are wave-uniform (PTX = 64 lanes share a row group and channel)
+            if (CERT) {
+                wave_fix(a.fix, dmask, [&](int l, int m) {
+                    const int xl = bcast_i(xg, l);
+                    const int yl = y0 + yr + m;
+                    const double e = sep_exact_wave<Ti>(a.src, xl, yl, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border,
+                                                        a.delta, a.bval);
+                    if (lane_id() == l)
+                        put<To>(static_cast<char*>(a.dst.data) + static_cast<size_t>(yl) * a.dst.stride +
+                                    (static_cast<size_t>(xl) * CN + c) * sizeof(To),
+                                e);
+                });
             }
         }
     }
-    if (CERT) {
-        const size_t rowLen = static_cast<size_t>(W) * CN;
-        fix_epilogue(a.fix, static_cast<unsigned long long>(H) * rowLen, [&](unsigned long long s) {
-            const int y = static_cast<int>(s / rowLen);
-            const size_t q = s - static_cast<size_t>(y) * rowLen;
-            const int x = static_cast<int>(q / CN), c = static_cast<int>(q - static_cast<size_t>(x) * CN);
-            const double v = sep_exact_sample(a.src, x, y, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border, a.delta, a.bval);
-            put<To>(static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride + q * sizeof(To), v);
-        });
-    }
+    if (CERT) fix_finish(a.fix);
 }
 
 template <class Ti, class Acc, int CN, int SJ, bool CERT>

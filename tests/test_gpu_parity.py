@@ -175,15 +175,26 @@ def test_concurrent_host_threads(ipf, oracle):
         assert np.array_equal(g, w)
 
 
-def test_certification_overflow_falls_back_to_exact(ipf, oracle):
+@pytest.mark.parametrize("shape", [(1024, 1024), (333, 517, 3), (77, 1100, 4)])
+def test_certification_heavy_fixups_sep(ipf, oracle, shape):
     # kernel [0.5, 0.5]: every sample whose 2x2 sum is 2 mod 4 sits exactly on
-    # a rounding tie, flagging more samples than the fix-up list holds, so the
-    # last workgroup recomputes the whole image.
+    # a rounding tie, so about a quarter of the samples are recomputed in the
+    # reference's order inside the launch.
     rng = np.random.default_rng(14)
-    img = rng.integers(0, 256, (1024, 1024), dtype=np.uint8)
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
     k = np.array([0.5, 0.5])
     got = ipf.sep_filter2d(img, k, k, 1)
     assert np.array_equal(got, oracle.sep_filter(img, k, k, border=1))
+    assert ipf.last_fixups() > img.size // 8
+
+
+@pytest.mark.parametrize("shape", [(1024, 1024), (61, 203, 3)])
+def test_certification_heavy_fixups_filter2d(ipf, oracle, shape):
+    rng = np.random.default_rng(16)
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
+    kern = np.array([[0.5, 0.5], [0.5, 0.5]], dtype=np.float32) * 0.5
+    got = ipf.filter2d(img, kern, 2)
+    assert np.array_equal(got, oracle.filter2d(img, kern, border=2))
     assert ipf.last_fixups() > img.size // 8
 
 
