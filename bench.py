@@ -1,24 +1,34 @@
 """Benchmark of the MI355X filtering path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extras]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4]
+                    [--no-cpu] [--cpu-budget S]
 
-Workload (one step): the separable-Gaussian / box roofline case on a
-7680x4320x1 float32 image (16 seeded sinusoids + N(0, 0.05) noise, [0, 1]):
-gaussianBlur sigma 1, 3, 8 and blur 15x15, each under all five border modes
-(20 filter calls). Inputs are resident in HBM; three input/output sets
-(795 MB) rotate so the 256 MB Infinity Cache cannot hold a working set.
+Workloads (one step = every call listed once; inputs resident in HBM; enough
+input/output sets rotate that the working set exceeds the 256 MB Infinity
+Cache):
+  c2 (default)  7680x4320x1 float32 (16 seeded sinusoids + N(0, 0.05) noise,
+                clamped to [0, 1]): gaussianBlur sigma 1, 3, 8 and blur 15x15,
+                each under all five border modes (20 calls).
+  c1            1920x1080x3 uint8 uniform: gaussianBlur sigma 2, Reflect101.
+  c3            3840x2160x3 uint8 (64 Voronoi regions + N(0, 8) + 5% salt and
+                pepper): medianBlur 3 and 5 (Replicate), bilateralFilter(9, 75, 75).
+  c4            4096x4096x1 uint8 and float32 uniform: filter2D with a dense
+                31x31 N(0, 1) kernel normalised to sum 1, Reflect101.
+All calls use the default precision: FP64 accumulation, certified bit-identical
+to the reference (median is compare-only).
 
 Prints ONE JSON line: value = Gpix/s over the whole step, the roofline of the
 dominant kernel (algorithmic bytes per launch / mean launch time from HIP
 events on the launch stream), per-filter Gpix/s and roofline fraction, call
 latency percentiles, and the reference's multithreaded CPU path timed on a
-bounded sample on this host.
+bounded sample of the same calls on this host.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import statistics
 import sys
@@ -31,10 +41,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-W, H = 7680, 4320
+MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 / fp16
 BORDERS = ["constant", "replicate", "reflect", "reflect101", "wrap"]
-FILTERS = [("gauss_s1", 1.0), ("gauss_s3", 3.0), ("gauss_s8", 8.0), ("box15", None)]
-KERNEL_OF = {"gauss": "sep_fast_kernel<float, 1>", "box": "box_fast_kernel<float>"}
+METRIC = "Gpix/s + % HBM roofline per filter; p50/p99 latency, batched vs plain"
 
 
 class HipEvents:
@@ -46,6 +55,7 @@ class HipEvents:
         self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
         self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
 
     def make(self, n):
         evs = []
@@ -65,7 +75,87 @@ class HipEvents:
         return out.value
 
 
-def cpu_baseline(img, budget_s: float):
+class Call:
+    """One filter call of a step: `launch(lib, src, dst, stream)` enqueues it
+    on device descriptors; `cpu(oracle, img)` runs the reference's CPU path."""
+
+    def __init__(self, name, group, image, out_dtype, launch, cpu, kernel, flops=0.0):
+        self.name, self.group, self.image, self.out_dtype = name, group, image, out_dtype
+        self.launch, self.cpu, self.kernel, self.flops = launch, cpu, kernel, flops
+
+
+def _c2():
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.sinusoid_image(4320, 7680, seed=2)
+    calls = []
+    for name, sigma in (("gauss_s1", 1.0), ("gauss_s3", 3.0), ("gauss_s8", 8.0), ("box15", None)):
+        for b in range(5):
+            if sigma is None:
+                launch = (lambda b: lambda lib, s, d, st: lib.ipfg_box_filter(s, d, 15, 15, 1, b, 0.0, 0, st))(b)
+                cpu = (lambda b: lambda o, im: o.box_filter(im, 15, 15, True, b))(b)
+                kern = "box_fast_kernel<float, float>"
+            else:
+                launch = (lambda b, sg: lambda lib, s, d, st: lib.ipfg_gaussian_blur(s, d, sg, 0.0, 0, b, 0, st))(b, sigma)
+                cpu = (lambda b, sg: lambda o, im: o.gaussian_blur(im, sg, border=b))(b, sigma)
+                kern = "sep_plane_kernel<float, float, double, 1, 2, true>"
+            calls.append(Call(f"{name}/{BORDERS[b]}", name, "img", np.float32, launch, cpu, kern))
+    return {"images": {"img": img}, "calls": calls, "dtype": "f64",
+            "workload": "gaussianBlur sigma 1/3/8 + blur 15x15, x 5 border modes, 7680x4320x1 float32"}
+
+
+def _c1():
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.c1_image()
+    calls = [Call("gauss_s2/reflect101", "gauss_s2", "img", np.uint8,
+                  lambda lib, s, d, st: lib.ipfg_gaussian_blur(s, d, 2.0, 0.0, 0, 3, 0, st),
+                  lambda o, im: o.gaussian_blur(im, 2.0, border=3),
+                  "sep_plane_kernel<unsigned char, unsigned char, double, 3, 2, true>")]
+    return {"images": {"img": img}, "calls": calls, "dtype": "f64",
+            "workload": "gaussianBlur sigma 2 (13 taps), Reflect101, 1920x1080x3 uint8"}
+
+
+def _c3():
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.polygon_image()
+    calls = [
+        Call("median3/replicate", "median3", "img", np.uint8,
+             lambda lib, s, d, st: lib.ipfg_median(s, d, 3, 1, st), lambda o, im: o.median(im, 3, 1),
+             "median_kernel<unsigned char, 3>"),
+        Call("median5/replicate", "median5", "img", np.uint8,
+             lambda lib, s, d, st: lib.ipfg_median(s, d, 5, 1, st), lambda o, im: o.median(im, 5, 1),
+             "median_kernel<unsigned char, 5>"),
+        Call("bilateral9/reflect101", "bilateral9", "img", np.uint8,
+             lambda lib, s, d, st: lib.ipfg_bilateral(s, d, 9, 75.0, 75.0, 3, 0, st),
+             lambda o, im: o.bilateral(im, 9, 75.0, 75.0, 3),
+             "bilateral_u8_kernel<3, false, double, true>"),
+    ]
+    return {"images": {"img": img}, "calls": calls, "dtype": "f64",
+            "workload": "medianBlur 3x3 and 5x5 (Replicate) + bilateralFilter(9, 75, 75) (Reflect101), "
+                        "3840x2160x3 uint8 polygons + noise"}
+
+
+def _c4():
+    from imageprocessingflex_amd import workloads as wl
+    u8, f32 = wl.c4_images(4096)
+    k, _ = wl.c4_kernel(31)
+    kp = np.ascontiguousarray(k, np.float32)
+    kptr = kp.ctypes.data_as(C.POINTER(C.c_float))
+    flops = 2.0 * 4096 * 4096 * 31 * 31
+    calls = []
+    for name, key, dt in (("filter2d_u8", "u8", np.uint8), ("filter2d_f32", "f32", np.float32)):
+        calls.append(Call(f"{name}/reflect101", name, key, dt,
+                          lambda lib, s, d, st, kptr=kptr: lib.ipfg_filter2d(s, d, kptr, 31, 31, 15, 15, 3, 0.0, 0.0, 0, st),
+                          lambda o, im: o.filter2d(im, kp, border=3),
+                          "f2d_fast_kernel", flops=flops))
+    return {"images": {"u8": u8, "f32": f32}, "calls": calls, "dtype": "f64",
+            "workload": "filter2D dense 31x31 float32 kernel (N(0,1), sum 1), Reflect101, 4096x4096x1 uint8 and float32",
+            "keep": kp}
+
+
+WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4}
+
+
+def cpu_baseline(wd, budget_s: float):
     """The reference's CPU path (oracle/_ref when built, else the oracle
     restatement) with all host threads, on a bounded sample of the step."""
     try:
@@ -75,34 +165,29 @@ def cpu_baseline(img, budget_s: float):
         return {"value": None, "unit": "Gpix/s", "cores": os.cpu_count(), "kind": None, "sample": f"unavailable: {e}"}
     o.set_num_threads(0)
     threads = o.get_num_threads()
-    done, t_total = [], 0.0
-    for (name, sigma) in FILTERS:
-        for b in range(5):
+    done, t_total, px = [], 0.0, 0
+    while t_total < budget_s:  # whole passes over the step's calls until the budget is spent
+        for c in wd["calls"]:
+            im = wd["images"][c.image]
             t0 = time.perf_counter()
-            if sigma is None:
-                o.box_filter(img, 15, 15, True, b)
-            else:
-                o.gaussian_blur(img, sigma, border=b)
-            dt = time.perf_counter() - t0
-            t_total += dt
-            done.append(f"{name}/{BORDERS[b]}")
+            c.cpu(o, im)
+            t_total += time.perf_counter() - t0
+            done.append(c.name)
+            px += im.shape[0] * im.shape[1]
             if t_total > budget_s:
                 break
-        if t_total > budget_s:
-            break
-    gpix = len(done) * W * H / 1e9
-    return {"value": round(gpix / t_total, 4), "unit": "Gpix/s", "cores": threads,
+    return {"value": round(px / t_total / 1e9, 5), "unit": "Gpix/s", "cores": threads,
             "kind": "reference" if o.which == "ref" else "port",
-            "sample": f"{len(done)} of 20 filter calls of the step, 1 call each, all {threads} threads, "
-                      f"{t_total:.1f} s: " + ", ".join(done)}
+            "sample": f"{len(done)} calls ({len(wd['calls'])} per step, taken in step order), {threads} threads, "
+                      f"{t_total:.1f} s: " + ", ".join(sorted(set(done)))}
 
 
-def traffic_from_profiles(kernel_tag: str):
+def traffic_from_profiles(group: str):
     p = ROOT / "profiles" / "traffic.json"
     if not p.exists():
         return None
     d = json.loads(p.read_text())
-    return d.get(kernel_tag, {}).get("bytes_per_launch")
+    return d.get(group, {}).get("bytes_per_launch")
 
 
 def main():
@@ -110,6 +195,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -119,77 +205,91 @@ def main():
         sys.exit(2)
 
     import imageprocessingflex_amd as ipf
-    from imageprocessingflex_amd import workloads as wl
 
     ipf.init(0)
     t0 = time.time()
-    img = wl.sinusoid_image(H, W, seed=2)
-    print(f"# input ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    wd = WORKLOADS[args.workload]()
+    print(f"# {args.workload} inputs ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     stream = ipf.Stream()
-    sets = [ipf.DeviceImage.from_array(img, stream) for _ in range(3)]
-    outs = [ipf.DeviceImage.like(s) for s in sets]
+    # rotating device sets: > 256 MB of inputs + outputs in flight
+    set_bytes = sum(im.nbytes * 2 for im in wd["images"].values())
+    nsets = max(3, math.ceil(300e6 / set_bytes))
+    sets = [{k: ipf.DeviceImage.from_array(im, stream) for k, im in wd["images"].items()} for _ in range(nsets)]
+    outs = [{(c.image, np.dtype(c.out_dtype).str): ipf.DeviceImage.like(s[c.image], c.out_dtype) for c in wd["calls"]}
+            for s in sets]
+    calls = wd["calls"]
     ev = HipEvents()
-    calls = [(name, sigma, b) for (name, sigma) in FILTERS for b in range(5)]
     evs = ev.make(2 * len(calls))
     lib = ipf._lib.load()
-
-    def run_call(i, name, sigma, b, k):
-        src, dst = sets[k % 3].descriptor(), outs[k % 3].descriptor()
-        if sigma is None:
-            rc = lib.ipfg_box_filter(C.byref(src), C.byref(dst), 15, 15, 1, b, 0.0, 0, stream.handle)
-        else:
-            rc = lib.ipfg_gaussian_blur(C.byref(src), C.byref(dst), sigma, 0.0, 0, b, 0, stream.handle)
-        ipf._lib.check(rc)
-
-    per_call = {c: [] for c in calls}
-    ipf.synchronize()
     k = 0
+
+    def run(c):
+        nonlocal k
+        s = sets[k % nsets][c.image].descriptor()
+        d = outs[k % nsets][(c.image, np.dtype(c.out_dtype).str)].descriptor()
+        ipf._lib.check(c.launch(lib, C.byref(s), C.byref(d), stream.handle))
+        k += 1
+
+    per_call = {c.name: [] for c in calls}
     for _ in range(args.warmup):
-        for (name, sigma, b) in calls:
-            run_call(0, name, sigma, b, k)
-            k += 1
+        for c in calls:
+            run(c)
     stream.synchronize()
     ipf.synchronize()
     t_start = time.perf_counter()
-    for step in range(args.steps):
-        for i, (name, sigma, b) in enumerate(calls):
-            ev.record(evs[2 * i], stream.handle)
-            run_call(i, name, sigma, b, k)
-            ev.record(evs[2 * i + 1], stream.handle)
-            k += 1
+    for _ in range(args.steps):
         for i, c in enumerate(calls):
-            per_call[c].append(ev.ms(evs[2 * i], evs[2 * i + 1]))
+            ev.record(evs[2 * i], stream.handle)
+            run(c)
+            ev.record(evs[2 * i + 1], stream.handle)
+        for i, c in enumerate(calls):
+            per_call[c.name].append(ev.ms(evs[2 * i], evs[2 * i + 1]))
     stream.synchronize()
     ipf.synchronize()
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1e3 / args.steps
-    step_px = len(calls) * W * H
+    step_px = sum(wd["images"][c.image].shape[0] * wd["images"][c.image].shape[1] for c in calls)
     value = step_px / (ms_per_step * 1e-3) / 1e9
 
-    bytes_per_call = W * H * (4 + 4)  # algorithmic: read + write float32
     per_filter = {}
-    for (name, sigma) in FILTERS:
-        ms = [m for (n, s, b), v in per_call.items() if n == name for m in v]
-        mean = statistics.mean(ms)
-        per_filter[name] = {
-            "mean_ms": round(mean, 4),
-            "gpix_s": round(W * H / (mean * 1e-3) / 1e9, 2),
-            "gb_s": round(bytes_per_call / (mean * 1e-3) / 1e9, 1),
-            "hbm_roofline_frac": round(bytes_per_call / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        }
-    dom = max(per_filter, key=lambda n: per_filter[n]["mean_ms"])
+    for c in calls:
+        im = wd["images"][c.image]
+        g = per_filter.setdefault(c.group, {"ms": [], "bytes": im.nbytes + im.size * np.dtype(c.out_dtype).itemsize,
+                                            "px": im.shape[0] * im.shape[1], "flops": c.flops, "kernel": c.kernel})
+        g["ms"] += per_call[c.name]
+    summary = {}
+    for name, g in per_filter.items():
+        mean = statistics.mean(g["ms"])
+        gbs = g["bytes"] / (mean * 1e-3) / 1e9
+        summary[name] = {"mean_ms": round(mean, 4), "gpix_s": round(g["px"] / (mean * 1e-3) / 1e9, 2),
+                         "gb_s": round(gbs, 1), "hbm_roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if g["flops"]:
+            summary[name]["tflop_s"] = round(g["flops"] / (mean * 1e-3) / 1e12, 2)
+    dom = max(summary, key=lambda n: summary[n]["mean_ms"])
     all_ms = sorted(m for v in per_call.values() for m in v)
     p50 = all_ms[len(all_ms) // 2]
     p99 = all_ms[min(len(all_ms) - 1, int(round(0.99 * (len(all_ms) - 1))))]
-    achieved = per_filter[dom]["gb_s"]
-    tag = KERNEL_OF["box" if dom.startswith("box") else "gauss"]
+    achieved = summary[dom]["gb_s"]
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
-            "kernel": tag, "filter": dom,
-            "algorithmic_bytes_per_launch": bytes_per_call, "mean_launch_ms": per_filter[dom]["mean_ms"]}
-    cpu = None if args.no_cpu else cpu_baseline(img, args.cpu_budget)
+            "kernel": per_filter[dom]["kernel"], "filter": dom,
+            "algorithmic_bytes_per_launch": per_filter[dom]["bytes"], "mean_launch_ms": summary[dom]["mean_ms"]}
+    # a plain device-to-device copy of the largest input, for context and as
+    # the calibration kernel of the HBM byte counters
+    big = max(sets[0].values(), key=lambda d: d.nbytes)
+    dst_big = ipf.DeviceImage.like(big)
+    for _ in range(2):
+        assert ev.hip.hipMemcpyAsync(dst_big.ptr, big.ptr, big.nbytes, 3, stream.handle) == 0
+    ev.record(evs[0], stream.handle)
+    for _ in range(5):
+        assert ev.hip.hipMemcpyAsync(dst_big.ptr, big.ptr, big.nbytes, 3, stream.handle) == 0
+    ev.record(evs[1], stream.handle)
+    copy_ms = ev.ms(evs[0], evs[1]) / 5
+    copy = {"bytes_per_launch": 2 * big.nbytes, "mean_ms": round(copy_ms, 4),
+            "gb_s": round(2 * big.nbytes / (copy_ms * 1e-3) / 1e9, 1)}
+    cpu = None if args.no_cpu else cpu_baseline(wd, args.cpu_budget)
     line = {
-        "metric": "Gpix/s + % HBM roofline per filter; p50/p99 latency, batched vs plain",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "Gpix/s",
         "n_gpus": 1,
@@ -199,15 +299,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded sinusoids + noise, generated in-process)",
-        "config": {"workload": "gaussianBlur sigma 1/3/8 + blur 15x15, x 5 border modes, 7680x4320x1 float32",
-                   "calls_per_step": len(calls), "image": [H, W, 1], "precision": "default (FP32)",
-                   "rotating_sets": 3},
+        "dtype": wd["dtype"],
+        "data": "synthetic (seeded project RNG, generated in-process)",
+        "config": {"workload": wd["workload"], "calls_per_step": len(calls),
+                   "precision": "default: FP64 accumulation, certified bit-identical to the reference",
+                   "rotating_sets": nsets},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "per_filter": per_filter,
+        "per_filter": summary,
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4), "what": "single filter call, HIP events"},
+        "device_copy": copy,
     }
     print(json.dumps(line), flush=True)
 

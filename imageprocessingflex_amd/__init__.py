@@ -183,6 +183,32 @@ def last_fixups() -> int:
     return n.value
 
 
+def autotune(enable: bool = True) -> None:
+    """Turns launch-shape measurement on the first large call of each shape class on or off."""
+    _lib.check(_lib.load().ipfg_autotune(1 if enable else 0))
+
+
+def tune_export() -> str:
+    """The autotuner's table as text (one shape class per line)."""
+    lib = _lib.load()
+    need = C.c_size_t()
+    _lib.check(lib.ipfg_tune_export(None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    _lib.check(lib.ipfg_tune_export(buf, need.value, C.byref(need)))
+    return buf.value.decode()
+
+
+def tune_import(text: str) -> int:
+    """Adds table lines (as written by tune_export); returns their number."""
+    n = C.c_int()
+    _lib.check(_lib.load().ipfg_tune_import(text.encode(), C.byref(n)))
+    return n.value
+
+
+def tune_clear() -> None:
+    _lib.check(_lib.load().ipfg_tune_clear())
+
+
 def gaussian_kernel_1d(sigma: float, ksize: int = 0) -> np.ndarray:
     """gaussianKernel1D (filter.hpp:129-145)."""
     lib = _lib.load()

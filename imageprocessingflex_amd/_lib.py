@@ -46,6 +46,10 @@ SIGNATURES = {
     "ipfg_last_error": (C.c_char_p, []),
     "ipfg_abi_version": (_I, []),
     "ipfg_last_fixups": (_I, [_P(C.c_longlong)]),
+    "ipfg_autotune": (_I, [_I]),
+    "ipfg_tune_export": (_I, [C.c_char_p, C.c_size_t, _P(C.c_size_t)]),
+    "ipfg_tune_import": (_I, [C.c_char_p, _P(_I)]),
+    "ipfg_tune_clear": (_I, []),
     "ipfg_set_num_threads": (None, [_I]),
     "ipfg_get_num_threads": (_I, []),
     "ipfg_malloc": (_I, [_P(_V), C.c_size_t]),

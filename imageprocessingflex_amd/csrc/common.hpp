@@ -139,22 +139,19 @@ __device__ inline bool needs_exact(double r, double bound, double lo, double hi)
     return fr <= bound || fr >= 1.0 - bound;
 }
 
-// Same question for a float32 output: static_cast<float>(v) is constant
-// between the midpoints of r's float neighbours. The midpoint on r's side of
-// f = (float)r is half a float spacing away: ulp(f) / 2 above, and below
-// either ulp(f) / 2 or, for a power of two, ulp(f) / 4.
+// Same question for a float32 output. Rounding to float is monotone, so v's
+// float is fixed when both ends of [r - b, r + b] round to the same float
+// (compared as bits, so -0 and +0 differ). The ends are computed in FP64:
+// widening the bound by 4.5e-16 |r| keeps each rounded end outside the exact
+// interval whenever b <= 1.5 |r|; a wider interval spans zero, its ends get
+// opposite signs and the sample is recomputed. A bound equal to the callers'
+// 1e-300 floor means every term and the offset were exact zeros: both the FMA
+// chain and the reference's chain then give +0.
 __device__ inline bool needs_exact_f32(double r, double bound) {
-    const float f = static_cast<float>(r);
-    const unsigned bits = __float_as_uint(f);
-    const int e = static_cast<int>((bits >> 23) & 0xffu);
-    if (e == 0xff || !(bound < 1e300)) return true;  // inf or NaN
-    const double side = r - static_cast<double>(f);
-    const bool toward_zero = (side < 0.0) == (f > 0.0f);
-    const bool halved = toward_zero && (bits & 0x7fffffu) == 0u && e > 1;
-    // 2^(max(e, 1) - 151) is half a spacing; one more halving on the short side
-    const int ee = (e > 1 ? e : 1) - (halved ? 1 : 0);
-    const double half = __longlong_as_double(static_cast<long long>(ee + 872) << 52);
-    return fabs(side) + bound >= half;
+    if (!(fabs(r) < 1e300) || !(bound < 1e300)) return true;  // inf or NaN
+    if (bound <= 1e-300) return false;
+    const double b = fma(fabs(r), 4.5e-16, bound);
+    return __float_as_uint(static_cast<float>(r - b)) != __float_as_uint(static_cast<float>(r + b));
 }
 
 // ------------------------------------------------------------------ wave helpers

@@ -1,6 +1,8 @@
 // Launch interfaces between the C-ABI runtime (runtime.hip) and the kernels.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace ipfg {
@@ -22,8 +24,11 @@ struct SepLaunch {
     double bcoef, bdelta, xbound;
     FixList fix;
     double* tmp;  // MODE_EXACT: (H + kh - 1) * W * cn doubles
+    int sh;       // strip height of the fast kernels (0: built-in choice)
 };
 hipError_t launch_sep(const SepLaunch& a, hipStream_t s);
+// Strip heights worth measuring for an image of height h.
+std::vector<int> sep_sh_candidates(int h);
 // The fast separable kernel supports this channel count and kernel size.
 bool sep_fast_fits(int cn, int kw, int kh);
 size_t sep_exact_tmp_bytes(int w, int h, int cn, int kh);

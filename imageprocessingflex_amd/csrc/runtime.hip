@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "tuner.hpp"
 
 namespace ipfg {
 namespace {
@@ -256,6 +257,7 @@ private:
     size_t total_ = 0;
 };
 ParamCache g_params;
+Tuner g_tuner;
 
 // ------------------------------------------------------------------ fix-up state
 // Each stream owns the counter and workgroup ticket of its certified
@@ -518,7 +520,18 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         a.xbound = input_bound(src->dtype, bval, border);
         a.fix = sc.fixlist();
     }
-    if (a.mode == MODE_EXACT) a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
+    if (a.mode == MODE_EXACT) {
+        a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
+    } else {
+        const TuneKey key{"sep", src->dtype, dst->dtype, src->channels, kw, kh, a.mode,
+                          ceil_log2(src->width), ceil_log2(src->height)};
+        a.sh = g_tuner.pick(key, static_cast<long long>(src->width) * src->height * src->channels,
+                            sep_sh_candidates(src->height), s, [&](int v) {
+                                SepLaunch t = a;
+                                t.sh = v;
+                                return launch_sep(t, s);
+                            });
+    }
     HIPCALL(launch_sep(a, s));
     if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
@@ -773,6 +786,32 @@ int ipfg_last_fixups(long long* count) {
         abi_check(count != nullptr, "null count");
         *count = (t_ctx.fix_valid && t_ctx.fix_host) ? static_cast<long long>(*t_ctx.fix_host) : 0;
     });
+}
+
+int ipfg_autotune(int enable) {
+    return guarded([&] { g_tuner.set_enabled(enable != 0); });
+}
+
+int ipfg_tune_export(char* buf, size_t capacity, size_t* needed) {
+    return guarded([&] {
+        abi_check(needed != nullptr, "null length");
+        const std::string t = g_tuner.export_text();
+        *needed = t.size() + 1;
+        if (buf && capacity >= t.size() + 1) std::memcpy(buf, t.c_str(), t.size() + 1);
+    });
+}
+
+int ipfg_tune_import(const char* text, int* entries) {
+    return guarded([&] {
+        abi_check(text != nullptr, "null text");
+        const int n = g_tuner.import_text(text);
+        abi_check(n >= 0, "malformed tuning table");
+        if (entries) *entries = n;
+    });
+}
+
+int ipfg_tune_clear(void) {
+    return guarded([&] { g_tuner.clear(); });
 }
 
 void ipfg_set_num_threads(int n) { g_threads = n < 0 ? 0 : n; }

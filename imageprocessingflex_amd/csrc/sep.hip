@@ -100,6 +100,13 @@ hipError_t launch_zero_counter(unsigned long long* count, unsigned int* done, hi
     return hipGetLastError();
 }
 
+std::vector<int> sep_sh_candidates(int h) {
+    std::vector<int> c;
+    for (int v : {16, 32, 48, 64, 96, 128, 192, 256, 384, 512})
+        if (v < 2 * h) c.push_back(v);
+    return c;
+}
+
 size_t sep_exact_tmp_bytes(int w, int h, int cn, int kh) {
     return sizeof(double) * static_cast<size_t>(h + kh - 1) * static_cast<size_t>(w) * cn;
 }
@@ -135,6 +142,7 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     a.bdelta = l.bdelta;
     a.xbound = l.xbound;
     a.fix = l.fix;
+    a.SH = l.sh;
     const int cn = l.src.cn;
     const bool planar = cn == 1 || cn == 3 || cn == 4;
     if (planar && l.src.dt == IPFG_U8 && (l.dst.dt == IPFG_U8 || l.dst.dt == IPFG_F32)) return plane_u8(a, cert, s);

@@ -635,10 +635,12 @@ hipError_t sep_fast_launch(SepArgs a, hipStream_t s) {
     if (lds > 156 * 1024 || xs > 64 * 8) return hipErrorInvalidConfiguration;
     // Strip height: long strips amortise the kh - 1 history rows, but the
     // grid must still cover the 256 CUs several times over.
-    const long long gx = (a.src.w + TX - 1) / TX;
-    int SH = 512;
-    while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
-    a.SH = SH;
+    if (a.SH <= 0) {
+        const long long gx = (a.src.w + TX - 1) / TX;
+        int SH = 512;
+        while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
+        a.SH = SH;
+    }
     if (xs <= 128) return sep_fast_launch_v<Ti, Acc, CN, 2, CERT>(a, lds, s);
     return sep_fast_launch_v<Ti, Acc, CN, 8, CERT>(a, lds, s);
 }
@@ -666,10 +668,12 @@ hipError_t plane_launch(SepArgs a, hipStream_t s) {
     const size_t lds = plane_lds_bytes(CN, a.kw, a.kh, sizeof(Acc));
     const int xs = (PTX + a.kw - 1) * CN;
     if (lds > 156 * 1024 || xs > 64 * 8) return hipErrorInvalidConfiguration;
-    const long long gx = (a.src.w + PTX - 1) / PTX;
-    int SH = 512;
-    while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
-    a.SH = SH;
+    if (a.SH <= 0) {  // long strips amortise the kh - 1 halo rows; keep >= 2048 workgroups
+        const long long gx = (a.src.w + PTX - 1) / PTX;
+        int SH = 512;
+        while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
+        a.SH = SH;
+    }
     if (xs <= 128) return plane_launch_v<Ti, To, Acc, CN, 2, CERT>(a, lds, s);
     return plane_launch_v<Ti, To, Acc, CN, 8, CERT>(a, lds, s);
 }

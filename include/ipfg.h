@@ -54,11 +54,14 @@ typedef enum ipfg_border {
 
 /* Arithmetic used by the linear filters and the bilateral filter. */
 typedef enum ipfg_precision {
-    /* FP32 arithmetic on the GPU. Integer outputs are certified: every sample
-     * whose FP32 value lies within the FP32 error bound of a rounding tie is
-     * recomputed in FP64 in the reference's operation order, so integer
-     * outputs are bit-identical to the reference. Float outputs carry FP32
-     * rounding error (bounded, see DESIGN). FP64 images always take EXACT. */
+    /* FP64 fused multiply-add accumulation, certified: every output whose
+     * FP64 value lies within the accumulated error bound of a rounding
+     * boundary of the output type (an integer's half-way point, a float's
+     * midpoint) is recomputed in FP64 in the reference's operation order by
+     * the same launch. Integer and float32 outputs are bit-identical to the
+     * reference; float64 outputs take EXACT. Box filters on float inputs use
+     * window sums instead of the reference's running sums and carry FP64
+     * rounding error. */
     IPFG_PREC_DEFAULT = 0,
     /* FP64 in the reference's operation order for every sample, no fused
      * multiply-add: bit-identical to the reference for every pixel type. */
@@ -103,6 +106,21 @@ int ipfg_abi_version(void);
 /* Number of samples recomputed in FP64 by the last certified call of this
  * thread (device-resident calls: after the stream has been synchronised). */
 int ipfg_last_fixups(long long* count);
+
+/* Launch-shape autotuner (no counterpart in the reference, whose CPU loops
+ * have no launch shape). enable != 0: the first call of each shape class
+ * with at least 2^20 samples times the candidate launch shapes on its stream
+ * (synchronising it once) and keeps the fastest; 0: built-in choices, except
+ * for classes already in the table. Default on unless IPFG_AUTOTUNE=0. */
+int ipfg_autotune(int enable);
+/* Writes the table as text (one "<op> <tin> <tout> <cn> <kw> <kh> <mode>
+ * <log2 w> <log2 h> <value>" line per class) into buf (capacity bytes, NUL
+ * terminated when it fits); *needed receives the full length + 1. */
+int ipfg_tune_export(char* buf, size_t capacity, size_t* needed);
+/* Adds the lines of text to the table; *entries receives their number. */
+int ipfg_tune_import(const char* text, int* entries);
+/* Empties the table. */
+int ipfg_tune_clear(void);
 
 /* Thread count for host-side staging; same contract as ipf::setNumThreads /
  * getNumThreads (parallel.hpp:38-51). GPU work does not depend on it. */

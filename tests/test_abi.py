@@ -73,3 +73,21 @@ def test_reference_check_order():
     import imageprocessingflex_amd as ipf
     with pytest.raises(ipf.IpfError, match="ksize must be odd"):
         ipf.median_blur(np.zeros((3, 3), np.uint8), 2, ipf.BorderMode.CONSTANT)
+
+
+def test_tuning_table_round_trip():
+    # the table functions are host-only: no device is touched
+    import imageprocessingflex_amd as ipf
+    ipf.tune_clear()
+    assert ipf.tune_export() == ""
+    text = "sep 0 0 3 13 13 1 11 11 64\nsep 4 4 1 49 49 1 13 13 128\n"
+    assert ipf.tune_import(text) == 2
+    got = ipf.tune_export()
+    assert sorted(got.splitlines()) == sorted(text.splitlines())
+    assert ipf.tune_import("# comment\n\nsep 0 0 3 13 13 1 11 11 96\n") == 1
+    assert "sep 0 0 3 13 13 1 11 11 96" in ipf.tune_export()
+    for bad in ("sep 0 0 3 13 13 1 11 11\n", "sep 0 0 3 13 13 1 11 11 0\n", "sep a b c\n"):
+        with pytest.raises((ipf.IpfError, ipf.DeviceError), match="malformed tuning table"):
+            ipf.tune_import(bad)
+    ipf.tune_clear()
+    assert ipf.tune_export() == ""

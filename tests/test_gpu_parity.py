@@ -271,3 +271,43 @@ def test_full_c2_exact_mode_bit_exact_and_fast_within_bound(ipf):
         assert _sha(exact) == FULL[f"c2_box15_{BNAME[b]}_f32"], b
         fast = ipf.blur(d, 15, b).download()
         assert np.all(np.abs(fast.astype(np.float64) - exact) <= 1e-12 * 225 + U * np.abs(exact)), b
+
+
+# ------------------------------------------------------------- autotuner
+def test_every_strip_height_gives_the_reference_result(ipf, oracle):
+    # the autotuner may pick any candidate strip height: each must be exact
+    rng = np.random.default_rng(21)
+    cases = [(rng.integers(0, 256, (203, 301, 3), dtype=np.uint8), 2.0),
+             (rng.random((157, 263)).astype(np.float32), 3.0)]
+    try:
+        for img, sigma in cases:
+            want = oracle.gaussian_blur(img, sigma)
+            h, w = img.shape[:2]
+            cn = 1 if img.ndim == 2 else img.shape[2]
+            dt = 0 if img.dtype == np.uint8 else 4
+            k = 2 * max(1, int(3 * sigma + 0.5)) + 1
+            wl, hl = int(np.ceil(np.log2(w))), int(np.ceil(np.log2(h)))
+            for sh in (16, 32, 48, 64, 96, 128, 192, 256, 384, 512):
+                ipf.tune_clear()
+                ipf.tune_import(f"sep {dt} {dt} {cn} {k} {k} 1 {wl} {hl} {sh}\n")
+                got = ipf.gaussian_blur(img, sigma)
+                assert np.array_equal(got, want), (img.dtype, sh)
+    finally:
+        ipf.tune_clear()
+
+
+def test_autotuned_call_matches_untuned(ipf):
+    rng = np.random.default_rng(22)
+    img = rng.integers(0, 256, (1100, 1000, 1), dtype=np.uint8)
+    try:
+        ipf.autotune(False)
+        ipf.tune_clear()
+        plain = ipf.gaussian_blur(img, 2.5)
+        ipf.autotune(True)
+        tuned = ipf.gaussian_blur(img, 2.5)
+        assert "sep 0 0 1 17 17 1" in ipf.tune_export()
+        again = ipf.gaussian_blur(img, 2.5)
+        assert np.array_equal(tuned, plain) and np.array_equal(again, plain)
+    finally:
+        ipf.autotune(True)
+        ipf.tune_clear()
