@@ -22,6 +22,7 @@ struct SepLaunch {
     // MODE_CERTIFY: |FP64-FMA result - reference| <= bcoef * X + bdelta where
     // X bounds |input| (xbound, or measured per tile when xbound < 0)
     double bcoef, bdelta, xbound;
+    double kabs;  // sum|kx| * sum|ky|
     FixList fix;
     double* tmp;  // MODE_EXACT: (H + kh - 1) * W * cn doubles
     int sh;       // strip height of the fast kernels (0: built-in choice)
@@ -29,8 +30,8 @@ struct SepLaunch {
 hipError_t launch_sep(const SepLaunch& a, hipStream_t s);
 // Strip heights worth measuring for an image of height h.
 std::vector<int> sep_sh_candidates(int h);
-// The fast separable kernel supports this channel count and kernel size.
-bool sep_fast_fits(int cn, int kw, int kh);
+// The fast separable kernel supports this call (channels, kernel size, types).
+bool sep_fast_fits(int cn, int kw, int kh, int src_dt, int dst_dt, bool cert);
 size_t sep_exact_tmp_bytes(int w, int h, int cn, int kh);
 
 // ---------------------------------------------------------------- box

@@ -484,7 +484,9 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
     a.delta = delta;
     a.bval = bval;
     a.mode = resolve_mode(precision, src->dtype, dst->dtype);
-    if (a.mode != MODE_EXACT && !sep_fast_fits(src->channels, kw, kh)) a.mode = MODE_EXACT;
+    if (a.mode != MODE_EXACT &&
+        !sep_fast_fits(src->channels, kw, kh, src->dtype, dst->dtype, a.mode == MODE_CERTIFY))
+        a.mode = MODE_EXACT;
     // Parameter block: kx and ky as double, then as float; every array is
     // followed by kWeightPad zeros (the kernels prefetch weights ahead).
     constexpr int kWeightPad = 16;
@@ -514,6 +516,7 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         // the exact value; a Constant value staged as float adds its own
         // rounding. Factors are padded.
         a.bcoef = 2.02 * (kw + kh + 4) * kU64 * sx * sy;
+        a.kabs = sx * sy;
         a.bdelta = 2.02 * (kh + 2) * kU64 * std::fabs(delta) + 1e-300;
         if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
             a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sx * sy;

@@ -89,10 +89,18 @@ int sep_grid(size_t n) {
 
 using namespace sepk;
 
-bool sep_fast_fits(int cn, int kw, int kh) {
-    if (cn == 1 || cn == 3 || cn == 4)
-        return plane_lds_bytes(cn, kw, kh, sizeof(double)) <= 156 * 1024 && (PTX + kw - 1) * cn <= 64 * 8;
-    return sep_lds_bytes(32, cn, kw, kh, sizeof(double)) <= 156 * 1024 && (32 + kw - 1) * cn <= 64 * 8;
+static bool sep_planar(int cn, int src_dt, int dst_dt) {
+    return (cn == 1 || cn == 3 || cn == 4) &&
+           ((src_dt == IPFG_U8 && (dst_dt == IPFG_U8 || dst_dt == IPFG_F32)) || (src_dt == IPFG_F32 && dst_dt == IPFG_F32));
+}
+
+bool sep_fast_fits(int cn, int kw, int kh, int src_dt, int dst_dt, bool cert) {
+    const size_t acc = cert ? sizeof(double) : sizeof(float);
+    if (sep_planar(cn, src_dt, dst_dt)) {
+        const size_t hr = (cert && !plane_ring_is_float(dst_dt)) ? sizeof(double) : sizeof(float);
+        return plane_lds_bytes(cn, kw, kh, hr) <= 156 * 1024 && (PTX + kw - 1) * cn <= 64 * 8;
+    }
+    return sep_lds_bytes(32, cn, kw, kh, acc) <= 156 * 1024 && (32 + kw - 1) * cn <= 64 * 8;
 }
 
 hipError_t launch_zero_counter(unsigned long long* count, unsigned int* done, hipStream_t s) {
@@ -144,9 +152,27 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     a.fix = l.fix;
     a.SH = l.sh;
     const int cn = l.src.cn;
-    const bool planar = cn == 1 || cn == 3 || cn == 4;
-    if (planar && l.src.dt == IPFG_U8 && (l.dst.dt == IPFG_U8 || l.dst.dt == IPFG_F32)) return plane_u8(a, cert, s);
-    if (planar && l.src.dt == IPFG_F32 && l.dst.dt == IPFG_F32) return plane_f32(a, cert, s);
+    const bool planar = sep_planar(cn, l.src.dt, l.dst.dt);
+    if (planar && l.src.dt == IPFG_U8) {
+        if (cert && plane_ring_is_float(l.dst.dt)) {
+            // the ring holds the FP64 horizontal sums rounded to float
+            a.bcoef += 1.01 * 5.9604644775390625e-08 * l.kabs;
+            if (l.dst.dt == IPFG_U8) {
+                const hipError_t e = cn == 1 ? plane_u8_k1(a, s) : cn == 3 ? plane_u8_k3(a, s) : plane_u8_k4(a, s);
+                if (e != hipErrorNotSupported) return e;
+                (void)hipGetLastError();
+            }
+        }
+        return plane_u8(a, cert, s);
+    }
+    if (planar) {  // float32 -> float32
+        if (cert && cn == 1) {
+            const hipError_t e = plane_f32_k1(a, s);
+            if (e != hipErrorNotSupported) return e;
+            (void)hipGetLastError();
+        }
+        return plane_f32(a, cert, s);
+    }
     return generic_any(a, cert, s);
 }
 

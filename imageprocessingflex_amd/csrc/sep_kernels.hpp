@@ -351,8 +351,17 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
 // Channel counts 1, 3 and 4: staged rows are de-interleaved into one LDS plane
 // per channel, so every plane is a contiguous row of pixels and the
 // horizontal pass reads its windows with 16-byte LDS loads and writes four
-// results with one 16-byte store. Ring rows are addressed once per group of
-// M rows (groups never straddle the ring's wrap) with immediate offsets.
+// results with one store. Ring rows are addressed once per group of M rows
+// (groups never straddle the ring's wrap) with immediate offsets.
+//
+// KS > 0 specialises the kernel for kw = kh = KS: every tap loop is unrolled,
+// the tile geometry is constant and weights become scalar loads at fixed
+// offsets; KS = 0 reads the sizes at run time.
+//
+// Hr is the type of the horizontally filtered rows kept in the ring. For
+// integer outputs a certified launch keeps them as float: the FP64 sums are
+// rounded once (relative error 2^-24), which the caller adds to the bound,
+// and the ring takes half the LDS.
 constexpr int PTX = 64;  // tile width in pixels (per plane)
 
 struct PlaneTile {
@@ -363,8 +372,8 @@ struct PlaneTile {
     int RPL;    // ring plane size (elements): R * PTX + pad
 };
 
-__host__ __device__ inline PlaneTile plane_tile(int cn, int kw, int kh) {
-    PlaneTile t;
+__host__ __device__ constexpr PlaneTile plane_tile(int cn, int kw, int kh) {
+    PlaneTile t{};
     t.XS = (PTX + kw - 1) * cn;
     t.XPP = ((PTX + kw - 1 + P + 3) / 4) * 4;
     t.K0 = (kh - 1 + CH - 1) / CH;
@@ -375,11 +384,14 @@ __host__ __device__ inline PlaneTile plane_tile(int cn, int kw, int kh) {
     return t;
 }
 
-__host__ inline size_t plane_lds_bytes(int cn, int kw, int kh, size_t acc_size) {
+__host__ inline size_t plane_lds_bytes(int cn, int kw, int kh, size_t ring_elem) {
     const PlaneTile t = plane_tile(cn, kw, kh);
-    const size_t ring = ((static_cast<size_t>(cn) * t.RPL * acc_size + 15) / 16) * 16;
+    const size_t ring = ((static_cast<size_t>(cn) * t.RPL * ring_elem + 15) / 16) * 16;
     return ring + sizeof(float) * static_cast<size_t>(cn) * CH * t.XPP;
 }
+
+// Samples per lane of the stager for a kernel width (0: run time).
+constexpr int plane_sj(int cn, int ks) { return ks ? ((PTX + ks - 1) * cn + 63) / 64 : 0; }
 
 template <class To>
 __device__ __forceinline__ void put(char* p, double r) {
@@ -397,20 +409,32 @@ __device__ __forceinline__ bool doubtful(double r, double bound) {
                            static_cast<double>(std::numeric_limits<To>::max()));
 }
 
-template <class Ti, class To, class Acc, int CN, int SJ, bool CERT>
+template <class Hr, class Acc>
+__device__ __forceinline__ void store_h4(Hr* p, const Acc (&v)[4]) {
+    if constexpr (sizeof(Hr) == 8) {
+        reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+        reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+    } else {
+        *reinterpret_cast<float4*>(p) = make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                                                    static_cast<float>(v[2]), static_cast<float>(v[3]));
+    }
+}
+
+template <class Ti, class To, class Acc, class Hr, int CN, int SJ, bool CERT, int KS>
 __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc* __restrict__ kx,
                                                            const Acc* __restrict__ ky) {
     extern __shared__ __align__(16) unsigned char sep_smem[];
-    constexpr int G = PTX / P;
+    constexpr int G = PTX / P;                  // horizontal tasks per row
+    constexpr int U = KS ? 2 : 1;              // unroll of the tap loops (more hoists weights into VGPRs)
+    static_assert(CH * G == kBlock && (CH / M) * PTX == kBlock, "one task per thread per plane");
     const int W = a.src.w, H = a.src.h;
-    const int kw = a.kw, kh = a.kh;
+    const int kw = KS ? KS : a.kw, kh = KS ? KS : a.kh;
     const PlaneTile T = plane_tile(CN, kw, kh);
-    Acc* ring = reinterpret_cast<Acc*>(sep_smem);
-    float* in_buf = reinterpret_cast<float*>(
-        sep_smem + ((static_cast<size_t>(CN) * T.RPL * sizeof(Acc) + 15) / 16) * 16);
+    Hr* ring = reinterpret_cast<Hr*>(sep_smem);
+    float* in_buf = reinterpret_cast<float*>(sep_smem + ((static_cast<size_t>(CN) * T.RPL * sizeof(Hr) + 15) / 16) * 16);
     __shared__ unsigned int tile_max_bits;
     const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * a.SH;
     const int rows_out = min(a.SH, H - y0);
     const int nh = rows_out + kh - 1;
@@ -422,18 +446,24 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
     const bool measure = CERT && a.xbound < 0;
     if (tid == 0) tile_max_bits = 0u;
     float my_max = 0.f;
-    Stager<Ti, SJ> st;
     const int gx0 = x0 - a.ax, gy0 = y0 - a.ay;
     const int plane_in = CH * T.XPP;
     const int msk = T.R - 1;
+    // horizontal task of this thread: row hr of the chunk, pixels [hg * P, hg * P + P)
+    const int hr = tid / G, hg = tid % G;
+    // vertical task: column vx, rows [vv * M, vv * M + M) of the chunk (vv is wave-uniform)
+    const int vx = lane, vv = wave;
 
-    st.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0, min(CH, nh), T.XS, tid);
-    for (int k = 0; k < nstep; ++k) {
+    // Two stagers alternate so that the loads of the next two chunks are in
+    // flight while a chunk is filtered.
+    Stager<Ti, SJ> sa, sb;
+    sa.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0, min(CH, nh), T.XS, tid);
+    if (nhstep > 1) sb.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0 + CH, min(CH, nh - CH), T.XS, tid);
+    auto step = [&](Stager<Ti, SJ>& st, const int k) {
         const int r0 = k * CH;
         const int nr = min(CH, nh - r0);
         if (nr > 0) {
             // de-interleave: sample s of staged row r -> plane s % CN, pixel s / CN
-            const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = wave + 4 * i;
@@ -461,17 +491,14 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
             if (measure) wave_max_into(&tile_max_bits, my_max);
         }
         __syncthreads();
-        if (k + 1 < nhstep)
-            st.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0 + r0 + CH, min(CH, nh - r0 - CH), T.XS, tid);
-        // ---- horizontal taps (reference order: i ascending from 0) into the ring
-        if (nr > 0) {
-            for (int t = tid; t < CH * G * CN; t += kBlock) {
-                const int g = t % G;
-                const int rest = t / G;
-                const int c = rest % CN;
-                const int r = rest / CN;
-                if (r >= nr) continue;
-                const float* base = in_buf + c * plane_in + r * T.XPP + g * P;
+        if (k + 2 < nhstep)  // two chunks in flight: this one's registers take chunk k + 2
+            st.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0 + r0 + 2 * CH, min(CH, nh - r0 - 2 * CH), T.XS,
+                    tid);
+        // ---- horizontal taps (i ascending from 0) into the ring
+        if (nr > 0 && hr < nr) {
+#pragma unroll 1
+            for (int c = 0; c < CN; ++c) {
+                const float* base = in_buf + c * plane_in + hr * T.XPP + hg * P;
                 Acc acc[P], win[P];
                 {
                     const float4 f = *reinterpret_cast<const float4*>(base);
@@ -485,6 +512,7 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                 Acc kc[4];
                 load4(kx, 0, kc);
                 int i0 = 0;
+#pragma unroll U
                 for (; i0 + P <= kw; i0 += P) {
                     Acc kn[4];
                     load4(kx, i0 + P, kn);
@@ -513,48 +541,41 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                         }
                     }
                 }
-                Acc* hrow = ring + c * T.RPL + ((r0 + r) & msk) * PTX + g * P;
-                if constexpr (sizeof(Acc) == 8) {
-                    reinterpret_cast<double2*>(hrow)[0] = make_double2(acc[0], acc[1]);
-                    reinterpret_cast<double2*>(hrow)[1] = make_double2(acc[2], acc[3]);
-                } else {
-                    *reinterpret_cast<float4*>(hrow) = make_float4(acc[0], acc[1], acc[2], acc[3]);
-                }
+                store_h4(ring + c * T.RPL + ((r0 + hr) & msk) * PTX + hg * P, acc);
             }
         }
         __syncthreads();
         // ---- vertical taps for output rows [(k - K0) * CH, (k - K0 + 1) * CH)
         const int ob = (k - T.K0) * CH;
-        if (ob < 0) continue;
+        if (ob < 0) return;
+        const int yr = ob + vv * M;  // first output row of this task (strip-relative), wave-uniform
+        if (yr >= rows_out) return;
         double bound = 0.0;
         if (CERT) bound = a.bcoef * (measure ? static_cast<double>(__uint_as_float(tile_max_bits)) : a.xbound) + a.bdelta;
-        for (int t = tid; t < (CH / M) * CN * PTX; t += kBlock) {
-            const int x = t % PTX;
-            const int rest = t / PTX;
-            const int c = rest % CN;
-            const int v = rest / CN;
-            const int yr = ob + v * M;  // first output row of this task (strip-relative), multiple of M
-            if (yr >= rows_out) continue;
-            const Acc* plane = ring + c * T.RPL + x;
+        const int xg = x0 + vx;
+#pragma unroll 1
+        for (int c = 0; c < CN; ++c) {
+            const Hr* plane = ring + c * T.RPL + vx;
             Acc acc[M], win[M];
             {
-                const Acc* g0 = plane + (yr & msk) * PTX;  // rows yr .. yr + M - 1 do not wrap
+                const Hr* g0 = plane + (yr & msk) * PTX;  // rows yr .. yr + M - 1 do not wrap
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
                     acc[m] = static_cast<Acc>(a.delta);
-                    win[m] = g0[m * PTX];
+                    win[m] = static_cast<Acc>(g0[m * PTX]);
                 }
             }
             Acc kc[M];
             load4(ky, 0, kc);
             int j0 = 0;
+#pragma unroll U
             for (; j0 + M <= kh; j0 += M) {
                 Acc kn[M];
                 load4(ky, j0 + M, kn);
-                const Acc* gp = plane + ((yr + j0 + M) & msk) * PTX;
+                const Hr* gp = plane + ((yr + j0 + M) & msk) * PTX;
                 Acc nx[M];
 #pragma unroll
-                for (int jj = 0; jj < M; ++jj) nx[jj] = gp[jj * PTX];
+                for (int jj = 0; jj < M; ++jj) nx[jj] = static_cast<Acc>(gp[jj * PTX]);
 #pragma unroll
                 for (int jj = 0; jj < M; ++jj) {
 #pragma unroll
@@ -565,17 +586,16 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                 for (int jj = 0; jj < M; ++jj) kc[jj] = kn[jj];
             }
             if (j0 < kh) {
-                const Acc* gp = plane + ((yr + j0 + M) & msk) * PTX;
+                const Hr* gp = plane + ((yr + j0 + M) & msk) * PTX;
 #pragma unroll
                 for (int jj = 0; jj < M; ++jj) {
                     if (j0 + jj < kh) {
 #pragma unroll
                         for (int m = 0; m < M; ++m) acc[m] = fma_acc(kc[jj], win[(jj + m) % M], acc[m]);
-                        win[jj] = gp[jj * PTX];
+                        win[jj] = static_cast<Acc>(gp[jj * PTX]);
                     }
                 }
             }
-            const int xg = x0 + x;
             unsigned int dmask = 0u;  // doubtful outputs of this task
             if (xg < W) {
                 char* op = static_cast<char*>(a.dst.data) + static_cast<size_t>(y0 + yr) * a.dst.stride +
@@ -589,20 +609,23 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                     op += a.dst.stride;
                 }
             }
-            // yr and c are wave-uniform (PTX = 64 lanes share a row group and channel)
-            if (CERT) {
+            if (CERT) {  // yr and c are wave-uniform: only the column comes from lane l
                 wave_fix(a.fix, dmask, [&](int l, int m) {
                     const int xl = bcast_i(xg, l);
                     const int yl = y0 + yr + m;
                     const double e = sep_exact_wave<Ti>(a.src, xl, yl, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border,
                                                         a.delta, a.bval);
-                    if (lane_id() == l)
+                    if (lane == l)
                         put<To>(static_cast<char*>(a.dst.data) + static_cast<size_t>(yl) * a.dst.stride +
                                     (static_cast<size_t>(xl) * CN + c) * sizeof(To),
                                 e);
                 });
             }
         }
+    };
+    for (int k = 0; k < nstep; k += 2) {
+        step(sa, k);
+        if (k + 1 < nstep) step(sb, k + 1);
     }
     if (CERT) fix_finish(a.fix);
 }
@@ -645,11 +668,11 @@ hipError_t sep_fast_launch(SepArgs a, hipStream_t s) {
     return sep_fast_launch_v<Ti, Acc, CN, 8, CERT>(a, lds, s);
 }
 
-template <class Ti, class To, class Acc, int CN, int SJ, bool CERT>
+template <class Ti, class To, class Acc, class Hr, int CN, int SJ, bool CERT, int KS>
 hipError_t plane_launch_v(const SepArgs& a, size_t lds, hipStream_t s) {
-    auto kern = sep_plane_kernel<Ti, To, Acc, CN, SJ, CERT>;
+    auto kern = sep_plane_kernel<Ti, To, Acc, Hr, CN, SJ, CERT, KS>;
     if (lds > 64 * 1024) {
-        static size_t raised = 0;
+        static size_t raised = 0;  // largest dynamic LDS granted to this instantiation
         if (lds > raised) {
             const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
@@ -663,9 +686,15 @@ hipError_t plane_launch_v(const SepArgs& a, size_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <class Ti, class To, class Acc, int CN, bool CERT>
+// Ring element type of a planar launch: float for FP32 launches and for
+// certified launches with integer outputs, double for float outputs.
+template <class To, class Acc>
+using plane_hr = typename std::conditional<sizeof(Acc) == 8 && !std::is_integral<To>::value, double, float>::type;
+
+template <class Ti, class To, class Acc, int CN, bool CERT, int KS>
 hipError_t plane_launch(SepArgs a, hipStream_t s) {
-    const size_t lds = plane_lds_bytes(CN, a.kw, a.kh, sizeof(Acc));
+    using Hr = plane_hr<To, Acc>;
+    const size_t lds = plane_lds_bytes(CN, a.kw, a.kh, sizeof(Hr));
     const int xs = (PTX + a.kw - 1) * CN;
     if (lds > 156 * 1024 || xs > 64 * 8) return hipErrorInvalidConfiguration;
     if (a.SH <= 0) {  // long strips amortise the kh - 1 halo rows; keep >= 2048 workgroups
@@ -674,14 +703,39 @@ hipError_t plane_launch(SepArgs a, hipStream_t s) {
         while (SH > CH && gx * ((a.src.h + SH - 1) / SH) < 2048) SH /= 2;
         a.SH = SH;
     }
-    if (xs <= 128) return plane_launch_v<Ti, To, Acc, CN, 2, CERT>(a, lds, s);
-    return plane_launch_v<Ti, To, Acc, CN, 8, CERT>(a, lds, s);
+    if constexpr (KS > 0) {
+        return plane_launch_v<Ti, To, Acc, Hr, CN, plane_sj(CN, KS), CERT, KS>(a, lds, s);
+    } else {
+        if (xs <= 128) return plane_launch_v<Ti, To, Acc, Hr, CN, 2, CERT, 0>(a, lds, s);
+        return plane_launch_v<Ti, To, Acc, Hr, CN, 8, CERT, 0>(a, lds, s);
+    }
 }
 
+// Kernel widths with a specialised planar kernel (kw = kh): every odd size
+// from 3 to 49, which covers gaussianBlur for sigma in [0.5, 8].
+constexpr int kSpecMin = 3, kSpecMax = 49;
 
-// Instantiation groups (one per translation unit).
+template <class Ti, class To, class Acc, int CN, bool CERT, int KS = kSpecMin>
+hipError_t plane_launch_k(const SepArgs& a, hipStream_t s) {
+    if constexpr (KS > kSpecMax) {
+        return hipErrorNotSupported;
+    } else {
+        if (a.kw == KS) return plane_launch<Ti, To, Acc, CN, CERT, KS>(a, s);
+        return plane_launch_k<Ti, To, Acc, CN, CERT, KS + 2>(a, s);
+    }
+}
+
+// True when the certified launch of this call keeps float rows in the ring.
+inline bool plane_ring_is_float(int dst_dt) { return dtype_is_int(dst_dt); }
+
+// Instantiation groups (one per translation unit). The _k groups return
+// hipErrorNotSupported for sizes without a specialised kernel.
 hipError_t plane_u8(const SepArgs& a, bool cert, hipStream_t s);
 hipError_t plane_f32(const SepArgs& a, bool cert, hipStream_t s);
+hipError_t plane_u8_k1(const SepArgs& a, hipStream_t s);
+hipError_t plane_u8_k3(const SepArgs& a, hipStream_t s);
+hipError_t plane_u8_k4(const SepArgs& a, hipStream_t s);
+hipError_t plane_f32_k1(const SepArgs& a, hipStream_t s);
 hipError_t generic_any(const SepArgs& a, bool cert, hipStream_t s);
 
 }  // namespace sepk

@@ -1,5 +1,5 @@
-// Instantiates the planar separable kernels for uint8 images
-// (uint8 -> uint8 and uint8 -> float32 outputs).
+// Instantiates the planar separable kernels for uint8 images with run-time
+// kernel sizes (uint8 -> uint8 and uint8 -> float32 outputs).
 #include "sep_kernels.hpp"
 
 namespace ipfg {
@@ -8,9 +8,9 @@ namespace sepk {
 template <class To, class Acc, bool CERT>
 static hipError_t by_cn(const SepArgs& a, hipStream_t s) {
     switch (a.src.cn) {
-        case 1: return plane_launch<uint8_t, To, Acc, 1, CERT>(a, s);
-        case 3: return plane_launch<uint8_t, To, Acc, 3, CERT>(a, s);
-        case 4: return plane_launch<uint8_t, To, Acc, 4, CERT>(a, s);
+        case 1: return plane_launch<uint8_t, To, Acc, 1, CERT, 0>(a, s);
+        case 3: return plane_launch<uint8_t, To, Acc, 3, CERT, 0>(a, s);
+        case 4: return plane_launch<uint8_t, To, Acc, 4, CERT, 0>(a, s);
     }
     return hipErrorInvalidValue;
 }

@@ -311,3 +311,21 @@ def test_autotuned_call_matches_untuned(ipf):
     finally:
         ipf.autotune(True)
         ipf.tune_clear()
+
+
+# ------------------------------------------------------------- size-specialised kernels
+@pytest.mark.parametrize("ks", list(range(3, 51, 2)))
+def test_size_specialised_kernels_vs_oracle(ipf, oracle, ks):
+    # gaussian windows of every specialised size, on uint8 1/3/4-channel and
+    # float32 images, each under a different border mode
+    rng = np.random.default_rng(100 + ks)
+    k = oracle.gaussian_kernel_1d(0.0, ks)
+    cases = [rng.integers(0, 256, (83, 150), dtype=np.uint8),
+             rng.integers(0, 256, (61, 97, 3), dtype=np.uint8),
+             rng.integers(0, 256, (45, 70, 4), dtype=np.uint8),
+             rng.random((90, 130)).astype(np.float32)]
+    for i, img in enumerate(cases):
+        b = (ks // 2 + i) % 5
+        got = ipf.sep_filter2d(img, k, k, b)
+        want = oracle.sep_filter(img, k, k, border=b)
+        assert np.array_equal(got, want), (img.dtype, img.shape, b)
