@@ -33,8 +33,9 @@ inline Precision& precisionSetting() {
     thread_local Precision p = Precision::Default;
     return p;
 }
-/// Precision used by this thread's filter calls (Default: bit-exact integer
-/// outputs, FP32-accurate float outputs).
+/// Precision used by this thread's filter calls (Default: FP64 accumulation
+/// certified against the reference's rounding, bit-identical integer and
+/// float32 outputs).
 inline void setPrecision(Precision p) { precisionSetting() = p; }
 inline Precision getPrecision() { return precisionSetting(); }
 

@@ -20,7 +20,7 @@
 //   median .................. include/ipf/filter.hpp:269-332
 //   bilateral ............... include/ipf/filter.hpp:334-387
 //
-// Pinning: every function here is pinned by tests/test_oracle_*.py against the
+// Pinning: every function here is pinned by tests/test_oracle.py against the
 // reference-built fixtures in tests/golden/ and against closed forms, brute force
 // and the reference's own known-answer tests.
 #include <algorithm>
