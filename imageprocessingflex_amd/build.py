@@ -49,15 +49,28 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> Path:
-    """Compiles every csrc/*.hip for gfx950 and links libipfg.so. Always rebuilds."""
+def _stale(src: Path, newest_header: float) -> bool:
+    obj = BUILD / (src.stem + ".o")
+    return not obj.exists() or obj.stat().st_mtime < max(src.stat().st_mtime, newest_header)
+
+
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> Path:
+    """Compiles every csrc/*.hip for gfx950 and links libipfg.so. Rebuilds
+    everything unless `incremental` (then only sources older than their
+    object or than any header are recompiled)."""
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     BUILD.mkdir(exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
+    headers = list(CSRC.glob("*.hpp")) + [PKG.parent / "include" / "ipfg.h"]
+    newest = max(h.stat().st_mtime for h in headers if h.exists())
+    todo = [s for s in srcs if not incremental or _stale(s, newest)]
+    # largest translation units first: they bound the wall time
+    todo.sort(key=lambda p: -p.stat().st_size if "_k" not in p.stem else -10**9)
+    jobs = jobs or min(max(1, len(todo)), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [BUILD / (s.stem + ".o") for s in srcs]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *map(str, objs),
            "-Wl,-rpath,/opt/rocm/lib", "-pthread"]
@@ -71,4 +84,4 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
     Stager<Ti, SJ> st;
     auto conv = [&](Ti v) {
         const float f = static_cast<float>(v);
-        if (measure) my_max = fmaxf(my_max, (f == f) ? fabsf(f) : INFINITY);
+        if (measure) my_max = fmaxf(my_max, fabsf(f));  // NaN/inf inputs give NaN/inf sums: flagged anyway
         return f;
     };
     const int ncol = TX * cn;
@@ -434,7 +434,8 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
     float* in_buf = reinterpret_cast<float*>(sep_smem + ((static_cast<size_t>(CN) * T.RPL * sizeof(Hr) + 15) / 16) * 16);
     __shared__ unsigned int tile_max_bits;
     const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // the wave index is uniform: keep it (and what derives from it) in scalar registers
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * a.SH;
     const int rows_out = min(a.SH, H - y0);
     const int nh = rows_out + kh - 1;
@@ -474,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
                     if (s < T.XS) {
                         const int px = s / CN, c = s - px * CN;
                         const float f = static_cast<float>(st.val[i][j]);
-                        if (measure) my_max = fmaxf(my_max, (f == f) ? fabsf(f) : INFINITY);
+                        if (measure) my_max = fmaxf(my_max, fabsf(f));  // NaN/inf inputs give NaN/inf sums: flagged anyway
                         in_buf[c * plane_in + r * T.XPP + px] = f;
                     }
                 }

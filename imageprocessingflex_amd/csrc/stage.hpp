@@ -22,7 +22,7 @@ struct Stager {
 
     __device__ __forceinline__ void load(const Ti* __restrict__ src, size_t sstride, int W, int H, int cn,
                                          int border, Ti bval, int gx0, int gy0, int nr, int xs, int tid) {
-        const int wave = tid >> 6, lane = tid & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
         const bool inside = gx0 >= 0 && (gx0 * cn + xs) <= W * cn && gy0 >= 0 && gy0 + nr <= H;
         if (inside) {
             const char* p0 = reinterpret_cast<const char*>(src) + static_cast<size_t>(gy0 + wave) * sstride +
@@ -60,7 +60,7 @@ struct Stager {
 
     template <class Tl, class Conv>
     __device__ __forceinline__ void store(Tl* lds, int pitch, int nr, int xs, int tid, Conv conv) const {
-        const int wave = tid >> 6, lane = tid & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = wave + 4 * i;
