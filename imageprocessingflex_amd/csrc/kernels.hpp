@@ -60,9 +60,15 @@ struct F2dLaunch {
     double bcoef, bdelta, xbound;  // MODE_CERTIFY bound: bcoef * X + bdelta
     FixList fix;
     int precision;        // IPFG_PREC_* for the MFMA variants
-    const void* mfma_w;   // packed split weights for the MFMA path (device)
+    const void* mfma_w;   // split weight fragments for the MFMA path (device)
+    double mfma_wscale;   // power of two the fragments' weights were scaled by
 };
 hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s);
+// MFMA implicit-GEMM filter2D (precision BF16X3 / FP16X2).
+hipError_t launch_filter2d_mfma(const F2dLaunch& a, hipStream_t s);
+size_t f2d_mfma_lds_bytes(int kw, int kh, bool split_input);
+int f2d_mfma_pairs(int kw, int kh);
+void f2d_mfma_fragments(const double* kd, int kw, int kh, int precision, std::vector<uint16_t>& out, double& wscale);
 
 // ---------------------------------------------------------------- median
 struct MedianLaunch {

@@ -584,14 +584,41 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
     check_image(src, "null src image");
     check_image(dst, "null dst image");
     abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
-    if (precision == IPFG_PREC_BF16X3 || precision == IPFG_PREC_FP16X2)
-        throw Unsupported{"ipfg: split-precision MFMA filter2D is not available in this build"};
+    const bool mfma = precision == IPFG_PREC_BF16X3 || precision == IPFG_PREC_FP16X2;
+    if (mfma && f2d_mfma_lds_bytes(kw, kh, src->dtype != IPFG_U8) > 156 * 1024)
+        throw Unsupported{"ipfg: kernel too large for the MFMA filter2D tile"};
     ensure_init();
     if (static_cast<long long>(src->width) * src->height == 0) return;
     const hipStream_t s = pick_stream(stream);
     CallScratch sc(s);
     Staged io(src, dst, sc);
     F2dLaunch a{};
+    if (mfma) {
+        // split-precision weight fragments in register order (cached like any parameter block)
+        std::vector<double> kd(static_cast<size_t>(kw) * kh);
+        for (size_t i = 0; i < kd.size(); ++i) kd[i] = static_cast<double>(kernel[i]);
+        std::vector<uint16_t> frag;
+        double wscale = 1.0;
+        f2d_mfma_fragments(kd.data(), kw, kh, precision, frag, wscale);
+        std::vector<char> blob(sizeof(uint16_t) * frag.size());
+        std::memcpy(blob.data(), frag.data(), blob.size());
+        a.src = io.src();
+        a.dst = io.dst();
+        a.kw = kw;
+        a.kh = kh;
+        a.ax = ax;
+        a.ay = ay;
+        a.border = border;
+        a.delta = delta;
+        a.bval = bval;
+        a.precision = precision;
+        a.mfma_w = sc.params(blob);
+        a.mfma_wscale = wscale;
+        HIPCALL(launch_filter2d_mfma(a, s));
+        t_ctx.fix_valid = false;
+        io.finish();
+        return;
+    }
     a.src = io.src();
     a.dst = io.dst();
     a.kw = kw;

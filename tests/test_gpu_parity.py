@@ -329,3 +329,45 @@ def test_size_specialised_kernels_vs_oracle(ipf, oracle, ks):
         got = ipf.sep_filter2d(img, k, k, b)
         want = oracle.sep_filter(img, k, k, border=b)
         assert np.array_equal(got, want), (img.dtype, img.shape, b)
+
+
+# ------------------------------------------------------------- MFMA filter2D (split precision)
+def _mfma_bound(kernel, xmax, precision, float_input):
+    # input split error + weight split error + FP32 accumulation over every product
+    k = np.asarray(kernel, np.float64)
+    nprod = k.size * (3 if precision == 3 else 2) + (k.size if float_input else 0)
+    xs = (2.0 ** -16 if precision == 3 else 2.0 ** -22) if float_input else 0.0
+    ws = 2.0 ** -24 if precision == 3 else 2.0 ** -22
+    return (xs + ws + 2 * nprod * 2.0 ** -24) * np.abs(k).sum() * xmax
+
+
+@pytest.mark.parametrize("precision", [3, 4], ids=["bf16x3", "fp16x2"])
+@pytest.mark.parametrize("shape,ks,anchor", [((97, 131), (31, 31), (-1, -1)), ((64, 64, 3), (7, 5), (1, 3)),
+                                             ((70, 45), (3, 3), (-1, -1)), ((33, 200), (9, 17), (8, 0))])
+def test_filter2d_mfma_error_within_bound(ipf, oracle, precision, shape, ks, anchor):
+    rng = np.random.default_rng(31)
+    kernel = rng.standard_normal((ks[1], ks[0]))
+    kernel = (kernel / kernel.sum()).astype(np.float32)
+    for dt, scale in ((np.uint8, 255), (np.float32, 1.0), (np.float32, 1e-3), (np.float32, 1e3)):
+        img = (rng.integers(0, 256, shape).astype(dt) if dt == np.uint8
+               else (rng.random(shape) * scale).astype(np.float32))
+        for b in (0, 3, 4):
+            want = oracle.filter2d(img, kernel, border=b, anchor=anchor, out_dtype=np.float64)
+            got = ipf.filter2d(img, kernel, b, anchor=anchor, out_dtype=np.float32, precision=precision)
+            err = np.abs(got.astype(np.float64) - want)
+            bound = _mfma_bound(kernel, float(np.abs(img).max()), precision, dt != np.uint8) + \
+                2.0 ** -24 * np.abs(want).max()
+            assert err.max() <= bound, (dt, scale, b, err.max(), bound)
+
+
+@pytest.mark.parametrize("precision", [3, 4], ids=["bf16x3", "fp16x2"])
+def test_filter2d_mfma_u8_output_close_to_reference(ipf, oracle, precision):
+    # uint8 -> uint8: results differ from the reference only at rounding ties
+    rng = np.random.default_rng(32)
+    img = rng.integers(0, 256, (160, 190), dtype=np.uint8)
+    kernel = rng.standard_normal((31, 31))
+    kernel = (kernel / kernel.sum()).astype(np.float32)
+    want = oracle.filter2d(img, kernel, border=3)
+    got = ipf.filter2d(img, kernel, 3, precision=precision)
+    d = np.abs(got.astype(int) - want.astype(int))
+    assert d.max() <= 1 and (d > 0).mean() < 0.01
