@@ -137,19 +137,46 @@ def _c3():
 def _c4():
     from imageprocessingflex_amd import workloads as wl
     u8, f32 = wl.c4_images(4096)
-    k, _ = wl.c4_kernel(31)
+    k, ksum = wl.c4_kernel(31)
     kp = np.ascontiguousarray(k, np.float32)
     kptr = kp.ctypes.data_as(C.POINTER(C.c_float))
     flops = 2.0 * 4096 * 4096 * 31 * 31
+    names = {0: "", 3: "_bf16x3", 4: "_fp16x2"}
+    kern = {0: "f2d_fast_kernel<T, double, false, true>", 3: "f2d_mfma_kernel<T, true, 3, XS>",
+            4: "f2d_mfma_kernel<T, false, 2, XS>"}
     calls = []
-    for name, key, dt in (("filter2d_u8", "u8", np.uint8), ("filter2d_f32", "f32", np.float32)):
-        calls.append(Call(f"{name}/reflect101", name, key, dt,
-                          lambda lib, s, d, st, kptr=kptr: lib.ipfg_filter2d(s, d, kptr, 31, 31, 15, 15, 3, 0.0, 0.0, 0, st),
-                          lambda o, im: o.filter2d(im, kp, border=3),
-                          "f2d_fast_kernel", flops=flops))
-    return {"images": {"u8": u8, "f32": f32}, "calls": calls, "dtype": "f64",
-            "workload": "filter2D dense 31x31 float32 kernel (N(0,1), sum 1), Reflect101, 4096x4096x1 uint8 and float32",
-            "keep": kp}
+    for prec in (0, 4, 3):
+        for name, key, dt in (("filter2d_u8", "u8", np.uint8), ("filter2d_f32", "f32", np.float32)):
+            calls.append(Call(f"{name}{names[prec]}/reflect101", f"{name}{names[prec]}", key, dt,
+                              lambda lib, s, d, st, prec=prec: lib.ipfg_filter2d(s, d, kptr, 31, 31, 15, 15, 3, 0.0,
+                                                                                  0.0, prec, st),
+                              lambda o, im: o.filter2d(im, kp, border=3), kern[prec], flops=flops))
+
+    def accuracy(ipf):
+        """max / RMS error of every precision against the FP64 oracle (filter2D<double>)."""
+        from oracle.pyoracle import Oracle, ref_available
+        o = Oracle("ref") if ref_available() else Oracle("oracle")
+        o.set_num_threads(0)
+        out = {}
+        for key, img in (("u8", u8), ("f32", f32)):
+            ref = o.filter2d(img, kp, border=3, out_dtype=np.float64)
+            ref_u8 = o.filter2d(img, kp, border=3) if key == "u8" else None
+            for prec in (0, 4, 3):
+                got = ipf.filter2d(img, kp, 3, out_dtype=np.float32, precision=prec).astype(np.float64)
+                err = got - ref
+                e = {"max_abs_err": float(np.abs(err).max()), "rms_err": float(np.sqrt(np.mean(err * err))),
+                     "vs": "FP64 oracle (filter2D<double>), float32 output"}
+                if ref_u8 is not None:
+                    g8 = ipf.filter2d(img, kp, 3, precision=prec)
+                    e["u8_mismatches"] = int(np.count_nonzero(g8 != ref_u8))
+                    e["u8_max_diff"] = int(np.abs(g8.astype(int) - ref_u8.astype(int)).max())
+                out[f"filter2d_{key}{names[prec]}"] = e
+        return out
+
+    return {"images": {"u8": u8, "f32": f32}, "calls": calls, "dtype": "f64 / bf16x3 / fp16x2",
+            "workload": "filter2D dense 31x31 float32 kernel (N(0,1), sum 1), Reflect101, 4096x4096x1 uint8 and "
+                        "float32; default (FP64, certified), FP16X2 and BF16X3 MFMA",
+            "keep": kp, "post": accuracy, "kernel_raw_sum": ksum}
 
 
 WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4}
@@ -288,6 +315,14 @@ def main():
     copy = {"bytes_per_launch": 2 * big.nbytes, "mean_ms": round(copy_ms, 4),
             "gb_s": round(2 * big.nbytes / (copy_ms * 1e-3) / 1e9, 1)}
     cpu = None if args.no_cpu else cpu_baseline(wd, args.cpu_budget)
+    extra = {}
+    if "post" in wd and not args.no_cpu:
+        extra["accuracy"] = wd["post"](ipf)
+    mfma = {n: {"algorithmic_tflop_s": v["tflop_s"], "peak_tflop_s": MFMA_PEAK_TFLOPS,
+                "frac": round(v["tflop_s"] / MFMA_PEAK_TFLOPS, 4)}
+            for n, v in summary.items() if "tflop_s" in v and ("bf16" in n or "fp16" in n)}
+    if mfma:
+        extra["mfma_roofline"] = mfma
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -309,6 +344,7 @@ def main():
         "per_filter": summary,
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4), "what": "single filter call, HIP events"},
         "device_copy": copy,
+        **extra,
     }
     print(json.dumps(line), flush=True)
 

@@ -289,3 +289,87 @@ def bilateral_filter(src, d, sigma_color, sigma_space, border=BorderMode.REFLECT
     _lib.check(_lib.load().ipfg_bilateral(C.byref(s), C.byref(dd), int(d), float(sigma_color), float(sigma_space),
                                           int(border), int(precision), _sh(stream)))
     return _ret(holder)
+
+
+# ------------------------------------------------------------------ batches and serving
+def job(src, dst, op: str, border=BorderMode.REFLECT101, sigma=0.0, ksize=0, d=0, sigma_color=0.0,
+        sigma_space=0.0) -> _lib.IpfgJob:
+    """A batch / server job: op is "gaussian", "median" or "bilateral"; src and
+    dst are uint8 DeviceImages (batches) or contiguous numpy arrays / DeviceImages
+    (servers)."""
+    ops = {"gaussian": _lib.OP_GAUSSIAN, "median": _lib.OP_MEDIAN, "bilateral": _lib.OP_BILATERAL}
+    sd = src.descriptor() if isinstance(src, DeviceImage) else _desc(np.ascontiguousarray(src))
+    dd = dst.descriptor() if isinstance(dst, DeviceImage) else _desc(dst)
+    return _lib.IpfgJob(sd, dd, ops[op], int(border), float(sigma), int(ksize), int(d), float(sigma_color),
+                        float(sigma_space))
+
+
+class Batch:
+    """Persistent tile-queue executor for many device-resident uint8 images
+    (ipfg_batch_*); with graph=True the launches are a replayed hipGraph."""
+
+    def __init__(self, max_jobs: int = 256, graph: bool = True):
+        self.handle = C.c_void_p()
+        _lib.check(_lib.load().ipfg_batch_create(C.byref(self.handle), int(max_jobs), 1 if graph else 0))
+
+    def run(self, jobs, stream: Stream | None = None) -> None:
+        arr = (_lib.IpfgJob * len(jobs))(*jobs)
+        _lib.check(_lib.load().ipfg_batch_run(self.handle, arr, len(jobs), _sh(stream)))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.check(_lib.load().ipfg_batch_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (ipfg_server_*)."""
+
+    def __init__(self, device_budget_bytes: int, max_batch: int = 128, graph: bool = True):
+        self.handle = C.c_void_p()
+        self._keep = {}
+        _lib.check(_lib.load().ipfg_server_create(C.byref(self.handle), int(device_budget_bytes), int(max_batch),
+                                                  1 if graph else 0))
+
+    def submit(self, src, dst, op: str, **kw) -> int:
+        j = job(src, dst, op, **kw)
+        rid = C.c_longlong()
+        _lib.check(_lib.load().ipfg_server_submit(self.handle, C.byref(j), C.byref(rid)))
+        self._keep[rid.value] = (src, dst)  # buffers stay alive until completion
+        return rid.value
+
+    def poll(self, capacity: int = 4096):
+        ids = (C.c_longlong * capacity)()
+        lat = (C.c_double * capacity)()
+        n = C.c_int()
+        _lib.check(_lib.load().ipfg_server_poll(self.handle, ids, lat, capacity, C.byref(n)))
+        out = [(ids[i], lat[i]) for i in range(n.value)]
+        for i, _ in out:
+            self._keep.pop(i, None)
+        return out
+
+    def drain(self) -> None:
+        _lib.check(_lib.load().ipfg_server_drain(self.handle))
+
+    def stats(self):
+        b = C.c_longlong()
+        m = C.c_int()
+        _lib.check(_lib.load().ipfg_server_stats(self.handle, C.byref(b), C.byref(m)))
+        return b.value, m.value
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.check(_lib.load().ipfg_server_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

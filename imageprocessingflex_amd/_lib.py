@@ -19,6 +19,8 @@ CONSTANT, REPLICATE, REFLECT, REFLECT101, WRAP = range(5)
 PREC_DEFAULT, PREC_EXACT, PREC_FP32, PREC_BF16X3, PREC_FP16X2 = range(5)
 # ipfg_status
 OK, ERR_ARG, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE = range(6)
+# ipfg_op
+OP_GAUSSIAN, OP_MEDIAN, OP_BILATERAL = range(3)
 
 
 class IpfgImage(C.Structure):
@@ -33,8 +35,23 @@ class IpfgImage(C.Structure):
     ]
 
 
+class IpfgJob(C.Structure):
+    _fields_ = [
+        ("src", IpfgImage),
+        ("dst", IpfgImage),
+        ("op", C.c_int),
+        ("border", C.c_int),
+        ("sigma", C.c_double),
+        ("ksize", C.c_int),
+        ("d", C.c_int),
+        ("sigma_color", C.c_double),
+        ("sigma_space", C.c_double),
+    ]
+
+
 _P = C.POINTER
 _IMG = _P(IpfgImage)
+_JOB = _P(IpfgJob)
 _D = C.c_double
 _I = C.c_int
 _V = C.c_void_p
@@ -68,6 +85,15 @@ SIGNATURES = {
     "ipfg_filter2d": (_I, [_IMG, _IMG, _P(C.c_float), _I, _I, _I, _I, _I, _D, _D, _I, _V]),
     "ipfg_median": (_I, [_IMG, _IMG, _I, _I, _V]),
     "ipfg_bilateral": (_I, [_IMG, _IMG, _I, _D, _D, _I, _I, _V]),
+    "ipfg_batch_create": (_I, [_P(_V), _I, _I]),
+    "ipfg_batch_run": (_I, [_V, _JOB, _I, _V]),
+    "ipfg_batch_destroy": (_I, [_V]),
+    "ipfg_server_create": (_I, [_P(_V), C.c_size_t, _I, _I]),
+    "ipfg_server_submit": (_I, [_V, _JOB, _P(C.c_longlong)]),
+    "ipfg_server_poll": (_I, [_V, _P(C.c_longlong), _P(_D), _I, _P(_I)]),
+    "ipfg_server_drain": (_I, [_V]),
+    "ipfg_server_stats": (_I, [_V, _P(C.c_longlong), _P(_I)]),
+    "ipfg_server_destroy": (_I, [_V]),
 }
 
 _lib = None

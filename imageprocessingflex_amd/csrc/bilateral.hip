@@ -13,6 +13,7 @@
 // pixels are recomputed in FP64 in the reference's order.
 //
 // Exact path: the reference loop in FP64 per pixel, for every pixel type.
+#include "batch.hpp"
 #include "kernels.hpp"
 
 namespace ipfg {
@@ -92,9 +93,11 @@ struct BilArgs {
     FixList fix;
 };
 
+// One TX x TH output tile (bx, by); the workgroup's LDS is bsm. Certified
+// launches recompute doubtful pixels in place (wave_fix); the caller runs
+// fix_finish once per workgroup.
 template <int CN, bool CONST_BORDER, class Acc, bool CERT>
-__global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
-    extern __shared__ __align__(16) uint32_t bsm[];
+__device__ __forceinline__ void bilateral_u8_tile(const BilArgs& a, int bx, int by, uint32_t* bsm) {
     const int r = a.r, TH = a.TH;
     const int W = a.src.w, H = a.src.h;
     const int XP = bil_xp(r);
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
     const Acc* __restrict__ space = static_cast<const Acc*>(a.space);
     const uint8_t* __restrict__ src = static_cast<const uint8_t*>(a.src.data);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TH;
+    const int x0 = bx * TX, y0 = by * TH;
     for (int i = tid; i < a.lut_len; i += kBlock) lut[i] = lutg[i];
     const int XW = TX + 2 * r;
     for (int rr = wave; rr < HR; rr += kBlock / 64) {
@@ -180,6 +183,12 @@ __global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
             });
         }
     }
+}
+
+template <int CN, bool CONST_BORDER, class Acc, bool CERT>
+__global__ __launch_bounds__(kBlock) void bilateral_u8_kernel(BilArgs a) {
+    extern __shared__ __align__(16) uint32_t bsm[];
+    bilateral_u8_tile<CN, CONST_BORDER, Acc, CERT>(a, blockIdx.x, blockIdx.y, bsm);
     if (CERT) fix_finish(a.fix);
 }
 
@@ -296,5 +305,61 @@ hipError_t launch_bilateral(const BilateralLaunch& a, hipStream_t s) {
     }
     return a.border == IPFG_BORDER_CONSTANT ? bil_fast_cn<true>(a, s) : bil_fast_cn<false>(a, s);
 }
+
+// ------------------------------------------------------------------ batched
+namespace {
+template <int CN>
+__device__ __forceinline__ void bil_batch_tile(const BilArgs& a, int tx, int ty, uint32_t* bsm) {
+    if (a.border == IPFG_BORDER_CONSTANT) bilateral_u8_tile<CN, true, double, true>(a, tx, ty, bsm);
+    else bilateral_u8_tile<CN, false, double, true>(a, tx, ty, bsm);
+}
+
+__global__ __launch_bounds__(kBlock) void bilateral_batch_kernel(const BatchClassHeader* __restrict__ hdr,
+                                                                 const BatchJob* __restrict__ jobs, BatchQueue* q,
+                                                                 FixList fix) {
+    extern __shared__ __align__(16) uint32_t bsm[];
+    batch_loop(hdr, jobs, q, [&](const BatchJob& J, int tx, int ty) {
+        BilArgs a{};
+        a.src = J.src;
+        a.dst = J.dst;
+        a.taps = J.taps;
+        a.space = J.spaced;
+        a.lut = J.lutd;
+        a.spaced = J.spaced;
+        a.lutd = J.lutd;
+        a.ntaps = J.ntaps;
+        a.lut_len = J.lut_len;
+        a.r = J.r;
+        a.border = J.border;
+        a.TH = J.rows;
+        a.cc = J.cc;
+        a.bound = J.bound;
+        a.fix = fix;
+        switch (J.src.cn) {
+            case 1: bil_batch_tile<1>(a, tx, ty, bsm); break;
+            case 3: bil_batch_tile<3>(a, tx, ty, bsm); break;
+            default: bil_batch_tile<4>(a, tx, ty, bsm); break;
+        }
+    });
+    fix_finish(fix);
+}
+}  // namespace
+
+hipError_t launch_bilateral_batch(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
+                                  int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(bilateral_batch_kernel, dim3(grid), dim3(kBlock), lds, s, hdr, jobs, q, fix);
+    return hipGetLastError();
+}
+
+int bilateral_batch_occupancy(size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(bilateral_batch_kernel), kBlock,
+                                                     lds) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+// Largest radius and range table the batched bilateral filter takes.
+size_t bilateral_batch_lds() { return bil_lds_bytes(kBatchTileRows, 5, 255 * 4 + 1, sizeof(double)); }
 
 }  // namespace ipfg

@@ -16,6 +16,7 @@
 //    gen_median_networks.py).
 //  - any other k: bitwise selection -- the largest key t with
 //    #(window keys < t) <= rank is built MSB first by counting in LDS.
+#include "batch.hpp"
 #include "kernels.hpp"
 
 namespace ipfg {
@@ -102,13 +103,13 @@ __device__ __forceinline__ K med3(K a, K b, K c) {
     return kmax(kmin(a, b), kmin(kmax(a, b), c));
 }
 
+// One TX x TH output tile (bx, by) of a median; the workgroup's LDS is smem.
 template <class T, int KS>
-__global__ __launch_bounds__(kBlock) void median_kernel(const T* __restrict__ src, size_t sstride, int W, int H,
-                                                        int cn, T* __restrict__ dst, size_t dstride, int k,
-                                                        int border, int TH) {
+__device__ __forceinline__ void median_tile(const T* __restrict__ src, size_t sstride, int W, int H, int cn,
+                                            T* __restrict__ dst, size_t dstride, int k, int border, int TH, int bx,
+                                            int by, unsigned char* smem_raw) {
     using KT = KeyOf<T>;
     using K = typename KT::K;
-    extern __shared__ unsigned char smem_raw[];
     K* tile = reinterpret_cast<K*>(smem_raw);
     const int ksz = KS > 0 ? KS : k;
     const int r = ksz / 2;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void median_kernel(const T* __restrict__ sr
     const int HR = TH + ksz - 1;
     const int plane = HR * XP;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TH;
+    const int x0 = bx * TX, y0 = by * TH;
     const int XW = TX + ksz - 1;
     const int rowSamples = XW * cn;
     const bool inside_x = (x0 - r >= 0) && (x0 - r + XW <= W);
@@ -220,6 +221,14 @@ __global__ __launch_bounds__(kBlock) void median_kernel(const T* __restrict__ sr
     }
 }
 
+template <class T, int KS>
+__global__ __launch_bounds__(kBlock) void median_kernel(const T* __restrict__ src, size_t sstride, int W, int H,
+                                                        int cn, T* __restrict__ dst, size_t dstride, int k,
+                                                        int border, int TH) {
+    extern __shared__ unsigned char smem_raw[];
+    median_tile<T, KS>(src, sstride, W, H, cn, dst, dstride, k, border, TH, blockIdx.x, blockIdx.y, smem_raw);
+}
+
 // Windows too large for an LDS tile: the same bitwise selection reading the
 // source through the border tables directly, one thread per sample.
 template <class T>
@@ -295,5 +304,39 @@ hipError_t launch_median(const MedianLaunch& a, hipStream_t s) {
     }
     return hipErrorInvalidValue;
 }
+
+// ------------------------------------------------------------------ batched
+namespace {
+__global__ __launch_bounds__(kBlock) void median_batch_kernel(const BatchClassHeader* __restrict__ hdr,
+                                                              const BatchJob* __restrict__ jobs, BatchQueue* q) {
+    extern __shared__ unsigned char smem_raw[];
+    batch_loop(hdr, jobs, q, [&](const BatchJob& J, int tx, int ty) {
+        const uint8_t* src = static_cast<const uint8_t*>(J.src.data);
+        uint8_t* dst = static_cast<uint8_t*>(J.dst.data);
+        if (J.k == 5)
+            median_tile<uint8_t, 5>(src, J.src.stride, J.src.w, J.src.h, J.src.cn, dst, J.dst.stride, 5, J.border, J.rows,
+                                    tx, ty, smem_raw);
+        else
+            median_tile<uint8_t, 3>(src, J.src.stride, J.src.w, J.src.h, J.src.cn, dst, J.dst.stride, 3, J.border, J.rows,
+                                    tx, ty, smem_raw);
+    });
+}
+}  // namespace
+
+hipError_t launch_median_batch(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, int grid, size_t lds,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(median_batch_kernel, dim3(grid), dim3(kBlock), lds, s, hdr, jobs, q);
+    return hipGetLastError();
+}
+
+int median_batch_occupancy(size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(median_batch_kernel), kBlock,
+                                                     lds) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+size_t median_batch_lds() { return med_lds_bytes<uint8_t>(4, kBatchTileRows, 5); }
 
 }  // namespace ipfg

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <list>
@@ -13,6 +17,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "tuner.hpp"
 
@@ -791,6 +796,550 @@ void bilateral_impl(const ipfg_image* src, const ipfg_image* dst, int d, double 
     io.finish();
 }
 
+
+// ------------------------------------------------------------------ batch geometry
+DevImage to_dev(const ipfg_image& im) {
+    DevImage d;
+    d.data = im.data;
+    d.w = im.width;
+    d.h = im.height;
+    d.cn = im.channels;
+    d.dt = im.dtype;
+    d.stride = im.row_stride_bytes ? im.row_stride_bytes : packed_stride(im);
+    return d;
+}
+
+bool images_overlap(const ipfg_image& a, const ipfg_image& b) {
+    const char* a0 = static_cast<const char*>(a.data);
+    const char* a1 = a0 + (a.row_stride_bytes ? a.row_stride_bytes : packed_stride(a)) * static_cast<size_t>(a.height);
+    const char* b0 = static_cast<const char*>(b.data);
+    const char* b1 = b0 + (b.row_stride_bytes ? b.row_stride_bytes : packed_stride(b)) * static_cast<size_t>(b.height);
+    return a0 < b1 && b0 < a1;
+}
+
+}  // namespace
+
+size_t batch_lds_bytes(int cls) {
+    switch (cls) {
+        case BCLS_SEP_SMALL: return sep_batch_lds(kBatchSepSmallK);
+        case BCLS_SEP_LARGE: return sep_batch_lds(kBatchSepLargeK);
+        case BCLS_MEDIAN: return median_batch_lds();
+        default: return bilateral_batch_lds();
+    }
+}
+
+int batch_grid(int cls, size_t lds) {
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    int per_cu = 1;
+    switch (cls) {
+        case BCLS_SEP_SMALL:
+        case BCLS_SEP_LARGE: per_cu = sep_batch_occupancy(lds); break;
+        case BCLS_MEDIAN: per_cu = median_batch_occupancy(lds); break;
+        default: per_cu = bilateral_batch_occupancy(lds); break;
+    }
+    return std::max(1, cus) * per_cu;
+}
+
+void batch_tiles(int cls, int w, int h, int cn, int k, int* tiles_x, int* tiles_y, int* rows) {
+    (void)cn;
+    (void)k;
+    *tiles_x = (w + 63) / 64;
+    if (cls == BCLS_SEP_SMALL || cls == BCLS_SEP_LARGE) {
+        *rows = h <= 128 ? (h + 15) / 16 * 16 : 128;  // strip height
+    } else {
+        *rows = kBatchTileRows;
+    }
+    *tiles_y = (h + *rows - 1) / *rows;
+}
+
+namespace {
+
+// ------------------------------------------------------------------ batches
+// Parameter blocks of one batch job (host side), built like the single-image
+// calls build theirs so that results are identical.
+struct JobParams {
+    std::vector<char> blob;
+    int cls = 0;
+};
+
+size_t image_bytes(const ipfg_image& im) {
+    return packed_stride(im) * static_cast<size_t>(im.height);
+}
+
+void check_job(const ipfg_job& j, bool device_only) {
+    check_image(&j.src, "null src image");
+    check_image(&j.dst, "null dst image");
+    abi_check(j.src.dtype == IPFG_U8 && j.dst.dtype == IPFG_U8, "batched jobs take uint8 images");
+    abi_check(j.src.channels == 1 || j.src.channels == 3 || j.src.channels == 4, "batched jobs take 1, 3 or 4 channels");
+    abi_check(j.src.width == j.dst.width && j.src.height == j.dst.height && j.src.channels == j.dst.channels,
+              "src and dst shapes differ");
+    abi_check(j.src.width > 0 && j.src.height > 0, "empty image in a batch");
+    abi_check(j.border >= IPFG_BORDER_CONSTANT && j.border <= IPFG_BORDER_WRAP, "invalid border mode");
+    if (device_only) abi_check(j.src.on_device && j.dst.on_device, "batch images must be device-resident");
+    abi_check(j.src.on_device != j.dst.on_device || !images_overlap(j.src, j.dst), "dst must not alias src");
+    switch (j.op) {
+        case IPFG_OP_GAUSSIAN: break;
+        case IPFG_OP_MEDIAN:
+            require(j.ksize > 0 && j.ksize % 2 == 1, "medianBlur: ksize must be odd and positive");
+            require(j.border != IPFG_BORDER_CONSTANT, "medianBlur: Constant border is not supported");
+            if (j.ksize != 3 && j.ksize != 5) throw Unsupported{"ipfg: batched median takes ksize 3 or 5"};
+            break;
+        case IPFG_OP_BILATERAL:
+            require(j.sigma_color > 0 && j.sigma_space > 0, "bilateralFilter: sigmas must be positive");
+            break;
+        default: abi_check(false, "unknown batch op");
+    }
+}
+
+// Builds the class, tiles and device parameters of one job.
+void build_job(const ipfg_job& j, BatchJob& out, JobParams& p) {
+    out.src = to_dev(j.src);
+    out.dst = to_dev(j.dst);
+    out.border = j.border;
+    const int w = j.src.width, h = j.src.height, cn = j.src.channels;
+    if (j.op == IPFG_OP_GAUSSIAN) {
+        const std::vector<double> k = gaussian_kernel(j.sigma, j.ksize);
+        const int kw = static_cast<int>(k.size());
+        if (kw > kBatchSepLargeK) throw Unsupported{"ipfg: batched gaussian takes windows up to 49"};
+        p.cls = kw <= kBatchSepSmallK ? BCLS_SEP_SMALL : BCLS_SEP_LARGE;
+        constexpr int kPad = 16;
+        p.blob.assign(sizeof(double) * 2 * (kw + kPad), 0);
+        double* d = reinterpret_cast<double*>(p.blob.data());
+        double sx = 0;
+        for (int i = 0; i < kw; ++i) {
+            d[i] = k[i];
+            d[kw + kPad + i] = k[i];
+            sx += std::fabs(k[i]);
+        }
+        out.kw = out.kh = kw;
+        // certified bound as for a single call, plus the float ring rounding
+        out.bcoef = 2.02 * (2 * kw + 4) * kU64 * sx * sx + 1.01 * 5.9604644775390625e-08 * sx * sx;
+        out.bdelta = 1e-300;
+    } else if (j.op == IPFG_OP_MEDIAN) {
+        p.cls = BCLS_MEDIAN;
+        out.k = j.ksize;
+    } else {
+        const int r = j.d <= 0 ? static_cast<int>(std::lround(j.sigma_space * 1.5)) : j.d / 2;
+        if (r > 5) throw Unsupported{"ipfg: batched bilateral takes radii up to 5"};
+        p.cls = BCLS_BILATERAL;
+        std::vector<int> taps;
+        std::vector<double> sw;
+        for (int jj = -r; jj <= r; ++jj)
+            for (int i = -r; i <= r; ++i)
+                if (i * i + jj * jj <= r * r) {
+                    taps.push_back(i);
+                    taps.push_back(jj);
+                    sw.push_back(std::exp(-(i * i + jj * jj) / (2 * j.sigma_space * j.sigma_space)));
+                }
+        const double cc = -1.0 / (2 * j.sigma_color * j.sigma_color);
+        std::vector<double> lut(static_cast<size_t>(255 * cn) + 1);
+        for (size_t q = 0; q < lut.size(); ++q) lut[q] = std::exp(static_cast<double>(q * q) * cc);
+        p.blob.resize(sizeof(double) * (sw.size() + lut.size()) + sizeof(int) * taps.size());
+        std::memcpy(p.blob.data(), sw.data(), sizeof(double) * sw.size());
+        std::memcpy(p.blob.data() + sizeof(double) * sw.size(), lut.data(), sizeof(double) * lut.size());
+        std::memcpy(p.blob.data() + sizeof(double) * (sw.size() + lut.size()), taps.data(), sizeof(int) * taps.size());
+        out.ntaps = static_cast<int>(sw.size());
+        out.lut_len = static_cast<int>(lut.size());
+        out.r = r;
+        out.cc = cc;
+        out.bound = 2.02 * (2.0 * out.ntaps + 6.0) * kU64 * 255.0 + 1e-300;
+    }
+    batch_tiles(p.cls, w, h, cn, out.kw, &out.tiles_x, &out.tiles_y, &out.rows);
+}
+
+// Runs batches on one stream: the job table goes from a pinned host buffer
+// (two, alternating) to a fixed device arena, then one persistent launch per
+// class; with use_graph the copy and the launches are a captured hipGraph.
+class BatchExecutor {
+public:
+    BatchExecutor(int max_jobs, bool use_graph) : max_jobs_(max_jobs), use_graph_(use_graph) {
+        abi_check(max_jobs > 0 && max_jobs <= (1 << 20), "max_jobs out of range");
+        ensure_init();
+        per_class_ = sizeof(BatchClassHeader) + sizeof(BatchJob) * static_cast<size_t>(max_jobs);
+        per_class_ = (per_class_ + 255) / 256 * 256;
+        arena_bytes_ = per_class_ * BCLS_COUNT;
+        HIPCALL(hipMalloc(&dev_arena_, arena_bytes_));
+        HIPCALL(hipMalloc(&dev_state_, 256));
+        for (int b = 0; b < 2; ++b) {
+            HIPCALL(hipHostMalloc(&host_arena_[b], arena_bytes_, hipHostMallocDefault));
+            HIPCALL(hipEventCreateWithFlags(&free_[b], hipEventDisableTiming));
+        }
+        queues_ = static_cast<BatchQueue*>(dev_state_);
+        fix_.count = reinterpret_cast<unsigned long long*>(static_cast<char*>(dev_state_) + 64);
+        fix_.done = reinterpret_cast<unsigned int*>(static_cast<char*>(dev_state_) + 72);
+        fix_.report = reinterpret_cast<unsigned long long*>(static_cast<char*>(dev_state_) + 80);
+        HIPCALL(hipMemset(dev_state_, 0, 256));
+        HIPCALL(hipDeviceSynchronize());
+        for (int c = 0; c < BCLS_COUNT; ++c) {
+            lds_[c] = batch_lds_bytes(c);
+            grid_[c] = batch_grid(c, lds_[c]);
+        }
+    }
+    ~BatchExecutor() {
+        for (int b = 0; b < 2; ++b) {
+            if (armed_[b]) (void)hipEventSynchronize(free_[b]);
+            if (exec_[b]) (void)hipGraphExecDestroy(exec_[b]);
+            release_params(b, nullptr);
+            (void)hipEventDestroy(free_[b]);
+            (void)hipHostFree(host_arena_[b]);
+        }
+        (void)hipFree(dev_arena_);
+        (void)hipFree(dev_state_);
+    }
+
+    void run(const ipfg_job* jobs, int n, hipStream_t s) {
+        abi_check(n >= 0 && n <= max_jobs_, "too many jobs for this batch executor");
+        const int b = next_;
+        next_ ^= 1;
+        if (armed_[b]) HIPCALL(hipEventSynchronize(free_[b]));  // this buffer's previous batch has been copied
+        release_params(b, s);
+        char* h = static_cast<char*>(host_arena_[b]);
+        BatchClassHeader* hdr[BCLS_COUNT];
+        BatchJob* tab[BCLS_COUNT];
+        for (int c = 0; c < BCLS_COUNT; ++c) {
+            hdr[c] = reinterpret_cast<BatchClassHeader*>(h + per_class_ * c);
+            tab[c] = reinterpret_cast<BatchJob*>(h + per_class_ * c + sizeof(BatchClassHeader));
+            *hdr[c] = BatchClassHeader{};
+        }
+        for (int i = 0; i < n; ++i) {
+            check_job(jobs[i], true);
+            BatchJob J{};
+            JobParams p;
+            build_job(jobs[i], J, p);
+            if (!p.blob.empty()) {
+                ParamCache::Entry* e = g_params.acquire(std::string(p.blob.data(), p.blob.size()), s);
+                held_[b].push_back(e);
+                const char* dev = static_cast<const char*>(e->dev);
+                if (p.cls == BCLS_BILATERAL) {
+                    J.spaced = reinterpret_cast<const double*>(dev);
+                    J.lutd = J.spaced + J.ntaps;
+                    J.taps = reinterpret_cast<const int*>(dev + sizeof(double) * (J.ntaps + J.lut_len));
+                } else {
+                    J.kx = reinterpret_cast<const double*>(dev);
+                    J.ky = J.kx + J.kw + 16;
+                }
+            }
+            BatchClassHeader& H = *hdr[p.cls];
+            J.tile0 = H.ntiles;
+            H.ntiles += J.tiles_x * J.tiles_y;
+            tab[p.cls][H.njobs++] = J;
+        }
+        if (use_graph_) {
+            if (!exec_[b]) capture(b, s);
+            HIPCALL(hipGraphLaunch(exec_[b], s));
+        } else {
+            enqueue(b, s);
+        }
+        HIPCALL(hipEventRecord(free_[b], s));
+        armed_[b] = true;
+        for (auto* e : held_[b]) g_params.release(e, s);
+    }
+
+private:
+    void enqueue(int b, hipStream_t s) {
+        HIPCALL(hipMemcpyAsync(dev_arena_, host_arena_[b], arena_bytes_, hipMemcpyHostToDevice, s));
+        char* d = static_cast<char*>(dev_arena_);
+        for (int c = 0; c < BCLS_COUNT; ++c) {
+            const auto* hdr = reinterpret_cast<const BatchClassHeader*>(d + per_class_ * c);
+            const auto* jobs = reinterpret_cast<const BatchJob*>(d + per_class_ * c + sizeof(BatchClassHeader));
+            BatchQueue* q = queues_ + c;
+            if (c == BCLS_SEP_SMALL || c == BCLS_SEP_LARGE)
+                HIPCALL(launch_sep_batch(c, hdr, jobs, q, fix_, grid_[c], lds_[c], s));
+            else if (c == BCLS_MEDIAN)
+                HIPCALL(launch_median_batch(hdr, jobs, q, grid_[c], lds_[c], s));
+            else
+                HIPCALL(launch_bilateral_batch(hdr, jobs, q, fix_, grid_[c], lds_[c], s));
+        }
+    }
+
+    void capture(int b, hipStream_t s) {
+        hipGraph_t g = nullptr;
+        HIPCALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(b, s);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCALL(hipStreamEndCapture(s, &g));
+        HIPCALL(hipGraphInstantiate(&exec_[b], g, nullptr, nullptr, 0));
+        HIPCALL(hipGraphDestroy(g));
+    }
+
+    // Parameter blocks of the batch that last used buffer b: their release was
+    // recorded after its launches; forget them now.
+    void release_params(int b, hipStream_t) { held_[b].clear(); }
+
+    int max_jobs_;
+    bool use_graph_;
+    size_t per_class_ = 0, arena_bytes_ = 0;
+    void* dev_arena_ = nullptr;
+    void* dev_state_ = nullptr;
+    void* host_arena_[2] = {nullptr, nullptr};
+    hipEvent_t free_[2] = {nullptr, nullptr};
+    bool armed_[2] = {false, false};
+    hipGraphExec_t exec_[2] = {nullptr, nullptr};
+    std::vector<ParamCache::Entry*> held_[2];
+    int next_ = 0;
+    BatchQueue* queues_ = nullptr;
+    FixList fix_{};
+    size_t lds_[BCLS_COUNT] = {};
+    int grid_[BCLS_COUNT] = {};
+};
+
+// ------------------------------------------------------------------ server
+class Server {
+public:
+    Server(size_t budget, int max_batch, bool use_graph) : budget_(budget), max_batch_(max_batch) {
+        abi_check(budget > 0, "device budget must be positive");
+        abi_check(max_batch > 0 && max_batch <= 65536, "max_batch out of range");
+        ensure_init();
+        device_ = g_device.load();
+        HIPCALL(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+        for (int i = 0; i < kLanes; ++i) {
+            HIPCALL(hipStreamCreateWithFlags(&lane_[i].stream, hipStreamNonBlocking));
+            lane_[i].exec = std::make_unique<BatchExecutor>(max_batch, use_graph);
+        }
+        worker_ = std::thread([this] { loop(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (worker_.joinable()) worker_.join();
+        for (int i = 0; i < kLanes; ++i) {
+            (void)hipStreamSynchronize(lane_[i].stream);
+            lane_[i].exec.reset();
+            (void)hipStreamDestroy(lane_[i].stream);
+        }
+        (void)hipStreamDestroy(up_);
+    }
+
+    long long submit(const ipfg_job& r) {
+        check_job(r, false);
+        const size_t need = device_need(r);
+        abi_check(need <= budget_, "request larger than the device budget");
+        std::lock_guard<std::mutex> lock(mu_);
+        abi_check(error_.empty(), error_.c_str());
+        Req q;
+        q.job = r;
+        q.id = next_id_++;
+        q.t_submit = std::chrono::steady_clock::now();
+        q.need = need;
+        pending_.push_back(q);
+        ++outstanding_;
+        cv_.notify_all();
+        return q.id;
+    }
+
+    int poll(long long* ids, double* lat, int cap) {
+        std::lock_guard<std::mutex> lock(mu_);
+        abi_check(error_.empty(), error_.c_str());
+        int n = 0;
+        while (n < cap && !done_.empty()) {
+            ids[n] = done_.front().first;
+            if (lat) lat[n] = done_.front().second;
+            done_.pop_front();
+            ++n;
+        }
+        return n;
+    }
+
+    void drain() {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_done_.wait(lock, [&] { return outstanding_ == 0 || !error_.empty(); });
+        abi_check(error_.empty(), error_.c_str());
+    }
+
+    void stats(long long* batches, int* max_jobs) {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (batches) *batches = batches_;
+        if (max_jobs) *max_jobs = max_jobs_seen_;
+    }
+
+private:
+    static constexpr int kLanes = 2;  // batches in flight
+    struct Req {
+        ipfg_job job{};
+        long long id = 0;
+        std::chrono::steady_clock::time_point t_submit;
+        size_t need = 0;
+        void* dsrc = nullptr;
+        void* ddst = nullptr;
+    };
+    struct Lane {
+        hipStream_t stream = nullptr;
+        std::unique_ptr<BatchExecutor> exec;
+        std::vector<Req> reqs;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+    };
+
+    static size_t device_need(const ipfg_job& r) {
+        return (r.src.on_device ? 0 : image_bytes(r.src)) + (r.dst.on_device ? 0 : image_bytes(r.dst));
+    }
+
+    void loop() {
+        if (hipSetDevice(device_) != hipSuccess) return fail("hipSetDevice failed in the server thread");
+        for (int i = 0; i < kLanes; ++i)
+            if (hipEventCreateWithFlags(&lane_[i].done, hipEventDisableTiming) != hipSuccess)
+                return fail("event creation failed");
+        try {
+            for (;;) {
+                bool progressed = retire();
+                std::vector<Req> admitted;
+                {
+                    std::unique_lock<std::mutex> lock(mu_);
+                    if (stop_ && pending_.empty() && !any_busy()) break;
+                    // admit in arrival order while the budget allows
+                    while (!pending_.empty() && static_cast<int>(admitted.size()) < max_batch_ &&
+                           in_use_ + pending_.front().need <= budget_) {
+                        in_use_ += pending_.front().need;
+                        admitted.push_back(pending_.front());
+                        pending_.pop_front();
+                    }
+                    if (admitted.empty() && !progressed) {
+                        if (any_busy()) {
+                            lock.unlock();
+                            std::this_thread::sleep_for(std::chrono::microseconds(20));
+                        } else {
+                            cv_.wait_for(lock, std::chrono::milliseconds(5));
+                        }
+                        continue;
+                    }
+                }
+                if (!admitted.empty()) launch(std::move(admitted));
+            }
+        } catch (const ArgError& e) {
+            fail(e.msg);
+        } catch (const Unsupported& e) {
+            fail(e.msg);
+        } catch (const DeviceError& e) {
+            fail(std::string("ipfg: HIP error '") + hipGetErrorString(e.err) + "' in " + e.where);
+        } catch (const std::exception& e) {
+            fail(std::string("ipfg: ") + e.what());
+        }
+        for (int i = 0; i < kLanes; ++i) (void)hipEventDestroy(lane_[i].done);
+    }
+
+    bool any_busy() const {
+        for (int i = 0; i < kLanes; ++i)
+            if (lane_[i].busy) return true;
+        return false;
+    }
+
+    // Uploads the admitted requests and runs them as one batch on a free lane
+    // (waiting for one when both are busy).
+    void launch(std::vector<Req> reqs) {
+        int li = -1;
+        while (li < 0) {
+            for (int i = 0; i < kLanes; ++i)
+                if (!lane_[i].busy) {
+                    li = i;
+                    break;
+                }
+            if (li < 0) {
+                HIPCALL(hipEventSynchronize(lane_[next_lane_].done));
+                retire();
+            }
+        }
+        Lane& L = lane_[li];
+        next_lane_ = li ^ 1;
+        std::vector<ipfg_job> jobs;
+        jobs.reserve(reqs.size());
+        for (Req& q : reqs) {
+            ipfg_job j = q.job;
+            if (!j.src.on_device) {
+                HIPCALL(hipMallocAsync(&q.dsrc, image_bytes(j.src), L.stream));
+                HIPCALL(hipMemcpy2DAsync(q.dsrc, packed_stride(j.src), j.src.data, host_stride(j.src), packed_stride(j.src),
+                                         j.src.height, hipMemcpyHostToDevice, L.stream));
+                j.src.data = q.dsrc;
+                j.src.row_stride_bytes = packed_stride(q.job.src);
+                j.src.on_device = 1;
+            }
+            if (!j.dst.on_device) {
+                HIPCALL(hipMallocAsync(&q.ddst, image_bytes(j.dst), L.stream));
+                j.dst.data = q.ddst;
+                j.dst.row_stride_bytes = packed_stride(q.job.dst);
+                j.dst.on_device = 1;
+            }
+            jobs.push_back(j);
+        }
+        L.exec->run(jobs.data(), static_cast<int>(jobs.size()), L.stream);
+        for (Req& q : reqs) {
+            if (q.ddst)
+                HIPCALL(hipMemcpy2DAsync(q.job.dst.data, host_stride(q.job.dst), q.ddst, packed_stride(q.job.dst),
+                                         packed_stride(q.job.dst), q.job.dst.height, hipMemcpyDeviceToHost, L.stream));
+            if (q.dsrc) HIPCALL(hipFreeAsync(q.dsrc, L.stream));
+            if (q.ddst) HIPCALL(hipFreeAsync(q.ddst, L.stream));
+        }
+        HIPCALL(hipEventRecord(L.done, L.stream));
+        L.reqs = std::move(reqs);
+        L.busy = true;
+        std::lock_guard<std::mutex> lock(mu_);
+        ++batches_;
+        max_jobs_seen_ = std::max(max_jobs_seen_, static_cast<int>(L.reqs.size()));
+    }
+
+    // Completes the requests of finished lanes. Returns whether any finished.
+    bool retire() {
+        bool any = false;
+        for (int i = 0; i < kLanes; ++i) {
+            Lane& L = lane_[i];
+            if (!L.busy) continue;
+            const hipError_t e = hipEventQuery(L.done);
+            if (e == hipErrorNotReady) continue;
+            HIPCALL(e);
+            const auto now = std::chrono::steady_clock::now();
+            std::lock_guard<std::mutex> lock(mu_);
+            for (const Req& q : L.reqs) {
+                in_use_ -= q.need;
+                done_.emplace_back(q.id, std::chrono::duration<double, std::milli>(now - q.t_submit).count());
+                --outstanding_;
+            }
+            L.reqs.clear();
+            L.busy = false;
+            any = true;
+        }
+        if (any) cv_done_.notify_all();
+        return any;
+    }
+
+    void fail(const std::string& msg) {
+        std::lock_guard<std::mutex> lock(mu_);
+        error_ = msg.empty() ? std::string("ipfg: server failed") : msg;
+        cv_done_.notify_all();
+    }
+
+    static size_t host_stride(const ipfg_image& im) {
+        return im.row_stride_bytes ? im.row_stride_bytes : packed_stride(im);
+    }
+
+    size_t budget_;
+    int max_batch_;
+    int device_ = 0;
+    hipStream_t up_ = nullptr;
+    Lane lane_[kLanes];
+    int next_lane_ = 0;
+    std::thread worker_;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    std::deque<Req> pending_;
+    std::deque<std::pair<long long, double>> done_;
+    size_t in_use_ = 0;
+    long long next_id_ = 0;
+    long long outstanding_ = 0;
+    long long batches_ = 0;
+    int max_jobs_seen_ = 0;
+    bool stop_ = false;
+    std::string error_;
+};
+
 }  // namespace
 }  // namespace ipfg
 
@@ -964,6 +1513,66 @@ int ipfg_median(const ipfg_image* src, const ipfg_image* dst, int ksize, int bor
 int ipfg_bilateral(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
                    int border, int precision, ipfg_stream stream) {
     return guarded([&] { bilateral_impl(src, dst, d, sigma_color, sigma_space, border, precision, stream); });
+}
+
+int ipfg_batch_create(ipfg_batch* batch, int max_jobs, int use_graph) {
+    return guarded([&] {
+        abi_check(batch != nullptr, "null batch");
+        *batch = new BatchExecutor(max_jobs, use_graph != 0);
+    });
+}
+
+int ipfg_batch_run(ipfg_batch batch, const ipfg_job* jobs, int njobs, ipfg_stream stream) {
+    return guarded([&] {
+        abi_check(batch != nullptr, "null batch");
+        abi_check(njobs >= 0 && (jobs != nullptr || njobs == 0), "null jobs");
+        ensure_init();
+        static_cast<BatchExecutor*>(batch)->run(jobs, njobs, pick_stream(stream));
+    });
+}
+
+int ipfg_batch_destroy(ipfg_batch batch) {
+    return guarded([&] { delete static_cast<BatchExecutor*>(batch); });
+}
+
+int ipfg_server_create(ipfg_server* server, size_t device_budget_bytes, int max_batch, int use_graph) {
+    return guarded([&] {
+        abi_check(server != nullptr, "null server");
+        *server = new Server(device_budget_bytes, max_batch, use_graph != 0);
+    });
+}
+
+int ipfg_server_submit(ipfg_server server, const ipfg_job* request, long long* id) {
+    return guarded([&] {
+        abi_check(server != nullptr && request != nullptr, "null server or request");
+        const long long v = static_cast<Server*>(server)->submit(*request);
+        if (id) *id = v;
+    });
+}
+
+int ipfg_server_poll(ipfg_server server, long long* ids, double* latency_ms, int capacity, int* count) {
+    return guarded([&] {
+        abi_check(server != nullptr && count != nullptr && (ids != nullptr || capacity == 0), "null argument");
+        *count = static_cast<Server*>(server)->poll(ids, latency_ms, capacity);
+    });
+}
+
+int ipfg_server_drain(ipfg_server server) {
+    return guarded([&] {
+        abi_check(server != nullptr, "null server");
+        static_cast<Server*>(server)->drain();
+    });
+}
+
+int ipfg_server_stats(ipfg_server server, long long* batches, int* max_jobs_seen) {
+    return guarded([&] {
+        abi_check(server != nullptr, "null server");
+        static_cast<Server*>(server)->stats(batches, max_jobs_seen);
+    });
+}
+
+int ipfg_server_destroy(ipfg_server server) {
+    return guarded([&] { delete static_cast<Server*>(server); });
 }
 
 }  // extern "C"

@@ -1,0 +1,70 @@
+// Persistent batched separable filter: certified uint8 -> uint8 strips of any
+// kernel size for 1-, 3- and 4-channel images, popped from a tile queue.
+#include "batch.hpp"
+#include "sep_kernels.hpp"
+
+namespace ipfg {
+namespace sepk {
+namespace {
+
+__global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeader* __restrict__ hdr,
+                                                           const BatchJob* __restrict__ jobs, BatchQueue* q,
+                                                           FixList fix) {
+    extern __shared__ __align__(16) unsigned char sep_smem[];
+    batch_loop(hdr, jobs, q, [&](const BatchJob& J, int tx, int ty) {
+        SepArgs a{};
+        a.src = J.src;
+        a.dst = J.dst;
+        a.kx = J.kx;
+        a.ky = J.ky;
+        a.kxd = J.kx;
+        a.kyd = J.ky;
+        a.kw = J.kw;
+        a.kh = J.kh;
+        a.ax = J.kw / 2;
+        a.ay = J.kh / 2;
+        a.border = J.border;
+        a.delta = 0.0;
+        a.bval = 0.0;
+        a.bcoef = J.bcoef;
+        a.bdelta = J.bdelta;
+        a.xbound = 255.0;
+        a.SH = J.rows;
+        a.fix = fix;
+        switch (J.src.cn) {
+            case 1: sep_plane_strip<uint8_t, uint8_t, double, float, 1, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem); break;
+            case 3: sep_plane_strip<uint8_t, uint8_t, double, float, 3, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem); break;
+            default: sep_plane_strip<uint8_t, uint8_t, double, float, 4, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem); break;
+        }
+    });
+    fix_finish(fix);
+}
+
+}  // namespace
+}  // namespace sepk
+
+hipError_t launch_sep_batch(int cls, const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
+                            int grid, size_t lds, hipStream_t s) {
+    (void)cls;
+    static size_t raised = 0;
+    if (lds > 64 * 1024 && lds > raised) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sepk::sep_batch_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+        raised = lds;
+    }
+    hipLaunchKernelGGL(sepk::sep_batch_kernel, dim3(grid), dim3(kBlock), lds, s, hdr, jobs, q, fix);
+    return hipGetLastError();
+}
+
+int sep_batch_occupancy(size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(sepk::sep_batch_kernel), kBlock,
+                                                     lds) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+size_t sep_batch_lds(int kmax) { return sepk::plane_lds_bytes(4, kmax, kmax, sizeof(float)); }
+
+}  // namespace ipfg

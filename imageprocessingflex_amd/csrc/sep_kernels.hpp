@@ -420,10 +420,13 @@ __device__ __forceinline__ void store_h4(Hr* p, const Acc (&v)[4]) {
     }
 }
 
+// One strip (bx, by): PTX pixels wide, a.SH rows tall. The workgroup's LDS is
+// sep_smem. Certified launches recompute doubtful samples in place
+// (wave_fix); the caller runs fix_finish once per workgroup.
 template <class Ti, class To, class Acc, class Hr, int CN, int SJ, bool CERT, int KS>
-__global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc* __restrict__ kx,
-                                                           const Acc* __restrict__ ky) {
-    extern __shared__ __align__(16) unsigned char sep_smem[];
+__device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __restrict__ kx,
+                                                const Acc* __restrict__ ky, int bx, int by,
+                                                unsigned char* __restrict__ sep_smem) {
     constexpr int G = PTX / P;                  // horizontal tasks per row
     constexpr int U = KS ? 2 : 1;              // unroll of the tap loops (more hoists weights into VGPRs)
     static_assert(CH * G == kBlock && (CH / M) * PTX == kBlock, "one task per thread per plane");
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
     const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
     // the wave index is uniform: keep it (and what derives from it) in scalar registers
     const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * a.SH;
+    const int x0 = bx * PTX, y0 = by * a.SH;
     const int rows_out = min(a.SH, H - y0);
     const int nh = rows_out + kh - 1;
     const int nhstep = (nh + CH - 1) / CH;
@@ -628,6 +631,13 @@ __global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc*
         step(sa, k);
         if (k + 1 < nstep) step(sb, k + 1);
     }
+}
+
+template <class Ti, class To, class Acc, class Hr, int CN, int SJ, bool CERT, int KS>
+__global__ __launch_bounds__(kBlock) void sep_plane_kernel(SepArgs a, const Acc* __restrict__ kx,
+                                                           const Acc* __restrict__ ky) {
+    extern __shared__ __align__(16) unsigned char sep_smem[];
+    sep_plane_strip<Ti, To, Acc, Hr, CN, SJ, CERT, KS>(a, kx, ky, blockIdx.x, blockIdx.y, sep_smem);
     if (CERT) fix_finish(a.fix);
 }
 

@@ -174,6 +174,55 @@ int ipfg_median(const ipfg_image* src, const ipfg_image* dst, int ksize, int bor
 int ipfg_bilateral(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
                    int border, int precision, ipfg_stream stream);
 
+/* ---------------------------------------------------------------- batches
+ * No counterpart in the reference (its filters run one image per call).
+ * A batch runs many uint8 images (1, 3 or 4 channels, any sizes) through
+ * gaussianBlur, medianBlur (3 or 5) and bilateralFilter (radius <= 5) with
+ * one persistent tile-queue launch per operation class; results are the
+ * single-image calls' results bit for bit (default precision). */
+typedef enum ipfg_op { IPFG_OP_GAUSSIAN = 0, IPFG_OP_MEDIAN = 1, IPFG_OP_BILATERAL = 2 } ipfg_op;
+
+typedef struct ipfg_job {
+    ipfg_image src;     /* uint8; device memory for batches, host or device for servers */
+    ipfg_image dst;     /* same shape as src */
+    int op;             /* ipfg_op */
+    int border;         /* ipfg_border (median: not Constant) */
+    double sigma;       /* gaussian: sigmaX = sigmaY */
+    int ksize;          /* gaussian window (0: from sigma) or median window (3 or 5) */
+    int d;              /* bilateral diameter (<= 0: from sigma_space) */
+    double sigma_color; /* bilateral */
+    double sigma_space; /* bilateral */
+} ipfg_job;
+
+typedef void* ipfg_batch;
+/* Creates an executor for batches of up to max_jobs jobs. use_graph != 0
+ * captures the launches of a batch in a hipGraph once and replays it. */
+int ipfg_batch_create(ipfg_batch* batch, int max_jobs, int use_graph);
+/* Enqueues one batch on stream (device-resident images only); asynchronous. */
+int ipfg_batch_run(ipfg_batch batch, const ipfg_job* jobs, int njobs, ipfg_stream stream);
+int ipfg_batch_destroy(ipfg_batch batch);
+
+/* ---------------------------------------------------------------- serving
+ * No counterpart in the reference. A request loop with continuous batching:
+ * a worker thread admits submitted requests in arrival order while their
+ * device buffers fit in device_budget_bytes, uploads host images, groups the
+ * admitted requests into batches of at most max_batch jobs and keeps two
+ * batches in flight on two streams (requests that arrive while a batch runs
+ * join the next one), downloads host results and frees the buffers. */
+typedef void* ipfg_server;
+int ipfg_server_create(ipfg_server* server, size_t device_budget_bytes, int max_batch, int use_graph);
+/* Queues a request (the buffers must stay valid until it completes); *id
+ * receives its number (0, 1, ...). Requests larger than the budget fail. */
+int ipfg_server_submit(ipfg_server server, const ipfg_job* request, long long* id);
+/* Completed requests since the last poll: ids and latency (submit to result
+ * in place, milliseconds); *count receives how many were written. */
+int ipfg_server_poll(ipfg_server server, long long* ids, double* latency_ms, int capacity, int* count);
+/* Blocks until every submitted request has completed. */
+int ipfg_server_drain(ipfg_server server);
+/* Batches launched so far and the largest number of jobs in one batch. */
+int ipfg_server_stats(ipfg_server server, long long* batches, int* max_jobs_seen);
+int ipfg_server_destroy(ipfg_server server);
+
 #ifdef __cplusplus
 }
 #endif

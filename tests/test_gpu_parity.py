@@ -371,3 +371,94 @@ def test_filter2d_mfma_u8_output_close_to_reference(ipf, oracle, precision):
     got = ipf.filter2d(img, kernel, 3, precision=precision)
     d = np.abs(got.astype(int) - want.astype(int))
     assert d.max() <= 1 and (d > 0).mean() < 0.01
+
+
+# ------------------------------------------------------------- batches and serving
+def _mixed_jobs(rng, n):
+    shapes = [(17, 23, 1), (64, 64, 3), (130, 257, 4), (300, 200, 1), (5, 700, 3), (96, 33, 4), (1, 1, 1),
+              (255, 129, 3), (40, 41, 1)]
+    out = []
+    for i in range(n):
+        h, w, cn = shapes[i % len(shapes)]
+        img = rng.integers(0, 256, (h, w, cn) if cn > 1 else (h, w), dtype=np.uint8)
+        kind = i % 4
+        if kind in (0, 1):
+            sigma = [0.5, 2.0, 5.3, 8.0, 1.1][i % 5]
+            out.append((img, "gaussian", dict(sigma=sigma, border=(i // 4) % 5)))
+        elif kind == 2:
+            out.append((img, "median", dict(ksize=5 if i % 8 == 2 else 3, border=1 + (i // 4) % 4)))
+        else:
+            out.append((img, "bilateral", dict(d=9 if i % 8 == 3 else 5, sigma_color=75.0 if i % 8 == 3 else 20.0,
+                                               sigma_space=75.0 if i % 8 == 3 else 10.0, border=(i // 4) % 5)))
+    return out
+
+
+def _oracle_result(oracle, img, op, kw):
+    if op == "gaussian":
+        return oracle.gaussian_blur(img, kw["sigma"], border=kw["border"])
+    if op == "median":
+        return oracle.median(img, kw["ksize"], kw["border"])
+    return oracle.bilateral(img, kw["d"], kw["sigma_color"], kw["sigma_space"], kw["border"])
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_batch_matches_single_image_results(ipf, oracle, graph):
+    rng = np.random.default_rng(41)
+    items = _mixed_jobs(rng, 36)
+    want = [_oracle_result(oracle, img, op, kw) for img, op, kw in items]
+    s = ipf.Stream()
+    srcs = [ipf.DeviceImage.from_array(img, s) for img, _, _ in items]
+    b = ipf.Batch(64, graph=graph)
+    for rep in range(3):  # replays of the same captured graph, with other work in between
+        dsts = [ipf.DeviceImage.like(d) for d in srcs]
+        jobs = [ipf.job(d, o, op, **kw) for d, o, (_, op, kw) in zip(srcs, dsts, items)]
+        b.run(jobs, stream=s)
+        ipf.gaussian_blur(srcs[1], 1.5, stream=s)
+        s.synchronize()
+        for i, (o, w_) in enumerate(zip(dsts, want)):
+            assert np.array_equal(o.download(s).reshape(w_.shape), w_), (rep, i, items[i][1], items[i][2])
+    b.close()
+
+
+def test_batch_rejects_unsupported_jobs(ipf):
+    s = ipf.Stream()
+    a = ipf.DeviceImage.from_array(np.zeros((8, 8), np.uint8), s)
+    o = ipf.DeviceImage.like(a)
+    b = ipf.Batch(4)
+    with pytest.raises(ipf.IpfError, match="medianBlur: Constant border is not supported"):
+        b.run([ipf.job(a, o, "median", ksize=3, border=0)], stream=s)
+    f = ipf.DeviceImage.from_array(np.zeros((8, 8), np.float32), s)
+    with pytest.raises(ipf.IpfError, match="uint8"):
+        b.run([ipf.job(f, ipf.DeviceImage.like(f), "gaussian", sigma=1.0)], stream=s)
+    b.close()
+
+
+@pytest.mark.parametrize("budget_mb,graph", [(256, True), (3, True), (64, False)], ids=["roomy", "tight", "eager"])
+def test_server_results_same_alone_and_in_the_mix(ipf, oracle, budget_mb, graph):
+    # every request's result equals the single-image (reference) result, whatever
+    # it was batched with and however the device budget throttled admission
+    rng = np.random.default_rng(42)
+    items = _mixed_jobs(rng, 45)
+    srv = ipf.Server(budget_mb << 20, max_batch=16, graph=graph)
+    outs = []
+    for img, op, kw in items:
+        out = np.empty_like(img)
+        outs.append(out)
+        srv.submit(img, out, op, **kw)
+    srv.drain()
+    done = srv.poll()
+    assert sorted(i for i, _ in done) == list(range(len(items)))
+    assert all(lat >= 0 for _, lat in done)
+    batches, biggest = srv.stats()
+    assert 1 <= batches <= len(items) and 1 <= biggest <= 16
+    for i, ((img, op, kw), out) in enumerate(zip(items, outs)):
+        assert np.array_equal(out, _oracle_result(oracle, img, op, kw)), (i, op, kw)
+    srv.close()
+
+
+def test_server_rejects_request_above_budget(ipf):
+    srv = ipf.Server(1 << 10, max_batch=4)
+    img = np.zeros((64, 64), np.uint8)
+    with pytest.raises(ipf.IpfError, match="larger than the device budget"):
+        srv.submit(img, np.empty_like(img), "gaussian", sigma=1.0)
+    srv.close()
