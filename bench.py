@@ -179,7 +179,209 @@ def _c4():
             "keep": kp, "post": accuracy, "kernel_raw_sum": ksum}
 
 
-WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4}
+WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None}
+
+C5_BUDGET = 256 << 20  # device-memory budget of the batched server (>= the largest request, 2 x 67 MB)
+C5_MAX_BATCH = 256
+
+
+def _pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(round(q * (len(v) - 1))))] if v else None
+
+
+def run_c5(args, ipf):
+    """Serving stream: plain per-image launches vs the batched server.
+
+    Requests (project RNG, seed 6): Poisson arrivals, W and H independently
+    log-uniform in [64, 4096], channels in {1, 3, 4}, uint8; 50% gaussianBlur
+    (sigma ~ U[0.5, 8], Reflect101), 25% medianBlur 5 (Replicate), 25%
+    bilateralFilter(9, 75, 75) (Reflect101). Sources are device-resident
+    views of seeded uniform banks (4096 x 4096 x cn); outputs are device
+    buffers. Throughput is measured closed-loop (all requests available);
+    latency open-loop at 0.8x and 1.2x the plain path's capacity.
+    """
+    from imageprocessingflex_amd import workloads as wl
+    n = args.requests
+    reqs = wl.serving_stream(n, rate=1.0, seed=6)  # unit-rate arrival times, scaled per run
+    t0 = time.time()
+    banks = {cn: ipf.DeviceImage.from_array(wl.squeeze(wl.uniform_u8((4096, 4096, cn), seed=6, stream=cn)))
+             for cn in (1, 3, 4)}
+    outbank = {cn: ipf.DeviceImage.like(banks[cn]) for cn in (1, 3, 4)}
+    print(f"# c5 banks ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    lib = ipf._lib.load()
+    views = []
+    for r in reqs:
+        oy = (r.rid * 7919) % (4096 - r.height + 1)
+        ox = (r.rid * 104729) % (4096 - r.width + 1)
+        views.append((ipf.device_view(banks[r.channels], oy, ox, r.height, r.width),
+                      ipf.device_view(outbank[r.channels], oy, ox, r.height, r.width)))
+    px = sum(r.height * r.width for r in reqs)
+    stream = ipf.Stream()
+    ev = HipEvents()
+    ipf.autotune(False)  # plain calls of many shape classes: built-in launch shapes, no first-call measuring
+
+    def plain_call(i):
+        r = reqs[i]
+        s, d = views[i]
+        if r.op == "gaussian":
+            rc = lib.ipfg_gaussian_blur(C.byref(s), C.byref(d), r.sigma, 0.0, 0, 3, 0, stream.handle)
+        elif r.op == "median5":
+            rc = lib.ipfg_median(C.byref(s), C.byref(d), 5, 1, stream.handle)
+        else:
+            rc = lib.ipfg_bilateral(C.byref(s), C.byref(d), 9, 75.0, 75.0, 3, 0, stream.handle)
+        ipf._lib.check(rc)
+
+    # ---- plain path, closed loop: one blocking call per image (the drop-in
+    # API's contract), kernel time from events between calls
+    for i in range(min(64, n)):  # warm-up: parameter caches
+        plain_call(i)
+    stream.synchronize()
+    evs = ev.make(n + 1)
+    ipf.synchronize()
+    w0 = time.perf_counter()
+    ev.record(evs[0], stream.handle)
+    for i in range(n):
+        plain_call(i)
+        ev.record(evs[i + 1], stream.handle)
+        stream.synchronize()
+    plain_wall = time.perf_counter() - w0
+    dur = [ev.ms(evs[i], evs[i + 1]) for i in range(n)]
+    plain_rate = n / plain_wall
+    # the same calls enqueued back to back without waiting (one stream)
+    ipf.synchronize()
+    w0 = time.perf_counter()
+    for i in range(n):
+        plain_call(i)
+    stream.synchronize()
+    async_wall = time.perf_counter() - w0
+    print(f"# plain: {plain_wall:.2f} s ({px / plain_wall / 1e9:.2f} Gpix/s), back-to-back {async_wall:.2f} s",
+          file=sys.stderr, flush=True)
+
+    def plain_latency(rate):
+        # a single FIFO queue in front of the measured per-call times (+ the
+        # mean host overhead per call)
+        over = max(0.0, plain_wall - sum(dur) / 1e3) / n
+        free, lat = 0.0, []
+        for r, d in zip(reqs, dur):
+            a = r.arrival / rate
+            start = max(a, free)
+            free = start + d / 1e3 + over
+            lat.append((free - a) * 1e3)
+        return lat
+
+    def server_run(rate, graph=True):
+        srv = ipf.Server(C5_BUDGET, max_batch=C5_MAX_BATCH, graph=graph)
+        jobs = []
+        for r, (s, _) in zip(reqs, views):
+            if r.op == "gaussian":
+                jobs.append(ipf.job(s, None, "gaussian", sigma=r.sigma, border=3))
+            elif r.op == "median5":
+                jobs.append(ipf.job(s, None, "median", ksize=5, border=1))
+            else:
+                jobs.append(ipf.job(s, None, "bilateral", d=9, sigma_color=75.0, sigma_space=75.0, border=3))
+        # warm-up batch (graph capture, parameter caches)
+        srv.submit_jobs(jobs[:min(64, n)])
+        srv.drain()
+        srv.poll(1 << 16)
+        lat = []
+        w0 = time.perf_counter()
+        if rate is None:
+            srv.submit_jobs(jobs)
+        else:
+            i = 0
+            while i < n:
+                now = time.perf_counter() - w0
+                j = i
+                while j < n and reqs[j].arrival / rate <= now:
+                    j += 1
+                if j > i:
+                    srv.submit_jobs(jobs[i:j])
+                    i = j
+                else:
+                    lat += [l for _, l in srv.poll(1 << 16)]
+                    time.sleep(min(0.0005, max(0.0, reqs[i].arrival / rate - now)))
+        srv.drain()
+        wall = time.perf_counter() - w0
+        lat += [l for _, l in srv.poll(1 << 16)]
+        batches, biggest = srv.stats()
+        srv.close()
+        return wall, lat, batches, biggest
+
+    bw, _, bb, bmax = server_run(None)
+    print(f"# batched: {bw:.2f} s, {px / bw / 1e9:.2f} Gpix/s, {bb} batches", file=sys.stderr, flush=True)
+    loads = {}
+    for f in (0.8, 1.2):
+        rate = f * plain_rate
+        wall, lat, batches, _ = server_run(rate)
+        pl = plain_latency(rate)
+        loads[f"{f}x_plain_capacity"] = {
+            "arrival_rate_req_s": round(rate, 1),
+            "batched": {"p50_ms": round(_pct(lat, 0.5), 3), "p99_ms": round(_pct(lat, 0.99), 3),
+                        "gpix_s": round(px / wall / 1e9, 2), "batches": batches},
+            "plain_fifo_model": {"p50_ms": round(_pct(pl, 0.5), 3), "p99_ms": round(_pct(pl, 0.99), 3)},
+        }
+    value = px / bw / 1e9
+    cpu = None if args.no_cpu else c5_cpu_baseline(reqs, banks, args.cpu_budget)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(bw * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded project RNG; device-resident source banks)",
+        "config": {"workload": f"serving stream, {n} requests: 50% gaussianBlur sigma~U[0.5,8], 25% median 5x5, "
+                               "25% bilateral d=9; W,H log-uniform [64,4096]; cn in {1,3,4}; uint8",
+                   "device_budget_bytes": C5_BUDGET, "max_batch": C5_MAX_BATCH, "requests": n,
+                   "total_pixels": px},
+        "batched_vs_plain": {"batched_gpix_s": round(value, 2), "plain_gpix_s": round(px / plain_wall / 1e9, 2),
+                             "speedup": round(plain_wall / bw, 2), "batches": bb, "largest_batch": bmax,
+                             "plain_kernel_ms_total": round(sum(dur), 2),
+                             "plain_back_to_back_gpix_s": round(px / async_wall / 1e9, 2),
+                             "speedup_vs_back_to_back": round(async_wall / bw, 2),
+                             "plain": "one blocking call per image (ipfg_* on device buffers + stream sync)"},
+        "latency": loads,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def c5_cpu_baseline(reqs, banks, budget_s):
+    """The reference's CPU path on the serving stream's requests in arrival
+    order (each op with all host threads), on the prefix that fits the budget."""
+    try:
+        from oracle.pyoracle import Oracle, ref_available
+        o = Oracle("ref") if ref_available() else Oracle("oracle")
+    except Exception as e:
+        return {"value": None, "unit": "Gpix/s", "cores": os.cpu_count(), "kind": None, "sample": f"unavailable: {e}"}
+    o.set_num_threads(0)
+    host = {cn: b.download() for cn, b in banks.items()}
+    t_total, px, done = 0.0, 0, 0
+    for r in reqs:
+        oy = (r.rid * 7919) % (4096 - r.height + 1)
+        ox = (r.rid * 104729) % (4096 - r.width + 1)
+        img = np.ascontiguousarray(host[r.channels][oy:oy + r.height, ox:ox + r.width])
+        t0 = time.perf_counter()
+        if r.op == "gaussian":
+            o.gaussian_blur(img, r.sigma, border=3)
+        elif r.op == "median5":
+            o.median(img, 5, 1)
+        else:
+            o.bilateral(img, 9, 75.0, 75.0, 3)
+        t_total += time.perf_counter() - t0
+        px += r.height * r.width
+        done += 1
+        if t_total > budget_s:
+            break
+    return {"value": round(px / t_total / 1e9, 5), "unit": "Gpix/s", "cores": o.get_num_threads(),
+            "kind": "reference" if o.which == "ref" else "port",
+            "sample": f"first {done} requests of the stream in arrival order, {t_total:.1f} s"}
 
 
 def cpu_baseline(wd, budget_s: float):
@@ -225,6 +427,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--requests", type=int, default=20000, help="c5: number of requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}).",
@@ -234,6 +437,9 @@ def main():
     import imageprocessingflex_amd as ipf
 
     ipf.init(0)
+    if args.workload == "c5":
+        run_c5(args, ipf)
+        return
     t0 = time.time()
     wd = WORKLOADS[args.workload]()
     print(f"# {args.workload} inputs ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)

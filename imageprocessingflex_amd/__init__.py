@@ -298,10 +298,28 @@ def job(src, dst, op: str, border=BorderMode.REFLECT101, sigma=0.0, ksize=0, d=0
     dst are uint8 DeviceImages (batches) or contiguous numpy arrays / DeviceImages
     (servers)."""
     ops = {"gaussian": _lib.OP_GAUSSIAN, "median": _lib.OP_MEDIAN, "bilateral": _lib.OP_BILATERAL}
-    sd = src.descriptor() if isinstance(src, DeviceImage) else _desc(np.ascontiguousarray(src))
-    dd = dst.descriptor() if isinstance(dst, DeviceImage) else _desc(dst)
+
+    def d_of(a):
+        if isinstance(a, _lib.IpfgImage):
+            return a
+        if isinstance(a, DeviceImage):
+            return a.descriptor()
+        return _desc(np.ascontiguousarray(a))
+
+    sd = d_of(src)
+    # dst None: a server-owned device output (dropped on completion)
+    dd = _lib.IpfgImage(None, sd.width, sd.height, sd.channels, _lib.U8, 0, 1) if dst is None else d_of(dst)
     return _lib.IpfgJob(sd, dd, ops[op], int(border), float(sigma), int(ksize), int(d), float(sigma_color),
                         float(sigma_space))
+
+
+def device_view(base: DeviceImage, y: int, x: int, h: int, w: int) -> _lib.IpfgImage:
+    """Descriptor of the h x w sub-image of a device image at (y, x) (row stride of base)."""
+    assert 0 <= y and 0 <= x and y + h <= base.height and x + w <= base.width
+    es = base.dtype.itemsize
+    row = base.width * base.channels * es
+    ptr = base.ptr.value + y * row + x * base.channels * es
+    return _lib.IpfgImage(ptr, w, h, base.channels, _NP2DT[base.dtype], row, 1)
 
 
 class Batch:
@@ -343,6 +361,14 @@ class Server:
         _lib.check(_lib.load().ipfg_server_submit(self.handle, C.byref(j), C.byref(rid)))
         self._keep[rid.value] = (src, dst)  # buffers stay alive until completion
         return rid.value
+
+    def submit_jobs(self, jobs) -> int:
+        """Queues prepared jobs (see job()) in order; returns the first id. The
+        caller keeps their buffers alive until they complete."""
+        arr = (_lib.IpfgJob * len(jobs))(*jobs)
+        first = C.c_longlong(-1)
+        _lib.check(_lib.load().ipfg_server_submit_many(self.handle, arr, len(jobs), C.byref(first)))
+        return first.value
 
     def poll(self, capacity: int = 4096):
         ids = (C.c_longlong * capacity)()

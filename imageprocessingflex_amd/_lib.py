@@ -90,6 +90,7 @@ SIGNATURES = {
     "ipfg_batch_destroy": (_I, [_V]),
     "ipfg_server_create": (_I, [_P(_V), C.c_size_t, _I, _I]),
     "ipfg_server_submit": (_I, [_V, _JOB, _P(C.c_longlong)]),
+    "ipfg_server_submit_many": (_I, [_V, _JOB, _I, _P(C.c_longlong)]),
     "ipfg_server_poll": (_I, [_V, _P(C.c_longlong), _P(_D), _I, _P(_I)]),
     "ipfg_server_drain": (_I, [_V]),
     "ipfg_server_stats": (_I, [_V, _P(C.c_longlong), _P(_I)]),

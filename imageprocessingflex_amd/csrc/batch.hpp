@@ -17,11 +17,17 @@
 
 namespace ipfg {
 
-enum BatchClass { BCLS_SEP_SMALL = 0, BCLS_SEP_LARGE = 1, BCLS_MEDIAN = 2, BCLS_BILATERAL = 3, BCLS_COUNT = 4 };
-
-// Largest separable kernel of each class (the classes differ in ring rows).
+// Separable jobs are split by channel count and kernel size so that each
+// class's LDS (and so its occupancy) fits its own jobs: classes 0..5 are
+// (cn 1, 3, 4) x (kernel <= 17, kernel <= 49).
+enum BatchClass { BCLS_SEP = 0, BCLS_MEDIAN = 6, BCLS_BILATERAL = 7, BCLS_COUNT = 8 };
 constexpr int kBatchSepSmallK = 17;
 constexpr int kBatchSepLargeK = 49;
+__host__ __device__ inline int batch_sep_class(int cn, int k) {
+    return 2 * (cn == 1 ? 0 : cn == 3 ? 1 : 2) + (k <= kBatchSepSmallK ? 0 : 1);
+}
+__host__ __device__ inline int batch_sep_cn(int cls) { return cls / 2 == 0 ? 1 : cls / 2 == 1 ? 3 : 4; }
+__host__ __device__ inline int batch_sep_kmax(int cls) { return cls % 2 ? kBatchSepLargeK : kBatchSepSmallK; }
 constexpr int kBatchTileRows = 32;  // TH of median and bilateral tiles
 
 struct BatchJob {
@@ -105,8 +111,8 @@ hipError_t launch_median_batch(const BatchClassHeader* hdr, const BatchJob* jobs
 hipError_t launch_bilateral_batch(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
                                   int grid, size_t lds, hipStream_t s);
 // Occupancy (workgroups per CU at the given LDS) and LDS of the class kernels.
-int sep_batch_occupancy(size_t lds);
-size_t sep_batch_lds(int kmax);
+int sep_batch_occupancy(int cn, size_t lds);
+size_t sep_batch_lds(int cn, int kmax);
 int median_batch_occupancy(size_t lds);
 size_t median_batch_lds();
 int bilateral_batch_occupancy(size_t lds);

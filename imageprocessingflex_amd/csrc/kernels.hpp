@@ -22,7 +22,8 @@ struct SepLaunch {
     // MODE_CERTIFY: |FP64-FMA result - reference| <= bcoef * X + bdelta where
     // X bounds |input| (xbound, or measured per tile when xbound < 0)
     double bcoef, bdelta, xbound;
-    double kabs;  // sum|kx| * sum|ky|
+    double kabs;  // sum|kx| * sum|ky| (0: the ring holds exact sums)
+    double oscale;  // results are multiplied by this (0 means 1)
     FixList fix;
     double* tmp;  // MODE_EXACT: (H + kh - 1) * W * cn doubles
     int sh;       // strip height of the fast kernels (0: built-in choice)

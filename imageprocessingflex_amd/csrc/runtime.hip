@@ -574,6 +574,61 @@ void box_impl(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int 
     a.mode = MODE_FAST;
     if (precision == IPFG_PREC_EXACT) a.mode = MODE_EXACT;
     else if (!exact_sums && precision == IPFG_PREC_DEFAULT && dtype_is_int(dst->dtype)) a.mode = MODE_EXACT;
+    const double scale = normalize ? 1.0 / (static_cast<double>(kw) * kh) : 1.0;
+    // Window sums on the separable kernels: unit weights, FP64 accumulation
+    // (exact integer sums for integer images, so that sum * scale rounds
+    // like the reference's), the scale applied to each result.
+    const bool sep_ok = a.mode == MODE_FAST && dtype_is_int(dst->dtype) == dtype_is_int(src->dtype) &&
+                        (src->dtype == IPFG_U8 || src->dtype == IPFG_U16 || src->dtype == IPFG_I16 ||
+                         src->dtype == IPFG_F32) &&
+                        sep_fast_fits(src->channels, kw, kh, src->dtype, dst->dtype, precision != IPFG_PREC_FP32);
+    if (sep_ok) {
+        SepLaunch l{};
+        l.src = a.src;
+        l.dst = a.dst;
+        l.kw = kw;
+        l.kh = kh;
+        l.ax = kw / 2;
+        l.ay = kh / 2;
+        l.border = border;
+        l.delta = 0.0;
+        l.bval = bval;
+        l.oscale = scale;
+        l.mode = precision == IPFG_PREC_FP32 ? MODE_FAST : MODE_CERTIFY;
+        constexpr int kPad = 16;
+        const int nx = kw + kPad, ny = kh + kPad;
+        std::vector<char> blob((sizeof(double) + sizeof(float)) * (nx + ny), 0);
+        double* bd = reinterpret_cast<double*>(blob.data());
+        float* bf = reinterpret_cast<float*>(bd + nx + ny);
+        for (int i = 0; i < kw; ++i) bd[i] = bf[i] = 1.0f;
+        for (int j = 0; j < kh; ++j) bd[nx + j] = bf[nx + j] = 1.0f;
+        char* dev = static_cast<char*>(sc.params(blob));
+        l.kxd = reinterpret_cast<const double*>(dev);
+        l.kyd = l.kxd + nx;
+        l.kxf = reinterpret_cast<const float*>(l.kxd + nx + ny);
+        l.kyf = l.kxf + nx;
+        if (l.mode == MODE_CERTIFY) {
+            // integer sums are exact (also in a float ring: they stay below
+            // 2^24); only results exactly on a rounding tie are recomputed
+            l.bcoef = 0.0;
+            l.kabs = 0.0;
+            l.bdelta = 1e-300;
+            l.xbound = 1.0;
+            l.fix = sc.fixlist();
+        }
+        const TuneKey key{"sep", src->dtype, dst->dtype, src->channels, kw, kh, l.mode, ceil_log2(src->width),
+                          ceil_log2(src->height)};
+        l.sh = g_tuner.pick(key, static_cast<long long>(src->width) * src->height * src->channels,
+                            sep_sh_candidates(src->height), s, [&](int v) {
+                                SepLaunch t = l;
+                                t.sh = v;
+                                return launch_sep(t, s);
+                            });
+        HIPCALL(launch_sep(l, s));
+        t_ctx.fix_valid = false;
+        io.finish();
+        return;
+    }
     if (a.mode == MODE_EXACT) a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
     HIPCALL(launch_box(a, s));
     t_ctx.fix_valid = false;
@@ -820,12 +875,9 @@ bool images_overlap(const ipfg_image& a, const ipfg_image& b) {
 }  // namespace
 
 size_t batch_lds_bytes(int cls) {
-    switch (cls) {
-        case BCLS_SEP_SMALL: return sep_batch_lds(kBatchSepSmallK);
-        case BCLS_SEP_LARGE: return sep_batch_lds(kBatchSepLargeK);
-        case BCLS_MEDIAN: return median_batch_lds();
-        default: return bilateral_batch_lds();
-    }
+    if (cls < BCLS_MEDIAN) return sep_batch_lds(batch_sep_cn(cls), batch_sep_kmax(cls));
+    if (cls == BCLS_MEDIAN) return median_batch_lds();
+    return bilateral_batch_lds();
 }
 
 int batch_grid(int cls, size_t lds) {
@@ -833,12 +885,9 @@ int batch_grid(int cls, size_t lds) {
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     int per_cu = 1;
-    switch (cls) {
-        case BCLS_SEP_SMALL:
-        case BCLS_SEP_LARGE: per_cu = sep_batch_occupancy(lds); break;
-        case BCLS_MEDIAN: per_cu = median_batch_occupancy(lds); break;
-        default: per_cu = bilateral_batch_occupancy(lds); break;
-    }
+    if (cls < BCLS_MEDIAN) per_cu = sep_batch_occupancy(batch_sep_cn(cls), lds);
+    else if (cls == BCLS_MEDIAN) per_cu = median_batch_occupancy(lds);
+    else per_cu = bilateral_batch_occupancy(lds);
     return std::max(1, cus) * per_cu;
 }
 
@@ -846,7 +895,7 @@ void batch_tiles(int cls, int w, int h, int cn, int k, int* tiles_x, int* tiles_
     (void)cn;
     (void)k;
     *tiles_x = (w + 63) / 64;
-    if (cls == BCLS_SEP_SMALL || cls == BCLS_SEP_LARGE) {
+    if (cls < BCLS_MEDIAN) {
         *rows = h <= 128 ? (h + 15) / 16 * 16 : 128;  // strip height
     } else {
         *rows = kBatchTileRows;
@@ -868,9 +917,19 @@ size_t image_bytes(const ipfg_image& im) {
     return packed_stride(im) * static_cast<size_t>(im.height);
 }
 
+// A server request may leave dst.data null with on_device set: the server
+// then allocates the output in device memory and drops it on completion.
+bool sink_output(const ipfg_job& j) { return j.dst.on_device && j.dst.data == nullptr; }
+
 void check_job(const ipfg_job& j, bool device_only) {
     check_image(&j.src, "null src image");
-    check_image(&j.dst, "null dst image");
+    if (device_only || !sink_output(j)) {
+        check_image(&j.dst, "null dst image");
+    } else {
+        ipfg_image d = j.dst;
+        d.data = j.src.data;  // checked for shape only
+        check_image(&d, "null dst image");
+    }
     abi_check(j.src.dtype == IPFG_U8 && j.dst.dtype == IPFG_U8, "batched jobs take uint8 images");
     abi_check(j.src.channels == 1 || j.src.channels == 3 || j.src.channels == 4, "batched jobs take 1, 3 or 4 channels");
     abi_check(j.src.width == j.dst.width && j.src.height == j.dst.height && j.src.channels == j.dst.channels,
@@ -878,7 +937,8 @@ void check_job(const ipfg_job& j, bool device_only) {
     abi_check(j.src.width > 0 && j.src.height > 0, "empty image in a batch");
     abi_check(j.border >= IPFG_BORDER_CONSTANT && j.border <= IPFG_BORDER_WRAP, "invalid border mode");
     if (device_only) abi_check(j.src.on_device && j.dst.on_device, "batch images must be device-resident");
-    abi_check(j.src.on_device != j.dst.on_device || !images_overlap(j.src, j.dst), "dst must not alias src");
+    abi_check(j.src.on_device != j.dst.on_device || sink_output(j) || !images_overlap(j.src, j.dst),
+              "dst must not alias src");
     switch (j.op) {
         case IPFG_OP_GAUSSIAN: break;
         case IPFG_OP_MEDIAN:
@@ -903,7 +963,7 @@ void build_job(const ipfg_job& j, BatchJob& out, JobParams& p) {
         const std::vector<double> k = gaussian_kernel(j.sigma, j.ksize);
         const int kw = static_cast<int>(k.size());
         if (kw > kBatchSepLargeK) throw Unsupported{"ipfg: batched gaussian takes windows up to 49"};
-        p.cls = kw <= kBatchSepSmallK ? BCLS_SEP_SMALL : BCLS_SEP_LARGE;
+        p.cls = batch_sep_class(cn, kw);
         constexpr int kPad = 16;
         p.blob.assign(sizeof(double) * 2 * (kw + kPad), 0);
         double* d = reinterpret_cast<double*>(p.blob.data());
@@ -1045,7 +1105,7 @@ private:
             const auto* hdr = reinterpret_cast<const BatchClassHeader*>(d + per_class_ * c);
             const auto* jobs = reinterpret_cast<const BatchJob*>(d + per_class_ * c + sizeof(BatchClassHeader));
             BatchQueue* q = queues_ + c;
-            if (c == BCLS_SEP_SMALL || c == BCLS_SEP_LARGE)
+            if (c < BCLS_MEDIAN)
                 HIPCALL(launch_sep_batch(c, hdr, jobs, q, fix_, grid_[c], lds_[c], s));
             else if (c == BCLS_MEDIAN)
                 HIPCALL(launch_median_batch(hdr, jobs, q, grid_[c], lds_[c], s));
@@ -1181,7 +1241,8 @@ private:
     };
 
     static size_t device_need(const ipfg_job& r) {
-        return (r.src.on_device ? 0 : image_bytes(r.src)) + (r.dst.on_device ? 0 : image_bytes(r.dst));
+        return (r.src.on_device ? 0 : image_bytes(r.src)) +
+               (r.dst.on_device && !sink_output(r) ? 0 : image_bytes(r.dst));
     }
 
     void loop() {
@@ -1262,7 +1323,7 @@ private:
                 j.src.row_stride_bytes = packed_stride(q.job.src);
                 j.src.on_device = 1;
             }
-            if (!j.dst.on_device) {
+            if (!j.dst.on_device || sink_output(j)) {
                 HIPCALL(hipMallocAsync(&q.ddst, image_bytes(j.dst), L.stream));
                 j.dst.data = q.ddst;
                 j.dst.row_stride_bytes = packed_stride(q.job.dst);
@@ -1272,7 +1333,7 @@ private:
         }
         L.exec->run(jobs.data(), static_cast<int>(jobs.size()), L.stream);
         for (Req& q : reqs) {
-            if (q.ddst)
+            if (q.ddst && !q.job.dst.on_device)
                 HIPCALL(hipMemcpy2DAsync(q.job.dst.data, host_stride(q.job.dst), q.ddst, packed_stride(q.job.dst),
                                          packed_stride(q.job.dst), q.job.dst.height, hipMemcpyDeviceToHost, L.stream));
             if (q.dsrc) HIPCALL(hipFreeAsync(q.dsrc, L.stream));
@@ -1547,6 +1608,16 @@ int ipfg_server_submit(ipfg_server server, const ipfg_job* request, long long* i
         abi_check(server != nullptr && request != nullptr, "null server or request");
         const long long v = static_cast<Server*>(server)->submit(*request);
         if (id) *id = v;
+    });
+}
+
+int ipfg_server_submit_many(ipfg_server server, const ipfg_job* requests, int n, long long* first_id) {
+    return guarded([&] {
+        abi_check(server != nullptr && (requests != nullptr || n == 0) && n >= 0, "null server or requests");
+        for (int i = 0; i < n; ++i) {
+            const long long v = static_cast<Server*>(server)->submit(requests[i]);
+            if (i == 0 && first_id) *first_id = v;
+        }
     });
 }
 

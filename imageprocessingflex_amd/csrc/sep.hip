@@ -151,6 +151,7 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     a.xbound = l.xbound;
     a.fix = l.fix;
     a.SH = l.sh;
+    a.oscale = l.oscale != 0.0 ? l.oscale : 1.0;
     const int cn = l.src.cn;
     const bool planar = sep_planar(cn, l.src.dt, l.dst.dt);
     if (planar && l.src.dt == IPFG_U8) {

@@ -7,6 +7,7 @@ namespace ipfg {
 namespace sepk {
 namespace {
 
+template <int CN>
 __global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeader* __restrict__ hdr,
                                                            const BatchJob* __restrict__ jobs, BatchQueue* q,
                                                            FixList fix) {
@@ -30,12 +31,9 @@ __global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeade
         a.bdelta = J.bdelta;
         a.xbound = 255.0;
         a.SH = J.rows;
+        a.oscale = 1.0;
         a.fix = fix;
-        switch (J.src.cn) {
-            case 1: sep_plane_strip<uint8_t, uint8_t, double, float, 1, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem); break;
-            case 3: sep_plane_strip<uint8_t, uint8_t, double, float, 3, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem); break;
-            default: sep_plane_strip<uint8_t, uint8_t, double, float, 4, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem); break;
-        }
+        sep_plane_strip<uint8_t, uint8_t, double, float, CN, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem);
     });
     fix_finish(fix);
 }
@@ -43,28 +41,41 @@ __global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeade
 }  // namespace
 }  // namespace sepk
 
-hipError_t launch_sep_batch(int cls, const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
-                            int grid, size_t lds, hipStream_t s) {
-    (void)cls;
+template <int CN>
+static hipError_t launch_cn(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix, int grid,
+                            size_t lds, hipStream_t s) {
     static size_t raised = 0;
+    auto kern = sepk::sep_batch_kernel<CN>;
     if (lds > 64 * 1024 && lds > raised) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sepk::sep_batch_kernel),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
         raised = lds;
     }
-    hipLaunchKernelGGL(sepk::sep_batch_kernel, dim3(grid), dim3(kBlock), lds, s, hdr, jobs, q, fix);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, hdr, jobs, q, fix);
     return hipGetLastError();
 }
 
-int sep_batch_occupancy(size_t lds) {
+hipError_t launch_sep_batch(int cls, const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
+                            int grid, size_t lds, hipStream_t s) {
+    switch (batch_sep_cn(cls)) {
+        case 1: return launch_cn<1>(hdr, jobs, q, fix, grid, lds, s);
+        case 3: return launch_cn<3>(hdr, jobs, q, fix, grid, lds, s);
+        default: return launch_cn<4>(hdr, jobs, q, fix, grid, lds, s);
+    }
+}
+
+int sep_batch_occupancy(int cn, size_t lds) {
+    const void* k = cn == 1 ? reinterpret_cast<const void*>(sepk::sep_batch_kernel<1>)
+                  : cn == 3 ? reinterpret_cast<const void*>(sepk::sep_batch_kernel<3>)
+                            : reinterpret_cast<const void*>(sepk::sep_batch_kernel<4>);
+    if (lds > 64 * 1024)  // the attribute must allow the LDS before occupancy is asked
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(sepk::sep_batch_kernel), kBlock,
-                                                     lds) != hipSuccess)
-        return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, lds) != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 
-size_t sep_batch_lds(int kmax) { return sepk::plane_lds_bytes(4, kmax, kmax, sizeof(float)); }
+size_t sep_batch_lds(int cn, int kmax) { return sepk::plane_lds_bytes(cn, kmax, kmax, sizeof(float)); }
 
 }  // namespace ipfg

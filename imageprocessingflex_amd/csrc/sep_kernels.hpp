@@ -117,6 +117,7 @@ struct SepArgs {
     double bcoef;        // certification bound = bcoef * X + bdelta, X = max |input| of the tile
     double bdelta;
     double xbound;       // fixed X for integer images (< 0: measure the tile)
+    double oscale;       // results are multiplied by this (1 for sepFilter2D; 1 / (kw kh) for a mean box)
     int SH;
     FixList fix;
 };
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
                         if (yr + m >= rows_out) break;
                         const int y = y0 + yr + m;
                         char* drow = static_cast<char*>(a.dst.data) + static_cast<size_t>(y) * a.dst.stride;
-                        const double r = static_cast<double>(acc[m]);
+                        const double r = static_cast<double>(acc[m]) * a.oscale;
                         bool doubt = false;
                         switch (a.dst.dt) {
                             case IPFG_U8:
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
                     const int xl = bcast_i(x, l), cl = bcast_i(c, l);
                     const int yl = y0 + bcast_i(yr, l) + m;
                     const double e = sep_exact_wave<Ti>(a.src, xl, yl, cl, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border,
-                                                        a.delta, a.bval);
+                                                        a.delta, a.bval) * a.oscale;
                     if (lane_id() == l)
                         store_sample(a.dst.data, static_cast<size_t>(yl) * a.dst.stride + (static_cast<size_t>(xl) * cn + cl) * es,
                                      a.dst.dt, e);
@@ -607,7 +608,7 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
                     if (yr + m >= rows_out) break;
-                    const double rv = static_cast<double>(acc[m]);
+                    const double rv = static_cast<double>(acc[m]) * a.oscale;
                     put<To>(op, rv);
                     if (CERT && doubtful<To>(rv, bound)) dmask |= 1u << m;
                     op += a.dst.stride;
@@ -618,7 +619,7 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
                     const int xl = bcast_i(xg, l);
                     const int yl = y0 + yr + m;
                     const double e = sep_exact_wave<Ti>(a.src, xl, yl, c, a.kxd, kw, a.kyd, kh, a.ax, a.ay, a.border,
-                                                        a.delta, a.bval);
+                                                        a.delta, a.bval) * a.oscale;
                     if (lane == l)
                         put<To>(static_cast<char*>(a.dst.data) + static_cast<size_t>(yl) * a.dst.stride +
                                     (static_cast<size_t>(xl) * CN + c) * sizeof(To),

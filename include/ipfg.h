@@ -212,8 +212,14 @@ int ipfg_batch_destroy(ipfg_batch batch);
 typedef void* ipfg_server;
 int ipfg_server_create(ipfg_server* server, size_t device_budget_bytes, int max_batch, int use_graph);
 /* Queues a request (the buffers must stay valid until it completes); *id
- * receives its number (0, 1, ...). Requests larger than the budget fail. */
+ * receives its number (0, 1, ...). Host images are uploaded / downloaded by
+ * the server; a dst with on_device = 1 and data = NULL is a server-owned
+ * device output that is dropped when the request completes (for measuring
+ * the device-side rate). Host buffers and server-owned outputs count against
+ * the budget; requests larger than the budget fail. */
 int ipfg_server_submit(ipfg_server server, const ipfg_job* request, long long* id);
+/* Queues n requests in order (ids first_id, first_id + 1, ...). */
+int ipfg_server_submit_many(ipfg_server server, const ipfg_job* requests, int n, long long* first_id);
 /* Completed requests since the last poll: ids and latency (submit to result
  * in place, milliseconds); *count receives how many were written. */
 int ipfg_server_poll(ipfg_server server, long long* ids, double* latency_ms, int capacity, int* count);
