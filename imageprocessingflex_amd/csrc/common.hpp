@@ -166,15 +166,27 @@ __device__ __forceinline__ double bcast_d(double v, int l) {
     return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
+// Maximum of v over the 64 lanes of the wave, returned wave-uniform. DPP row
+// shifts 1, 2, 4, 8 leave each row of 16 lanes with its maximum in its last
+// lane; the row broadcasts 15 and 31 carry those into the rows above, so lane
+// 63 ends with the wave's maximum. Lanes a shift reads from outside the row
+// contribute 0, the identity for unsigned values.
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+    v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false)));
+    v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false)));
+    return static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
 // Maximum of a non-negative float (or +inf) over the wave, added to *m (an
 // LDS word holding float bits) by one lane. Every lane of the wave calls it.
+// Non-negative floats order like their bit patterns; a NaN's bits exceed
+// +inf's, so a NaN makes the maximum a NaN.
 __device__ __forceinline__ void wave_max_into(unsigned int* m, float v) {
-    unsigned int b = __float_as_uint(v);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned int o = static_cast<unsigned int>(__shfl_xor(static_cast<int>(b), off));
-        b = o > b ? o : b;
-    }
+    const unsigned int b = wave_max_u32(__float_as_uint(v));
     if (lane_id() == 0) atomicMax(m, b);
 }
 

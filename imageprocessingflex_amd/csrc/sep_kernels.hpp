@@ -143,8 +143,6 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
     const int nhstep = (nh + CH - 1) / CH;
     const int nstep = T.K0 + (rows_out + CH - 1) / CH;
     const float bvalf = static_cast<float>(a.bval);
-    const Ti bv = static_cast<Ti>(bvalf);
-    const bool patch_bval = a.border == IPFG_BORDER_CONSTANT && static_cast<float>(bv) != bvalf;
     const bool measure = CERT && a.xbound < 0;
     if (tid == 0) tile_max_bits = 0u;
     float my_max = 0.f;
@@ -157,12 +155,15 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
     const int ncol = TX * cn;
     const int gx0 = x0 - a.ax, gy0 = y0 - a.ay;
 
-    st.load(src, a.src.stride, W, H, cn, a.border, bv, gx0, gy0, min(CH, nh), T.XS, tid);
+    st.load(src, a.src.stride, W, H, cn, a.border, gx0, gy0, min(CH, nh), T.XS, tid);
     for (int k = 0; k < nstep; ++k) {
         const int r0 = k * CH;
         const int nr = min(CH, nh - r0);
+        // Constant border: the stager left image samples at the outside positions
+        const bool cpatch = nr > 0 && a.border == IPFG_BORDER_CONSTANT &&
+                            chunk_outside(gx0, gy0 + r0, TX + kw - 1, nr, W, H);
         if (nr > 0) st.store(in_buf, T.XP, nr, T.XS, tid, conv);
-        if (nr > 0 && patch_bval) {  // a Constant value that Ti cannot hold
+        if (cpatch) {
             for (int e = tid; e < nr * T.XS; e += kBlock) {
                 const int r = e / T.XS, s = e - r * T.XS;
                 const int gy = gy0 + r0 + r, gx = gx0 + s / cn;
@@ -170,12 +171,12 @@ __global__ __launch_bounds__(kBlock) void sep_fast_kernel(SepArgs a, const Acc* 
             }
         }
         if (measure) {
-            if (patch_bval) my_max = fmaxf(my_max, fabsf(bvalf));
+            if (cpatch) my_max = fmaxf(my_max, fabsf(bvalf));
             wave_max_into(&tile_max_bits, my_max);
         }
         __syncthreads();
         if (k + 1 < nhstep)
-            st.load(src, a.src.stride, W, H, cn, a.border, bv, gx0, gy0 + r0 + CH, min(CH, nh - r0 - CH), T.XS, tid);
+            st.load(src, a.src.stride, W, H, cn, a.border, gx0, gy0 + r0 + CH, min(CH, nh - r0 - CH), T.XS, tid);
         // ---- horizontal taps (reference order: i ascending from 0) into the ring
         if (nr > 0) {
             for (int t = tid; t < CH * G * cn; t += kBlock) {
@@ -436,7 +437,11 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
     const PlaneTile T = plane_tile(CN, kw, kh);
     Hr* ring = reinterpret_cast<Hr*>(sep_smem);
     float* in_buf = reinterpret_cast<float*>(sep_smem + ((static_cast<size_t>(CN) * T.RPL * sizeof(Hr) + 15) / 16) * 16);
-    __shared__ unsigned int tile_max_bits;
+    // Per-wave running maximum of |staged value| (float bits): each wave
+    // rewrites its own word every chunk, readers take the maximum of the four.
+    // The words only grow, so a reader that sees a later chunk's value still
+    // gets an upper bound of every sample it uses.
+    __shared__ __align__(16) unsigned int wave_max_bits[4];
     const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
     // the wave index is uniform: keep it (and what derives from it) in scalar registers
     const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -446,11 +451,9 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
     const int nhstep = (nh + CH - 1) / CH;
     const int nstep = T.K0 + (rows_out + CH - 1) / CH;
     const float bvalf = static_cast<float>(a.bval);
-    const Ti bv = static_cast<Ti>(bvalf);
-    const bool patch_bval = a.border == IPFG_BORDER_CONSTANT && static_cast<float>(bv) != bvalf;
     const bool measure = CERT && a.xbound < 0;
-    if (tid == 0) tile_max_bits = 0u;
-    float my_max = 0.f;
+    if (lane == 0) wave_max_bits[wave] = 0u;
+    unsigned int my_max = 0u;  // float bits of max |staged value| of this lane
     const int gx0 = x0 - a.ax, gy0 = y0 - a.ay;
     const int plane_in = CH * T.XPP;
     const int msk = T.R - 1;
@@ -459,46 +462,126 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
     // vertical task: column vx, rows [vv * M, vv * M + M) of the chunk (vv is wave-uniform)
     const int vx = lane, vv = wave;
 
+    // One-channel float images are staged with 16-byte loads: slot j of this
+    // thread covers chunk row vrow[j], samples [4 vseg[j], 4 vseg[j] + 4) of
+    // the LR = ceil(XS / 4) segments of a row (SJ slots cover 16 rows). Other
+    // images use the sample-per-lane Stager and de-interleave into planes.
+    constexpr bool VEC = CN == 1 && std::is_same<Ti, float>::value;
+    constexpr int NV = VEC ? SJ : 1;
+    const int LR = (T.XS + 3) / 4;
+    int vrow[NV], vseg[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const int q = tid + kBlock * j;
+        vrow[j] = q / LR;
+        vseg[j] = q - vrow[j] * LR;
+    }
+    struct VStage {
+        float4 v[NV];
+    };
+    using Stage = typename std::conditional<VEC, VStage, Stager<Ti, SJ>>::type;
+    // Strips that reach past the left or right edge map their columns once,
+    // into LDS (kept out of registers: the map is read only by border chunks).
+    __shared__ __align__(16) int colmap[VEC ? 4 * 128 : 1];
+    const bool hin = gx0 >= 0 && gx0 + 4 * LR <= W;
+    if constexpr (VEC) {
+        if (!hin) {
+            for (int px = tid; px < 4 * LR; px += kBlock)
+                colmap[px] = max(border_index(gx0 + min(px, T.XS - 1), W, a.border), 0);  // Constant: overwritten later
+            __syncthreads();
+        }
+    }
+
     // Two stagers alternate so that the loads of the next two chunks are in
     // flight while a chunk is filtered.
-    Stager<Ti, SJ> sa, sb;
-    sa.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0, min(CH, nh), T.XS, tid);
-    if (nhstep > 1) sb.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0 + CH, min(CH, nh - CH), T.XS, tid);
-    auto step = [&](Stager<Ti, SJ>& st, const int k) {
-        const int r0 = k * CH;
-        const int nr = min(CH, nh - r0);
-        if (nr > 0) {
-            // de-interleave: sample s of staged row r -> plane s % CN, pixel s / CN
+    Stage sa, sb;
+    auto issue = [&](Stage& st, int k) {
+        const int gyc = gy0 + k * CH, nr = min(CH, nh - k * CH);
+        if constexpr (VEC) {
+            const char* csrc = reinterpret_cast<const char*>(src);
+            if (hin && gyc >= 0 && gyc + nr <= H) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = wave + 4 * i;
-                if (r >= nr) continue;
+                for (int j = 0; j < NV; ++j) {  // slots past the chunk re-read its last row (never stored)
+                    const int r = min(vrow[j], nr - 1);
+                    const float* p = reinterpret_cast<const float*>(csrc + static_cast<size_t>(gyc + r) * a.src.stride) +
+                                     gx0 + 4 * vseg[j];
+                    __builtin_memcpy(&st.v[j], p, sizeof(float4));
+                }
+            } else {  // border chunk (Constant: any sample, overwritten at the store)
 #pragma unroll
-                for (int j = 0; j < SJ; ++j) {
-                    const int s = lane + 64 * j;
-                    if (s < T.XS) {
-                        const int px = s / CN, c = s - px * CN;
-                        const float f = static_cast<float>(st.val[i][j]);
-                        if (measure) my_max = fmaxf(my_max, fabsf(f));  // NaN/inf inputs give NaN/inf sums: flagged anyway
-                        in_buf[c * plane_in + r * T.XPP + px] = f;
+                for (int j = 0; j < NV; ++j) {
+                    const int gy = gyc + min(vrow[j], nr - 1);
+                    const int sy = (gy >= 0 && gy < H) ? gy : max(border_index(gy, H, a.border), 0);
+                    const float* rp = reinterpret_cast<const float*>(csrc + static_cast<size_t>(sy) * a.src.stride);
+                    if (hin) {
+                        __builtin_memcpy(&st.v[j], rp + gx0 + 4 * vseg[j], sizeof(float4));
+                    } else {
+                        const int4 sx = *reinterpret_cast<const int4*>(colmap + 4 * vseg[j]);
+                        st.v[j] = make_float4(rp[sx.x], rp[sx.y], rp[sx.z], rp[sx.w]);
                     }
                 }
             }
-            if (patch_bval) {  // a Constant value that Ti cannot hold
-                for (int e = tid; e < nr * T.XS; e += kBlock) {
-                    const int r = e / T.XS, s = e - r * T.XS;
-                    const int px = s / CN, c = s - px * CN;
-                    const int gy = gy0 + r0 + r, gx = gx0 + px;
-                    if (gy < 0 || gy >= H || gx < 0 || gx >= W) in_buf[c * plane_in + r * T.XPP + px] = bvalf;
+        } else {
+            st.load(src, a.src.stride, W, H, CN, a.border, gx0, gyc, nr, T.XS, tid);
+        }
+    };
+    issue(sa, 0);
+    if (nhstep > 1) issue(sb, 1);
+    auto step = [&](Stage& st, const int k) {
+        const int r0 = k * CH;
+        const int nr = min(CH, nh - r0);
+        if (nr > 0) {
+            // Constant border: the stager left image samples at the outside positions
+            const bool cpatch = a.border == IPFG_BORDER_CONSTANT && chunk_outside(gx0, gy0 + r0, T.XS / CN, nr, W, H);
+            if constexpr (VEC) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    if (vrow[j] >= nr) continue;
+                    float e[4] = {st.v[j].x, st.v[j].y, st.v[j].z, st.v[j].w};
+                    if (cpatch) {
+                        const int gy = gy0 + r0 + vrow[j];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int gx = gx0 + 4 * vseg[j] + i;
+                            if (gy < 0 || gy >= H || gx < 0 || gx >= W) e[i] = bvalf;
+                        }
+                    }
+                    if (measure) {  // |x| as bits; a NaN's bits exceed +inf's, so it poisons the bound
+                        const unsigned int m01 = max(__float_as_uint(e[0]) & 0x7fffffffu, __float_as_uint(e[1]) & 0x7fffffffu);
+                        const unsigned int m23 = max(__float_as_uint(e[2]) & 0x7fffffffu, __float_as_uint(e[3]) & 0x7fffffffu);
+                        my_max = max(my_max, max(m01, m23));
+                    }
+                    *reinterpret_cast<float4*>(in_buf + vrow[j] * T.XPP + 4 * vseg[j]) = make_float4(e[0], e[1], e[2], e[3]);
                 }
-                if (measure) my_max = fmaxf(my_max, fabsf(bvalf));
+            } else {
+                // de-interleave: sample s of staged row r -> plane s % CN, pixel s / CN
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int r = wave + 4 * i;
+                    if (r >= nr) continue;
+#pragma unroll
+                    for (int j = 0; j < SJ; ++j) {
+                        const int s = lane + 64 * j;
+                        if (s < T.XS) {
+                            const int px = s / CN, c = s - px * CN;
+                            float f = static_cast<float>(st.val[i][j]);
+                            if (cpatch) {
+                                const int gy = gy0 + r0 + r, gx = gx0 + px;
+                                if (gy < 0 || gy >= H || gx < 0 || gx >= W) f = bvalf;
+                            }
+                            if (measure) my_max = max(my_max, __float_as_uint(f) & 0x7fffffffu);
+                            in_buf[c * plane_in + r * T.XPP + px] = f;
+                        }
+                    }
+                }
             }
-            if (measure) wave_max_into(&tile_max_bits, my_max);
+            if (measure) {
+                const unsigned int wm = wave_max_u32(my_max);
+                if (lane == 0) wave_max_bits[wave] = wm;
+            }
         }
         __syncthreads();
-        if (k + 2 < nhstep)  // two chunks in flight: this one's registers take chunk k + 2
-            st.load(src, a.src.stride, W, H, CN, a.border, bv, gx0, gy0 + r0 + 2 * CH, min(CH, nh - r0 - 2 * CH), T.XS,
-                    tid);
+        if (k + 2 < nhstep) issue(st, k + 2);  // two chunks in flight: this one's registers take chunk k + 2
         // ---- horizontal taps (i ascending from 0) into the ring
         if (nr > 0 && hr < nr) {
 #pragma unroll 1
@@ -556,7 +639,15 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
         const int yr = ob + vv * M;  // first output row of this task (strip-relative), wave-uniform
         if (yr >= rows_out) return;
         double bound = 0.0;
-        if (CERT) bound = a.bcoef * (measure ? static_cast<double>(__uint_as_float(tile_max_bits)) : a.xbound) + a.bdelta;
+        if (CERT) {
+            double xb = a.xbound;
+            if (measure) {
+                const uint4 m4 = *reinterpret_cast<const uint4*>(wave_max_bits);
+                const unsigned int mb = __builtin_amdgcn_readfirstlane(max(max(m4.x, m4.y), max(m4.z, m4.w)));
+                xb = static_cast<double>(__uint_as_float(mb));
+            }
+            bound = a.bcoef * xb + a.bdelta;
+        }
         const int xg = x0 + vx;
 #pragma unroll 1
         for (int c = 0; c < CN; ++c) {
