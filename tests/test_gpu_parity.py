@@ -198,6 +198,26 @@ def test_certification_heavy_fixups_filter2d(ipf, oracle, shape):
     assert ipf.last_fixups() > img.size // 8
 
 
+def test_non_finite_and_extreme_float_inputs(ipf, oracle):
+    # NaN, infinities, values near FLT_MAX and subnormals: certified launches
+    # must flag every sample they reach and reproduce the reference's result
+    rng = np.random.default_rng(31)
+    img = rng.random((150, 333)).astype(np.float32)
+    img[10, 20] = np.nan
+    img[70, 300] = np.inf
+    img[140, 5] = -np.inf
+    img[80:90, 100:110] = 3.0e38
+    img[100, 150:160] = 1e-41
+    kx, ky = rng.normal(size=7), rng.normal(size=5)
+    for b in range(5):
+        for s in (1.0, 3.0):
+            assert np.array_equal(ipf.gaussian_blur(img, s, border=b), oracle.gaussian_blur(img, s, border=b),
+                                  equal_nan=True), (s, b)
+        for to in (np.float32, np.uint8, np.int16):
+            got = ipf.sep_filter2d(img, kx, ky, b, 0.25, 7.0, to)
+            assert np.array_equal(got, oracle.sep_filter(img, kx, ky, None, b, 0.25, 7.0, to), equal_nan=True), (b, to)
+
+
 def test_certified_calls_reuse_stream_state(ipf, oracle):
     # The per-stream counters are reset by each launch: alternate ops that
     # flag samples with ones that do not, on one stream.
