@@ -42,6 +42,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 / fp16
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector and FP64 matrix (dense)
 BORDERS = ["constant", "replicate", "reflect", "reflect101", "wrap"]
 METRIC = "Gpix/s + % HBM roofline per filter; p50/p99 latency, batched vs plain"
 
@@ -93,12 +94,16 @@ def _c2():
             if sigma is None:
                 launch = (lambda b: lambda lib, s, d, st: lib.ipfg_box_filter(s, d, 15, 15, 1, b, 0.0, 0, st))(b)
                 cpu = (lambda b: lambda o, im: o.box_filter(im, 15, 15, True, b))(b)
-                kern = "box_fast_kernel<float, float>"
+                kern = "sep_plane_kernel<float, float, double, double, 1, 2, true, 15> (unit weights)"
             else:
                 launch = (lambda b, sg: lambda lib, s, d, st: lib.ipfg_gaussian_blur(s, d, sg, 0.0, 0, b, 0, st))(b, sigma)
                 cpu = (lambda b, sg: lambda o, im: o.gaussian_blur(im, sg, border=b))(b, sigma)
-                kern = "sep_plane_kernel<float, float, double, 1, 2, true>"
-            calls.append(Call(f"{name}/{BORDERS[b]}", name, "img", np.float32, launch, cpu, kern))
+                ks = 2 * max(1, int(3 * sigma + 0.5)) + 1
+                kern = f"sep_plane_kernel<float, float, double, double, 1, 2, true, {ks}>"
+            # one FMA per tap and output sample in each pass (halo rows not counted)
+            taps = 30 if sigma is None else 2 * (2 * max(1, int(3 * sigma + 0.5)) + 1)
+            calls.append(Call(f"{name}/{BORDERS[b]}", name, "img", np.float32, launch, cpu, kern,
+                              flops=2.0 * taps * img.shape[0] * img.shape[1]))
     return {"images": {"img": img}, "calls": calls, "dtype": "f64",
             "workload": "gaussianBlur sigma 1/3/8 + blur 15x15, x 5 border modes, 7680x4320x1 float32"}
 
@@ -173,7 +178,7 @@ def _c4():
                 out[f"filter2d_{key}{names[prec]}"] = e
         return out
 
-    return {"images": {"u8": u8, "f32": f32}, "calls": calls, "dtype": "f64 / bf16x3 / fp16x2",
+    return {"images": {"u8": u8, "f32": f32}, "calls": calls, "dtype": "f64 / bf16x3 / fp16x2", "roof": "mfma",
             "workload": "filter2D dense 31x31 float32 kernel (N(0,1), sum 1), Reflect101, 4096x4096x1 uint8 and "
                         "float32; default (FP64, certified), FP16X2 and BF16X3 MFMA",
             "keep": kp, "post": accuracy, "kernel_raw_sum": ksum}
@@ -507,6 +512,18 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
             "kernel": per_filter[dom]["kernel"], "filter": dom,
             "algorithmic_bytes_per_launch": per_filter[dom]["bytes"], "mean_launch_ms": summary[dom]["mean_ms"]}
+    fl = per_filter[dom]["flops"]
+    if fl:
+        # the FP64 arithmetic of the certified paths against MI355X's FP64
+        # peak (vector and matrix peaks coincide): 1 FMA = 2 FLOP
+        tf = summary[dom]["tflop_s"]
+        peak = MFMA_PEAK_TFLOPS if ("bf16" in dom or "fp16" in dom) else FP64_PEAK_TFLOPS
+        compute = {"achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 4),
+                   "algorithmic_flops_per_launch": fl}
+        if wd.get("roof") == "mfma":
+            roof.update(compute, bound="mfma", hbm_frac=round(achieved / HBM_PEAK_GBS, 4))
+        else:
+            roof["fp64_compute"] = compute
     # a plain device-to-device copy of the largest input, for context and as
     # the calibration kernel of the HBM byte counters
     big = max(sets[0].values(), key=lambda d: d.nbytes)
