@@ -4,14 +4,19 @@
 // unequal to zero take part, in row-major order; the sum starts at delta and is
 // rounded once with saturate_cast.
 //
-// Fast path (FP32): a workgroup tile of TX pixels x TH rows with the input
-// tile (plus kernel halo) staged in LDS, one plane per channel. Each thread
-// produces P = 4 horizontally adjacent outputs, walking the kernel rows and
-// sliding a register window along each row, so every LDS value feeds P FMAs.
-// Weights are read as wave-uniform scalars. Zero weights are skipped when the
-// kernel has any (a uniform branch), matching the reference's tap list for
-// non-finite inputs. Integer outputs are certified with a per-sample bound
-// from the accumulated |w| * |v|; doubtful samples are recomputed in FP64.
+// Fast path (FP64 certified, or FP32): a workgroup tile of TX pixels x TH
+// rows with the input tile (plus kernel halo) staged in LDS, one plane per
+// channel. Each thread produces P = 8 (kernels 8 or more wide) or 4
+// horizontally adjacent outputs, walking the kernel rows and sliding a P-value
+// register window along each row: taps go in blocks of 4, each block reads
+// the next 4 samples with one 16-byte LDS load, so every LDS value feeds P
+// FMAs and one conversion. The window is a ring whose origin moves by 4 per
+// block (for P = 8 two phases, O = 0 and 4). Weights
+// are read as wave-uniform scalars. Zero weights are skipped when the kernel
+// has any (a uniform branch), matching the reference's tap list for
+// non-finite inputs. Certified launches bound each output's distance to the
+// reference's value; doubtful samples are recomputed in FP64 in the
+// reference's order by the same launch.
 //
 // Exact path: the reference's tap list in FP64, one thread per sample.
 #include "kernels.hpp"
@@ -19,10 +24,12 @@
 namespace ipfg {
 namespace {
 
-constexpr int P = 4;
 constexpr int TX = 64;
 
-__host__ __device__ inline int f2d_xp(int kw) { return (TX + kw - 1 + P) | 1; }
+// LDS row pitch (floats): a multiple of 4 for 16-byte loads, 4 more than a
+// multiple of 8 to spread rows over the banks, and room for the last block's
+// load (up to TX + kw + 10 samples from the row start).
+__host__ __device__ inline int f2d_xp(int kw) { return (TX + kw + 11 + 7) / 8 * 8 + 4; }
 
 __host__ inline size_t f2d_lds_bytes(int cn, int TH, int kw, int kh) {
     return sizeof(float) * static_cast<size_t>(cn) * (TH + kh - 1) * f2d_xp(kw);
@@ -74,8 +81,61 @@ __device__ __forceinline__ Acc f2d_fma(Acc a, Acc b, Acc c) {
     else return fmaf(a, b, c);
 }
 
-template <class Ti, class Acc, bool HZ, bool CERT>
-__global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
+// Taps i0 .. i0 + nt - 1 (nt <= 4) of one kernel row for P outputs. On entry
+// win[(O + q) % P] holds sample i0 + q of the window (q < P); each tap then
+// retires its oldest sample for sample i0 + P + ii, which the following taps
+// address by the same formula.
+template <int P, int O, bool HZ, class Acc>
+__device__ __forceinline__ void f2d_block4(const float* __restrict__ base, int i0, int nt, const Acc* __restrict__ krow,
+                                           Acc (&acc)[P], Acc (&win)[P]) {
+    const float4 f = *reinterpret_cast<const float4*>(base + i0 + P);
+    const float nx[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii) {
+        if (ii < nt) {
+            const Acc k = krow[i0 + ii];
+            if (!HZ || k != Acc(0)) {
+#pragma unroll
+                for (int m = 0; m < P; ++m) acc[m] = f2d_fma(k, win[(O + ii + m) % P], acc[m]);
+            }
+            win[(O + ii) % P] = static_cast<Acc>(nx[ii]);
+        }
+    }
+}
+
+// One kernel row: P = 8 alternates the two window phases, P = 4 has one.
+template <int P, bool HZ, class Acc>
+__device__ __forceinline__ void f2d_row(const float* __restrict__ base, int kw, const Acc* __restrict__ krow,
+                                        Acc (&acc)[P]) {
+    Acc win[P];
+#pragma unroll
+    for (int q = 0; q < P; q += 4) {
+        const float4 f = *reinterpret_cast<const float4*>(base + q);
+        win[q] = f.x;
+        win[q + 1] = f.y;
+        win[q + 2] = f.z;
+        win[q + 3] = f.w;
+    }
+    int i0 = 0;
+    if constexpr (P == 8) {
+        for (; i0 + 8 <= kw; i0 += 8) {
+            f2d_block4<P, 0, HZ>(base, i0, 4, krow, acc, win);
+            f2d_block4<P, 4, HZ>(base, i0 + 4, 4, krow, acc, win);
+        }
+        if (i0 + 4 <= kw) {
+            f2d_block4<P, 0, HZ>(base, i0, 4, krow, acc, win);
+            if (i0 + 4 < kw) f2d_block4<P, 4, HZ>(base, i0 + 4, kw - i0 - 4, krow, acc, win);
+        } else if (i0 < kw) {
+            f2d_block4<P, 0, HZ>(base, i0, kw - i0, krow, acc, win);
+        }
+    } else {
+        for (; i0 + 4 <= kw; i0 += 4) f2d_block4<P, 0, HZ>(base, i0, 4, krow, acc, win);
+        if (i0 < kw) f2d_block4<P, 0, HZ>(base, i0, kw - i0, krow, acc, win);
+    }
+}
+
+template <class Ti, class Acc, bool HZ, bool CERT, int P>
+__global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a, const Acc* __restrict__ K) {
     extern __shared__ float smem[];
     __shared__ unsigned int tile_max_bits;
     const int W = a.src.w, H = a.src.h, cn = a.src.cn;
@@ -91,7 +151,6 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
     const float bval = static_cast<float>(a.bval);
     const bool measure = CERT && a.xbound < 0;
     const Ti* __restrict__ src = static_cast<const Ti*>(a.src.data);
-    const Acc* __restrict__ K = static_cast<const Acc*>(a.K);
     if (tid == 0) tile_max_bits = 0u;
     float my_max = 0.f;
     __syncthreads();
@@ -129,45 +188,23 @@ __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a) {
     constexpr int G = TX / P;
     const int ntask = TH * G * cn;
     const int ntask64 = (ntask + 63) & ~63;  // whole waves stay together for the fix-ups
-    for (int t = tid; t < ntask64; t += kBlock) {
+    // wave-uniform loop over groups of 64 tasks (the trip count and the tap
+    // loops below stay scalar)
+    for (int t0 = __builtin_amdgcn_readfirstlane(wave) * 64; t0 < ntask64; t0 += kBlock) {
+        const int t = t0 + lane;
+        // every lane walks the same taps (so the weights stay wave-uniform
+        // scalars); lanes past the last task filter row 0 of plane 0 and
+        // store nothing
         const bool act = t < ntask;
         const int g = t % G;
-        const int rest = t / G;
+        const int rest = act ? t / G : 0;
         const int r = rest % TH;
         const int c = rest / TH;
         Acc acc[P];
 #pragma unroll
         for (int m = 0; m < P; ++m) acc[m] = static_cast<Acc>(a.delta);
-        for (int j = 0; j < (act ? kh : 0); ++j) {
-            const float* base = smem + c * plane + (r + j) * XP + g * P;
-            const Acc* krow = K + j * kw;
-            Acc win[P];
-#pragma unroll
-            for (int m = 0; m < P; ++m) win[m] = base[m];
-            int i0 = 0;
-            for (; i0 + P <= kw; i0 += P) {
-#pragma unroll
-                for (int ii = 0; ii < P; ++ii) {
-                    const Acc k = krow[i0 + ii];
-                    if (!HZ || k != Acc(0)) {
-#pragma unroll
-                        for (int m = 0; m < P; ++m) acc[m] = f2d_fma(k, win[(ii + m) % P], acc[m]);
-                    }
-                    win[ii] = base[i0 + ii + P];
-                }
-            }
-#pragma unroll
-            for (int ii = 0; ii < P; ++ii) {
-                if (i0 + ii < kw) {
-                    const Acc k = krow[i0 + ii];
-                    if (!HZ || k != Acc(0)) {
-#pragma unroll
-                        for (int m = 0; m < P; ++m) acc[m] = f2d_fma(k, win[(ii + m) % P], acc[m]);
-                    }
-                    win[ii] = base[i0 + ii + P];
-                }
-            }
-        }
+        for (int j = 0; j < kh; ++j)
+            f2d_row<P, HZ>(smem + c * plane + (r + j) * XP + g * P, kw, K + j * kw, acc);
         const int y = y0 + r;
         unsigned int dmask = 0u;
         if (act && y < H) {
@@ -260,12 +297,14 @@ int grid_for(size_t n) {
     return static_cast<int>(b < 16384 ? (b ? b : 1) : 16384);
 }
 
-template <class Ti, class Acc, bool HZ, bool CERT>
+template <class Ti, class Acc, bool HZ, bool CERT, int P>
 hipError_t f2d_fast_launch(const F2dLaunch& l, hipStream_t s) {
     const int cn = l.src.cn;
     // rows per tile: halo overhead (TH + kh - 1) / TH <= 1.25 up to 64 rows; LDS within 64 KB
     int TH = 16;
-    while (TH < 64 && static_cast<long long>(TH + l.kh - 1) * 4 > static_cast<long long>(TH) * 5) TH *= 2;
+    if (static_cast<long long>(l.kw) * l.kh >= 64) TH = 32;  // arithmetic-bound: workgroups per CU over halo rows
+    else
+        while (TH < 64 && static_cast<long long>(TH + l.kh - 1) * 4 > static_cast<long long>(TH) * 5) TH *= 2;
     while (TH > 1 && f2d_lds_bytes(cn, TH, l.kw, l.kh) > 64 * 1024) TH /= 2;
     const size_t lds = f2d_lds_bytes(cn, TH, l.kw, l.kh);
     if (lds > 64 * 1024) return hipErrorInvalidConfiguration;
@@ -289,15 +328,22 @@ hipError_t f2d_fast_launch(const F2dLaunch& l, hipStream_t s) {
     a.xbound = l.xbound;
     a.fix = l.fix;
     const dim3 grid(static_cast<unsigned>((l.src.w + TX - 1) / TX), static_cast<unsigned>((l.src.h + TH - 1) / TH));
-    hipLaunchKernelGGL((f2d_fast_kernel<Ti, Acc, HZ, CERT>), grid, dim3(kBlock), lds, s, a);
+    hipLaunchKernelGGL((f2d_fast_kernel<Ti, Acc, HZ, CERT, P>), grid, dim3(kBlock), lds, s, a, static_cast<const Acc*>(a.K));
     return hipGetLastError();
+}
+
+template <class Ti, int P>
+hipError_t f2d_fast_dispatch_p(const F2dLaunch& a, hipStream_t s) {
+    if (a.mode == MODE_CERTIFY)
+        return a.has_zero ? f2d_fast_launch<Ti, double, true, true, P>(a, s)
+                          : f2d_fast_launch<Ti, double, false, true, P>(a, s);
+    return a.has_zero ? f2d_fast_launch<Ti, float, true, false, P>(a, s)
+                      : f2d_fast_launch<Ti, float, false, false, P>(a, s);
 }
 
 template <class Ti>
 hipError_t f2d_fast_dispatch(const F2dLaunch& a, hipStream_t s) {
-    if (a.mode == MODE_CERTIFY)
-        return a.has_zero ? f2d_fast_launch<Ti, double, true, true>(a, s) : f2d_fast_launch<Ti, double, false, true>(a, s);
-    return a.has_zero ? f2d_fast_launch<Ti, float, true, false>(a, s) : f2d_fast_launch<Ti, float, false, false>(a, s);
+    return a.kw >= 8 ? f2d_fast_dispatch_p<Ti, 8>(a, s) : f2d_fast_dispatch_p<Ti, 4>(a, s);
 }
 
 }  // namespace

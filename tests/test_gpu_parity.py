@@ -395,8 +395,9 @@ def test_filter2d_mfma_u8_output_close_to_reference(ipf, oracle, precision):
 
 # ------------------------------------------------------------- batches and serving
 def _mixed_jobs(rng, n):
+    # up to 600 x 520 x 3: many strips and tiles per job
     shapes = [(17, 23, 1), (64, 64, 3), (130, 257, 4), (300, 200, 1), (5, 700, 3), (96, 33, 4), (1, 1, 1),
-              (255, 129, 3), (40, 41, 1)]
+              (255, 129, 3), (40, 41, 1), (600, 520, 3), (530, 520, 1)]
     out = []
     for i in range(n):
         h, w, cn = shapes[i % len(shapes)]
