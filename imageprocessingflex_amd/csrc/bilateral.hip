@@ -7,10 +7,11 @@
 //
 // Fast path (uint8, 1..4 channels): pixels are packed into one 32-bit word in
 // an LDS tile with an r-pixel halo, so the L1 distance over all channels is a
-// single v_sad_u8; the range table (FP32, <= 1021 entries) sits in LDS and
-// the space weights are wave-uniform scalars. One thread per output pixel.
-// Every channel's FP32 result is checked against its error bound; doubtful
-// pixels are recomputed in FP64 in the reference's order.
+// single v_sad_u8; the range table (<= 1021 entries) and the tap list (LDS
+// offset, dx, dy, space weight) sit in LDS, so every lane of a wave walks the
+// same taps with broadcast reads. One thread per output pixel. Certified
+// launches check every channel's FP64 result against its error bound;
+// doubtful pixels are recomputed in FP64 in the reference's order.
 //
 // Exact path: the reference loop in FP64 per pixel, for every pixel type.
 #include "batch.hpp"
@@ -23,8 +24,19 @@ constexpr int TX = 64;
 
 __host__ __device__ inline int bil_xp(int r) { return (TX + 2 * r) | 1; }
 
+// Tap list entry in LDS: tile offset dy * XP + dx, and dx, dy for the
+// Constant border's bounds test.
+struct BilTap {
+    int off, dx, dy, pad;
+};
+
+// Largest tap count of a radius-r disc is (2r + 1)^2.
+__host__ __device__ inline int bil_max_taps(int r) { return (2 * r + 1) * (2 * r + 1); }
+
 __host__ inline size_t bil_lds_bytes(int TH, int r, int lut_len, size_t acc_size) {
-    return sizeof(uint32_t) * static_cast<size_t>(TH + 2 * r) * bil_xp(r) + acc_size * lut_len + 16;
+    const size_t tile = sizeof(uint32_t) * ((static_cast<size_t>(TH + 2 * r) * bil_xp(r) + 3) / 4 * 4);
+    const size_t lut = (acc_size * lut_len + 15) / 16 * 16;
+    return tile + lut + (sizeof(BilTap) + acc_size) * bil_max_taps(r) + 16;
 }
 
 // One pixel, every channel, in the reference's FP64 order (filter.hpp:366-383:
@@ -104,12 +116,20 @@ __device__ __forceinline__ void bilateral_u8_tile(const BilArgs& a, int bx, int 
     const int HR = TH + 2 * r;
     uint32_t* tile = bsm;
     Acc* lut = reinterpret_cast<Acc*>(bsm + ((HR * XP + 3) / 4) * 4);
+    BilTap* ltap = reinterpret_cast<BilTap*>(reinterpret_cast<char*>(lut) + (sizeof(Acc) * a.lut_len + 15) / 16 * 16);
+    Acc* lsw = reinterpret_cast<Acc*>(ltap + a.ntaps);
     const Acc* __restrict__ lutg = static_cast<const Acc*>(a.lut);
     const Acc* __restrict__ space = static_cast<const Acc*>(a.space);
     const uint8_t* __restrict__ src = static_cast<const uint8_t*>(a.src.data);
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = bx * TX, y0 = by * TH;
     for (int i = tid; i < a.lut_len; i += kBlock) lut[i] = lutg[i];
+    for (int i = tid; i < a.ntaps; i += kBlock) {
+        const int dx = a.taps[2 * i], dy = a.taps[2 * i + 1];
+        ltap[i] = BilTap{dy * XP + dx, dx, dy, 0};
+        lsw[i] = space[i];
+    }
     const int XW = TX + 2 * r;
     for (int rr = wave; rr < HR; rr += kBlock / 64) {
         const int gy = y0 - r + rr;
@@ -129,7 +149,11 @@ __device__ __forceinline__ void bilateral_u8_tile(const BilArgs& a, int bx, int 
         }
     }
     __syncthreads();
-    for (int t = tid; t < TX * TH; t += kBlock) {
+    // wave-uniform loops: every lane walks the same taps (lanes past the
+    // image filter a position inside the tile and store nothing)
+    const int ntaps = a.ntaps;
+    for (int t0 = wave * 64; t0 < TX * TH; t0 += kBlock) {
+        const int t = t0 + lane;
         const int lx = t % TX, ly = t / TX;
         const int x = x0 + lx, y = y0 + ly;
         const bool act = x < W && y < H;
@@ -139,16 +163,16 @@ __device__ __forceinline__ void bilateral_u8_tile(const BilArgs& a, int bx, int 
         Acc acc[CN];
 #pragma unroll
         for (int c = 0; c < CN; ++c) acc[c] = Acc(0);
-        for (int k = 0; k < (act ? a.ntaps : 0); ++k) {
-            const int dx = a.taps[2 * k], dy = a.taps[2 * k + 1];
-            if (CONST_BORDER) {
-                if (static_cast<unsigned>(x + dx) >= static_cast<unsigned>(W) ||
-                    static_cast<unsigned>(y + dy) >= static_cast<unsigned>(H))
-                    continue;
-            }
-            const uint32_t p = cen[dy * XP + dx];
+        for (int k = 0; k < ntaps; ++k) {
+            const BilTap tp = ltap[k];
+            const uint32_t p = cen[tp.off];
             const uint32_t dist = __builtin_amdgcn_sad_u8(p, p0, 0u);
-            const Acc w = space[k] * lut[dist];
+            Acc w = lsw[k] * lut[dist];
+            if (CONST_BORDER) {  // a tap outside the image takes no part (weight 0 adds exactly nothing)
+                if (static_cast<unsigned>(x + tp.dx) >= static_cast<unsigned>(W) ||
+                    static_cast<unsigned>(y + tp.dy) >= static_cast<unsigned>(H))
+                    w = Acc(0);
+            }
             wsum += w;
 #pragma unroll
             for (int c = 0; c < CN; ++c) {
