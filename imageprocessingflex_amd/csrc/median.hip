@@ -116,39 +116,67 @@ __device__ __forceinline__ void median_tile(const T* __restrict__ src, size_t ss
     const int XP = med_xp(ksz);
     const int HR = TH + ksz - 1;
     const int plane = HR * XP;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int x0 = bx * TX, y0 = by * TH;
     const int XW = TX + ksz - 1;
     const int rowSamples = XW * cn;
     const bool inside_x = (x0 - r >= 0) && (x0 - r + XW <= W);
+    // Staging: each wave takes rows wave, wave + 4, ...; per row, every lane
+    // issues its loads (samples lane + 64 j, j < SJ, clamped to the row)
+    // before it writes any, so a row costs one memory round trip.
+    constexpr int SJ = 5;  // (64 + 4) samples x 4 channels fit in one pass
+    int px_l[SJ], c_l[SJ];
+#pragma unroll
+    for (int j = 0; j < SJ; ++j) {
+        px_l[j] = (lane + 64 * j) / cn;
+        c_l[j] = lane + 64 * j - px_l[j] * cn;
+    }
     for (int rr = wave; rr < HR; rr += kBlock / 64) {
         const int gy = y0 - r + rr;
         const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, border);
         const T* srow = reinterpret_cast<const T*>(reinterpret_cast<const char*>(src) + static_cast<size_t>(sy) * sstride);
         K* trow = tile + rr * XP;
-        for (int s = lane; s < rowSamples; s += 64) {
-            const int px = s / cn, c = s - px * cn;
-            T v;
-            if (inside_x) {
-                v = srow[static_cast<size_t>(x0 - r) * cn + s];
-            } else {
-                const int gx = x0 - r + px;
-                const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, border);
-                v = srow[static_cast<size_t>(sx) * cn + c];
+        for (int s0 = 0; s0 < rowSamples; s0 += 64 * SJ) {
+            T v[SJ];
+#pragma unroll
+            for (int j = 0; j < SJ; ++j) {
+                const int s = min(s0 + lane + 64 * j, rowSamples - 1);
+                if (inside_x) {
+                    v[j] = srow[static_cast<size_t>(x0 - r) * cn + s];
+                } else {
+                    const int px = s0 == 0 && s == lane + 64 * j ? px_l[j] : s / cn;
+                    const int c = s - px * cn;
+                    const int gx = x0 - r + px;
+                    const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, border);
+                    v[j] = srow[static_cast<size_t>(sx) * cn + c];
+                }
             }
-            trow[c * plane + px] = KT::to(v);
+#pragma unroll
+            for (int j = 0; j < SJ; ++j) {
+                const int s = s0 + lane + 64 * j;
+                if (s < rowSamples) {
+                    const int px = s0 == 0 ? px_l[j] : s / cn;
+                    const int c = s0 == 0 ? c_l[j] : s - px * cn;
+                    trow[c * plane + px] = KT::to(v[j]);
+                }
+            }
         }
     }
     __syncthreads();
 
     if constexpr (KS == 3 || KS == 5) {
         constexpr int G = TX / P;
+        static_assert(kBlock % G == 0, "a thread keeps its column group");
         const int ntask = TH * G * cn;
-        for (int t = tid; t < ntask; t += kBlock) {
-            const int g = t % G;
-            const int rest = t / G;
-            const int row = rest % TH;
-            const int c = rest / TH;
+        // task t = (c * TH + row) * G + g: g stays tid % G, (c, row) advance
+        // by kBlock / G rows per step (no division per task)
+        const int g = tid % G;
+        int row = (tid / G) % TH, c = (tid / G) / TH;
+        for (int t = tid; t < ntask; t += kBlock, row += kBlock / G) {
+            while (row >= TH) {
+                row -= TH;
+                ++c;
+            }
             const K* base = tile + c * plane + row * XP + g * P;
             K res[P];
             if constexpr (KS == 3) {
