@@ -122,7 +122,7 @@ __device__ __forceinline__ void bilateral_u8_tile(const BilArgs& a, int bx, int 
     const Acc* __restrict__ space = static_cast<const Acc*>(a.space);
     const uint8_t* __restrict__ src = static_cast<const uint8_t*>(a.src.data);
     const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar loop bounds
     const int x0 = bx * TX, y0 = by * TH;
     for (int i = tid; i < a.lut_len; i += kBlock) lut[i] = lutg[i];
     for (int i = tid; i < a.ntaps; i += kBlock) {
@@ -130,22 +130,39 @@ __device__ __forceinline__ void bilateral_u8_tile(const BilArgs& a, int bx, int 
         ltap[i] = BilTap{dy * XP + dx, dx, dy, 0};
         lsw[i] = space[i];
     }
+    // Staging: each wave takes rows wave, wave + 4, ...; per row every lane
+    // loads all its pixels (px = lane + 64 j, clamped to the row) before it
+    // packs and writes them, so a row costs one memory round trip.
     const int XW = TX + 2 * r;
+    constexpr int PJ = 2;  // pixels per lane per pass: rows up to 128 pixels take one pass
     for (int rr = wave; rr < HR; rr += kBlock / 64) {
         const int gy = y0 - r + rr;
         const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, a.border);
         uint32_t* trow = tile + rr * XP;
         const uint8_t* srow = src + static_cast<size_t>(sy < 0 ? 0 : sy) * a.src.stride;
-        for (int px = lane; px < XW; px += 64) {
-            const int gx = x0 - r + px;
-            const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, a.border);
-            uint32_t v = 0;
-            if (sy >= 0 && sx >= 0) {
-                const uint8_t* p = srow + static_cast<size_t>(sx) * CN;
+        for (int p0x = 0; p0x < XW; p0x += 64 * PJ) {
+            uint8_t b[PJ][CN];
+            bool out[PJ];
 #pragma unroll
-                for (int c = 0; c < CN; ++c) v |= static_cast<uint32_t>(p[c]) << (8 * c);
+            for (int j = 0; j < PJ; ++j) {
+                const int px = min(p0x + lane + 64 * j, XW - 1);
+                const int gx = x0 - r + px;
+                const int sx = (gx >= 0 && gx < W) ? gx : border_index(gx, W, a.border);
+                out[j] = sy < 0 || sx < 0;
+                const uint8_t* p = srow + static_cast<size_t>(sx < 0 ? 0 : sx) * CN;
+#pragma unroll
+                for (int c = 0; c < CN; ++c) b[j][c] = p[c];
             }
-            trow[px] = v;
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const int px = p0x + lane + 64 * j;
+                if (px < XW) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int c = 0; c < CN; ++c) v |= static_cast<uint32_t>(b[j][c]) << (8 * c);
+                    trow[px] = out[j] ? 0u : v;
+                }
+            }
         }
     }
     __syncthreads();
