@@ -20,6 +20,8 @@ __all__ = [
     "BorderMode", "Precision", "DeviceImage", "Stream", "IpfError", "DeviceError",
     "gaussian_kernel_1d", "sep_filter2d", "gaussian_blur", "box_filter", "blur", "filter2d",
     "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
+    "sobel", "scharr", "prewitt", "laplacian", "magnitude", "phase", "gradient_magnitude", "unsharp_mask",
+    "sharpen", "emboss", "adaptive_threshold", "motion_blur_kernel", "AdaptiveMethod", "ThresholdType",
 ]
 
 
@@ -280,6 +282,150 @@ def median_blur(src, ksize, border=BorderMode.REPLICATE, stream: Stream | None =
     s, d, holder = _io(src, None)
     _lib.check(_lib.load().ipfg_median(C.byref(s), C.byref(d), int(ksize), int(border), _sh(stream)))
     return _ret(holder)
+
+
+# ---------------------------------------------------------------- derivatives and sharpening
+class AdaptiveMethod(IntEnum):
+    """ipf::AdaptiveMethod (filter.hpp:540)."""
+    MEAN = 0
+    GAUSSIAN = 1
+
+
+class ThresholdType(IntEnum):
+    """The ipf::ThresholdType values adaptiveThreshold takes."""
+    BINARY = 0
+    BINARY_INV = 1
+
+
+def _float_io(src):
+    return _io(src, np.float32)
+
+
+def sobel(src, dx, dy, ksize=3, scale=1.0, border=BorderMode.REFLECT101, precision=Precision.DEFAULT,
+          stream: Stream | None = None):
+    """sobel (filter.hpp:392-406): float32 derivative of order (dx, dy)."""
+    s, d, holder = _float_io(src)
+    _lib.check(_lib.load().ipfg_sobel(C.byref(s), C.byref(d), int(dx), int(dy), int(ksize), float(scale), int(border),
+                                      int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def scharr(src, dx, dy, scale=1.0, border=BorderMode.REFLECT101, precision=Precision.DEFAULT,
+           stream: Stream | None = None):
+    """scharr (filter.hpp:408-413)."""
+    s, d, holder = _float_io(src)
+    _lib.check(_lib.load().ipfg_scharr(C.byref(s), C.byref(d), int(dx), int(dy), float(scale), int(border),
+                                       int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def prewitt(src, dx, dy, border=BorderMode.REFLECT101, precision=Precision.DEFAULT, stream: Stream | None = None):
+    """prewitt (filter.hpp:415-421)."""
+    s, d, holder = _float_io(src)
+    _lib.check(_lib.load().ipfg_prewitt(C.byref(s), C.byref(d), int(dx), int(dy), int(border), int(precision),
+                                        _sh(stream)))
+    return _ret(holder)
+
+
+def laplacian(src, ksize=1, scale=1.0, border=BorderMode.REFLECT101, precision=Precision.DEFAULT,
+              stream: Stream | None = None):
+    """laplacian (filter.hpp:423-436)."""
+    s, d, holder = _float_io(src)
+    _lib.check(_lib.load().ipfg_laplacian(C.byref(s), C.byref(d), int(ksize), float(scale), int(border),
+                                          int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def _polar(gx, gy, mode_args, fn, stream):
+    if isinstance(gx, DeviceImage):
+        dst = DeviceImage.like(gx, np.float32)
+        gyd = gy.descriptor() if isinstance(gy, DeviceImage) else _desc(np.ascontiguousarray(gy, np.float32))
+        keep = (gy,)
+        _lib.check(fn(C.byref(gx.descriptor()), C.byref(gyd), C.byref(dst.descriptor()), *mode_args, _sh(stream)))
+        del keep
+        return dst
+    gx = np.ascontiguousarray(gx, np.float32)
+    gy = np.ascontiguousarray(gy, np.float32)
+    if gx.shape != gy.shape:
+        raise IpfError("ipf: magnitude: shape mismatch" if not mode_args else "ipf: phase: shape mismatch")
+    out = np.empty(gx.shape, np.float32)
+    _lib.check(fn(C.byref(_desc(gx)), C.byref(_desc(gy)), C.byref(_desc(out)), *mode_args, _sh(stream)))
+    return out
+
+
+def magnitude(gx, gy, stream: Stream | None = None):
+    """magnitude (filter.hpp:438-444): hypot of two float32 gradient images."""
+    return _polar(gx, gy, (), _lib.load().ipfg_magnitude, stream)
+
+
+def phase(gx, gy, degrees=True, stream: Stream | None = None):
+    """phase (filter.hpp:446-458): atan2(gy, gx) in [0, 360) degrees or [0, 2 pi) radians."""
+    return _polar(gx, gy, (int(bool(degrees)),), _lib.load().ipfg_phase, stream)
+
+
+def gradient_magnitude(src, ksize=3, stream: Stream | None = None):
+    """gradientMagnitude (filter.hpp:460-464)."""
+    return magnitude(sobel(src, 1, 0, ksize, stream=stream), sobel(src, 0, 1, ksize, stream=stream), stream)
+
+
+def unsharp_mask(src, sigma=1.0, amount=1.0, threshold=0.0, precision=Precision.DEFAULT,
+                 stream: Stream | None = None):
+    """unsharpMask (filter.hpp:472-483)."""
+    s, d, holder = _io(src, None)
+    _lib.check(_lib.load().ipfg_unsharp_mask(C.byref(s), C.byref(d), float(sigma), float(amount), float(threshold),
+                                             int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def sharpen(src, strength=1.0, precision=Precision.DEFAULT, stream: Stream | None = None):
+    """sharpen (filter.hpp:485-490)."""
+    s, d, holder = _io(src, None)
+    _lib.check(_lib.load().ipfg_sharpen(C.byref(s), C.byref(d), float(strength), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def emboss(src, precision=Precision.DEFAULT, stream: Stream | None = None):
+    """emboss (filter.hpp:492-496)."""
+    s, d, holder = _io(src, None)
+    _lib.check(_lib.load().ipfg_emboss(C.byref(s), C.byref(d), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def adaptive_threshold(src, maxval, method, ttype, block_size, c, precision=Precision.DEFAULT,
+                       stream: Stream | None = None):
+    """adaptiveThreshold (filter.hpp:545-565): method AdaptiveMethod, ttype ThresholdType."""
+    s, d, holder = _io(src, None)
+    _lib.check(_lib.load().ipfg_adaptive_threshold(C.byref(s), C.byref(d), float(maxval), int(method), int(ttype),
+                                                   int(block_size), float(c), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def motion_blur_kernel(length: int, angle_degrees: float) -> np.ndarray:
+    """motionBlurKernel (filter.hpp:498-516): a normalised line of `length`
+    pixels through the centre of an odd square kernel (host-side weights, like
+    gaussian_kernel_1d)."""
+    import math
+    if length < 1:
+        raise IpfError("ipf: motionBlurKernel: length must be >= 1")
+    size = length if length % 2 else length + 1
+    k = np.zeros((size, size), np.float32)
+    a = angle_degrees * math.pi / 180.0
+    c = (size - 1) / 2.0
+    samples = 4 * length
+
+    def lround(v):  # C lround: halves away from zero (v - trunc(v) is exact)
+        r = math.trunc(v)
+        return r + (1 if v > 0 else -1) if abs(v - r) >= 0.5 else r
+
+    for i in range(samples):
+        t = (length - 1) * (i / (samples - 1) - 0.5)
+        x = lround(c + t * math.cos(a))
+        y = lround(c - t * math.sin(a))
+        k[min(max(y, 0), size - 1), min(max(x, 0), size - 1)] = 1.0
+    n = np.float32(0.0)
+    for v in k.ravel():
+        n = np.float32(n + v)
+    return (k / n).astype(np.float32)
 
 
 def bilateral_filter(src, d, sigma_color, sigma_space, border=BorderMode.REFLECT101, precision=Precision.DEFAULT,

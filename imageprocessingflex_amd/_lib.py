@@ -85,6 +85,16 @@ SIGNATURES = {
     "ipfg_filter2d": (_I, [_IMG, _IMG, _P(C.c_float), _I, _I, _I, _I, _I, _D, _D, _I, _V]),
     "ipfg_median": (_I, [_IMG, _IMG, _I, _I, _V]),
     "ipfg_bilateral": (_I, [_IMG, _IMG, _I, _D, _D, _I, _I, _V]),
+    "ipfg_sobel": (_I, [_IMG, _IMG, _I, _I, _I, _D, _I, _I, _V]),
+    "ipfg_scharr": (_I, [_IMG, _IMG, _I, _I, _D, _I, _I, _V]),
+    "ipfg_prewitt": (_I, [_IMG, _IMG, _I, _I, _I, _I, _V]),
+    "ipfg_laplacian": (_I, [_IMG, _IMG, _I, _D, _I, _I, _V]),
+    "ipfg_magnitude": (_I, [_IMG, _IMG, _IMG, _V]),
+    "ipfg_phase": (_I, [_IMG, _IMG, _IMG, _I, _V]),
+    "ipfg_unsharp_mask": (_I, [_IMG, _IMG, _D, _D, _D, _I, _V]),
+    "ipfg_sharpen": (_I, [_IMG, _IMG, _D, _I, _V]),
+    "ipfg_emboss": (_I, [_IMG, _IMG, _I, _V]),
+    "ipfg_adaptive_threshold": (_I, [_IMG, _IMG, _D, _I, _I, _I, _D, _I, _V]),
     "ipfg_batch_create": (_I, [_P(_V), _I, _I]),
     "ipfg_batch_run": (_I, [_V, _JOB, _I, _V]),
     "ipfg_batch_destroy": (_I, [_V]),

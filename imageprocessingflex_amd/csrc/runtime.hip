@@ -872,6 +872,284 @@ bool images_overlap(const ipfg_image& a, const ipfg_image& b) {
     return a0 < b1 && b0 < a1;
 }
 
+// ------------------------------------------------------------------ derivatives and sharpening
+// filter.hpp:388-567: compositions of the separable, 2-D, box and Gaussian
+// filters with elementwise steps, which run as small kernels on the device
+// (FP64 and float arithmetic in the reference's order; hypotf/atan2f from
+// the device math library).
+
+// derivKernel (filter.hpp:93-114): binomial smoothing (1, 1) applied
+// ksize - 1 - order times, then the difference (-1, 1) order times; ksize 1
+// gives the unsmoothed central differences. Integer weights, exact in FP64.
+std::vector<double> deriv_kernel(int order, int ksize) {
+    if (ksize == 1) {
+        if (order == 0) return {1.0};
+        if (order == 1) return {-1.0, 0.0, 1.0};
+        if (order == 2) return {1.0, -2.0, 1.0};
+        throw ArgError{"ipf: derivKernel: order > 2 requires ksize >= 3"};
+    }
+    require(ksize % 2 == 1 && ksize >= 3 && ksize <= 31, "derivKernel: ksize must be odd in [3, 31]");
+    require(order >= 0 && order < ksize, "derivKernel: invalid derivative order");
+    std::vector<double> k{1.0};
+    auto step = [&k](double a, double b) {
+        std::vector<double> r(k.size() + 1, 0.0);
+        for (size_t i = 0; i < k.size(); ++i) {
+            r[i] += k[i] * a;
+            r[i + 1] += k[i] * b;
+        }
+        k.swap(r);
+    };
+    for (int i = 0; i < ksize - 1 - order; ++i) step(1, 1);
+    for (int i = 0; i < order; ++i) step(-1, 1);
+    return k;
+}
+
+ipfg_image dev_view(const DevImage& d) {
+    return ipfg_image{d.data, d.w, d.h, d.cn, d.dt, d.stride, 1};
+}
+
+// A packed device image of the given shape and type from the call's scratch.
+DevImage scratch_image(CallScratch& sc, int w, int h, int cn, int dt) {
+    DevImage d{nullptr, w, h, cn, dt, static_cast<size_t>(w) * cn * dtype_size(dt)};
+    d.data = sc.alloc(d.stride * static_cast<size_t>(h > 0 ? h : 1));
+    return d;
+}
+
+// Elementwise kernels: one thread per sample of a W x H x cn image, rows at
+// their strides.
+__device__ inline bool sample_at(const DevImage& im, size_t i, size_t* off) {
+    const size_t row = static_cast<size_t>(im.w) * im.cn;
+    if (i >= row * static_cast<size_t>(im.h)) return false;
+    const size_t y = i / row;
+    *off = y * im.stride + (i - y * row) * dtype_size(im.dt);
+    return true;
+}
+
+// a += b (float images): laplacian's sum of the two second derivatives.
+__global__ __launch_bounds__(kBlock) void add_f32_kernel(DevImage a, DevImage b) {
+    const size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+    size_t oa, ob;
+    if (!sample_at(a, i, &oa) || !sample_at(b, i, &ob)) return;
+    float* pa = reinterpret_cast<float*>(static_cast<char*>(a.data) + oa);
+    *pa = *pa + *reinterpret_cast<const float*>(static_cast<const char*>(b.data) + ob);
+}
+
+// unsharpMask's combine (filter.hpp:476-482): v + amount * (v - blurred) where
+// |v - blurred| > threshold, else v; saturated to the image type.
+__global__ __launch_bounds__(kBlock) void unsharp_kernel(DevImage src, DevImage blurred, DevImage dst, double amount,
+                                                         double threshold) {
+    const size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+    size_t os, ob, od;
+    if (!sample_at(src, i, &os) || !sample_at(blurred, i, &ob) || !sample_at(dst, i, &od)) return;
+    const double v = load_sample(src.data, os, src.dt);
+    const double diff = v - *reinterpret_cast<const double*>(static_cast<const char*>(blurred.data) + ob);
+    store_sample(dst.data, od, dst.dt, fabs(diff) > threshold ? v + amount * diff : v);
+}
+
+// adaptiveThreshold's decision (filter.hpp:559-563).
+__global__ __launch_bounds__(kBlock) void adaptive_kernel(DevImage src, DevImage mean, DevImage dst, double maxval,
+                                                          double c, int binary) {
+    const size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+    size_t os, om, od;
+    if (!sample_at(src, i, &os) || !sample_at(mean, i, &om) || !sample_at(dst, i, &od)) return;
+    const bool above = load_sample(src.data, os, src.dt) - load_sample(mean.data, om, mean.dt) > -c;
+    store_sample(dst.data, od, dst.dt, (above == (binary != 0)) ? maxval : 0.0);
+}
+
+// magnitude / phase of two float images (filter.hpp:438-458).
+// magnitude: sqrt(x^2 + y^2) evaluated in FP64 (the squares and their sum of
+// two floats are exact or rounded once; the square root is correctly rounded)
+// and rounded to float; an infinite operand gives +inf, else NaN propagates.
+// This is the formulation of glibc's hypotf, so the host and the device agree
+// bit for bit. phase: atan2 in FP64 rounded to float stands in for the host's
+// atan2f (within an ulp of it); the degree scaling and the wrap to [0, full)
+// follow the reference in FP64.
+__global__ __launch_bounds__(kBlock) void polar_kernel(DevImage gx, DevImage gy, DevImage dst, int mode) {
+    const size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+    size_t ox, oy, od;
+    if (!sample_at(gx, i, &ox) || !sample_at(gy, i, &oy) || !sample_at(dst, i, &od)) return;
+    const float x = *reinterpret_cast<const float*>(static_cast<const char*>(gx.data) + ox);
+    const float y = *reinterpret_cast<const float*>(static_cast<const char*>(gy.data) + oy);
+    float* o = reinterpret_cast<float*>(static_cast<char*>(dst.data) + od);
+    if (mode == 0) {
+        const double xd = x, yd = y;
+        *o = (__builtin_isinf(x) || __builtin_isinf(y)) ? __builtin_inff()
+                                                        : static_cast<float>(__builtin_sqrt(xd * xd + yd * yd));
+    } else {
+        constexpr double kPi = 3.14159265358979323846;
+        const double full = mode == 1 ? 360.0 : 2 * kPi;
+        double a = static_cast<float>(atan2(static_cast<double>(y), static_cast<double>(x)));
+        if (mode == 1) a *= 180.0 / kPi;
+        if (a < 0) a += full;
+        *o = static_cast<float>(a);
+    }
+}
+
+unsigned elementwise_grid(const DevImage& d) {
+    const size_t n = static_cast<size_t>(d.w) * d.h * d.cn;
+    return static_cast<unsigned>((n + kBlock - 1) / kBlock);
+}
+
+void require_float_dst(const ipfg_image* src, const ipfg_image* dst, const char* what) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(dst->dtype == IPFG_F32, what);
+}
+
+// sobel (filter.hpp:392-406)
+void sobel_impl(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, int ksize, double scale, int border,
+                int precision, ipfg_stream stream) {
+    require(dx >= 0 && dy >= 0 && dx + dy > 0, "sobel: need dx + dy > 0");
+    std::vector<double> kx = deriv_kernel(dx, ksize), ky = deriv_kernel(dy, ksize);
+    if (ksize == 1) {  // both kernels odd-length so the anchors stay centred
+        if (kx.size() == 1) kx = {0.0, 1.0, 0.0};
+        if (ky.size() == 1) ky = {0.0, 1.0, 0.0};
+    }
+    for (double& v : kx) v *= scale;
+    require_float_dst(src, dst, "sobel: dst must be float32");
+    const int kw = static_cast<int>(kx.size()), kh = static_cast<int>(ky.size());
+    sep_impl(src, dst, kx.data(), kw, ky.data(), kh, kw / 2, kh / 2, border, 0.0, 0.0, precision, stream);
+}
+
+// scharr, prewitt (filter.hpp:408-421): a central difference along the
+// derivative axis, a smoothing triple across it.
+void diff3_impl(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, const std::vector<double>& d,
+                const std::vector<double>& s, int border, int precision, ipfg_stream stream, const char* what) {
+    require((dx == 1 && dy == 0) || (dx == 0 && dy == 1), what);
+    require_float_dst(src, dst, "derivative: dst must be float32");
+    const std::vector<double>& kx = dx ? d : s;
+    const std::vector<double>& ky = dx ? s : d;
+    sep_impl(src, dst, kx.data(), 3, ky.data(), 3, 1, 1, border, 0.0, 0.0, precision, stream);
+}
+
+// laplacian (filter.hpp:423-436)
+void laplacian_impl(const ipfg_image* src, const ipfg_image* dst, int ksize, double scale, int border, int precision,
+                    ipfg_stream stream) {
+    require_float_dst(src, dst, "laplacian: dst must be float32");
+    if (ksize == 1) {
+        const float base[9] = {0, 1, 0, 1, -4, 1, 0, 1, 0};
+        float k[9];
+        for (int i = 0; i < 9; ++i) k[i] = static_cast<float>(base[i] * scale);
+        f2d_impl(src, dst, k, 3, 3, 1, 1, border, 0.0, 0.0, precision, stream);
+        return;
+    }
+    (void)deriv_kernel(2, ksize);  // the argument errors of sobel(src, 2, 0, ksize), before any device work
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    const ipfg_image ds = dev_view(io.src()), dd = dev_view(io.dst());
+    const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F32);
+    const ipfg_image db = dev_view(b);
+    const ipfg_stream st = reinterpret_cast<ipfg_stream>(s);
+    sobel_impl(&ds, &dd, 2, 0, ksize, scale, border, precision, st);
+    sobel_impl(&ds, &db, 0, 2, ksize, scale, border, precision, st);
+    hipLaunchKernelGGL(add_f32_kernel, dim3(elementwise_grid(io.dst())), dim3(kBlock), 0, s, io.dst(), b);
+    HIPCALL(hipGetLastError());
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
+// magnitude (mode 0), phase in degrees (1) or radians (2) (filter.hpp:438-458)
+void polar_impl(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, int mode, ipfg_stream stream) {
+    check_image(gx, "null src image");
+    check_image(gy, "null src image");
+    check_image(dst, "null dst image");
+    require(gx->width == gy->width && gx->height == gy->height && gx->channels == gy->channels,
+            mode == 0 ? "magnitude: shape mismatch" : "phase: shape mismatch");
+    abi_check(gx->dtype == IPFG_F32 && gy->dtype == IPFG_F32 && dst->dtype == IPFG_F32, "gradients must be float32");
+    ensure_init();
+    if (static_cast<long long>(gx->width) * gx->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(gx, dst, sc);
+    DevImage y;
+    if (gy->on_device) {
+        y = to_dev(*gy);
+    } else {
+        y = scratch_image(sc, gy->width, gy->height, gy->channels, IPFG_F32);
+        const size_t row = packed_stride(*gy);
+        HIPCALL(hipMemcpy2DAsync(y.data, y.stride, gy->data, gy->row_stride_bytes ? gy->row_stride_bytes : row, row,
+                                 gy->height, hipMemcpyHostToDevice, s));
+    }
+    hipLaunchKernelGGL(polar_kernel, dim3(elementwise_grid(io.dst())), dim3(kBlock), 0, s, io.src(), y, io.dst(), mode);
+    HIPCALL(hipGetLastError());
+    io.finish();
+}
+
+// unsharpMask (filter.hpp:472-483): FP64 Gaussian blur (Reflect101), then the
+// elementwise combine.
+void unsharp_impl(const ipfg_image* src, const ipfg_image* dst, double sigma, double amount, double threshold,
+                  int precision, ipfg_stream stream) {
+    const std::vector<double> k = gaussian_kernel(sigma, 0);
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(src->dtype == dst->dtype, "unsharpMask: dst must have the pixel type of src");
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    const ipfg_image ds = dev_view(io.src());
+    const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F64);
+    const ipfg_image db = dev_view(b);
+    const int ks = static_cast<int>(k.size());
+    sep_impl(&ds, &db, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0, precision,
+             reinterpret_cast<ipfg_stream>(s));
+    hipLaunchKernelGGL(unsharp_kernel, dim3(elementwise_grid(io.dst())), dim3(kBlock), 0, s, io.src(), b, io.dst(),
+                       amount, threshold);
+    HIPCALL(hipGetLastError());
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
+// sharpen, emboss (filter.hpp:485-496): 3 x 3 filter2D to the input type.
+void filter3x3_impl(const ipfg_image* src, const ipfg_image* dst, const float (&k)[9], int precision,
+                    ipfg_stream stream) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(src->dtype == dst->dtype, "dst must have the pixel type of src");
+    f2d_impl(src, dst, k, 3, 3, 1, 1, IPFG_BORDER_REFLECT101, 0.0, 0.0, precision, stream);
+}
+
+// adaptiveThreshold (filter.hpp:545-565): local mean (box, or Gaussian of the
+// block size) with Replicate borders in the image type, then the decision.
+// Float box means take the reference's running sums (EXACT) so that the
+// comparison sees the reference's mean.
+void adaptive_impl(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type, int block,
+                   double c, int precision, ipfg_stream stream) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    require(src->channels == 1, "adaptiveThreshold: expected a single-channel image");
+    require(block >= 3 && block % 2 == 1, "adaptiveThreshold: blockSize must be odd and >= 3");
+    require(type == 0 || type == 1, "adaptiveThreshold: type must be Binary or BinaryInv");
+    abi_check(method == 0 || method == 1, "adaptiveThreshold: method must be Mean or Gaussian");
+    abi_check(src->dtype == dst->dtype, "adaptiveThreshold: dst must have the pixel type of src");
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    const ipfg_image ds = dev_view(io.src());
+    const DevImage m = scratch_image(sc, src->width, src->height, 1, src->dtype);
+    const ipfg_image dm = dev_view(m);
+    const ipfg_stream st = reinterpret_cast<ipfg_stream>(s);
+    if (method == 0) {
+        const int p = dtype_is_int(src->dtype) ? precision : IPFG_PREC_EXACT;
+        box_impl(&ds, &dm, block, block, 1, IPFG_BORDER_REPLICATE, 0.0, p, st);
+    } else {
+        const std::vector<double> k = gaussian_kernel(0.0, block);
+        sep_impl(&ds, &dm, k.data(), block, k.data(), block, block / 2, block / 2, IPFG_BORDER_REPLICATE, 0.0, 0.0,
+                 precision, st);
+    }
+    hipLaunchKernelGGL(adaptive_kernel, dim3(elementwise_grid(io.dst())), dim3(kBlock), 0, s, io.src(), m, io.dst(),
+                       maxval, c, type == 0 ? 1 : 0);
+    HIPCALL(hipGetLastError());
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
 }  // namespace
 
 size_t batch_lds_bytes(int cls) {
@@ -1574,6 +1852,65 @@ int ipfg_median(const ipfg_image* src, const ipfg_image* dst, int ksize, int bor
 int ipfg_bilateral(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
                    int border, int precision, ipfg_stream stream) {
     return guarded([&] { bilateral_impl(src, dst, d, sigma_color, sigma_space, border, precision, stream); });
+}
+
+int ipfg_sobel(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, int ksize, double scale, int border,
+               int precision, ipfg_stream stream) {
+    return guarded([&] { sobel_impl(src, dst, dx, dy, ksize, scale, border, precision, stream); });
+}
+
+int ipfg_scharr(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, double scale, int border, int precision,
+                ipfg_stream stream) {
+    return guarded([&] {
+        diff3_impl(src, dst, dx, dy, {-scale, 0.0, scale}, {3.0, 10.0, 3.0}, border, precision, stream,
+                   "scharr: (dx, dy) must be (1,0) or (0,1)");
+    });
+}
+
+int ipfg_prewitt(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, int border, int precision,
+                 ipfg_stream stream) {
+    return guarded([&] {
+        diff3_impl(src, dst, dx, dy, {-1.0, 0.0, 1.0}, {1.0, 1.0, 1.0}, border, precision, stream,
+                   "prewitt: (dx, dy) must be (1,0) or (0,1)");
+    });
+}
+
+int ipfg_laplacian(const ipfg_image* src, const ipfg_image* dst, int ksize, double scale, int border, int precision,
+                   ipfg_stream stream) {
+    return guarded([&] { laplacian_impl(src, dst, ksize, scale, border, precision, stream); });
+}
+
+int ipfg_magnitude(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, ipfg_stream stream) {
+    return guarded([&] { polar_impl(gx, gy, dst, 0, stream); });
+}
+
+int ipfg_phase(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, int degrees, ipfg_stream stream) {
+    return guarded([&] { polar_impl(gx, gy, dst, degrees ? 1 : 2, stream); });
+}
+
+int ipfg_unsharp_mask(const ipfg_image* src, const ipfg_image* dst, double sigma, double amount, double threshold,
+                      int precision, ipfg_stream stream) {
+    return guarded([&] { unsharp_impl(src, dst, sigma, amount, threshold, precision, stream); });
+}
+
+int ipfg_sharpen(const ipfg_image* src, const ipfg_image* dst, double strength, int precision, ipfg_stream stream) {
+    return guarded([&] {
+        const float s = static_cast<float>(strength);
+        const float k[9] = {0, -s, 0, -s, 1 + 4 * s, -s, 0, -s, 0};
+        filter3x3_impl(src, dst, k, precision, stream);
+    });
+}
+
+int ipfg_emboss(const ipfg_image* src, const ipfg_image* dst, int precision, ipfg_stream stream) {
+    return guarded([&] {
+        const float k[9] = {-2, -1, 0, -1, 1, 1, 0, 1, 2};
+        filter3x3_impl(src, dst, k, precision, stream);
+    });
+}
+
+int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type,
+                            int block_size, double c, int precision, ipfg_stream stream) {
+    return guarded([&] { adaptive_impl(src, dst, maxval, method, type, block_size, c, precision, stream); });
 }
 
 int ipfg_batch_create(ipfg_batch* batch, int max_jobs, int use_graph) {

@@ -5,7 +5,10 @@
 //   Kernel / makeKernel (118-125), gaussianKernel1D (127-145),
 //   gaussianKernel2D (147-155), filter2D (157-194), sepFilter2D (196-203),
 //   boxFilter (205-253), blur (255-259), gaussianBlur (261-267),
-//   medianBlur (300-332), bilateralFilter (334-387).
+//   medianBlur (300-332), bilateralFilter (334-387), and the derivative and
+//   sharpening filters of 388-565: sobel, scharr, prewitt, laplacian,
+//   magnitude, phase, gradientMagnitude, unsharpMask, sharpen, emboss,
+//   motionBlurKernel, adaptiveThreshold.
 // Every call validates its arguments in the reference's order, then runs the
 // gfx950 kernels of libipfg.so through the C ABI (ipfg.h) and returns when
 // the result is in the output image. Link with -lipfg.
@@ -14,6 +17,7 @@
 // stream per thread); they do not change the reference's signatures.
 #pragma once
 
+#include <cmath>
 #include <memory>
 #include <string>
 #include <vector>
@@ -211,6 +215,148 @@ Image<T> bilateralFilter(const Image<T>& src, int d, double sigmaColor, double s
     const ipfg_image s = detail::describe(src), o = detail::describe(out);
     detail::check(ipfg_bilateral(&s, &o, d, sigmaColor, sigmaSpace, static_cast<int>(border), detail::precision(),
                                  gpu::streamSetting()));
+    return out;
+}
+
+/// Sobel derivative of order (dx, dy), odd ksize in [1, 31] (1: central
+/// differences without smoothing); scale multiplies the result.
+template <class T>
+ImageF sobel(const Image<T>& src, int dx, int dy, int ksize = 3, double scale = 1.0,
+             BorderMode border = BorderMode::Reflect101) {
+    ImageF out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_sobel(&s, &d, dx, dy, ksize, scale, static_cast<int>(border), detail::precision(),
+                             gpu::streamSetting()));
+    return out;
+}
+
+/// 3x3 Scharr derivative; (dx, dy) is (1, 0) or (0, 1).
+template <class T>
+ImageF scharr(const Image<T>& src, int dx, int dy, double scale = 1.0, BorderMode border = BorderMode::Reflect101) {
+    ImageF out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_scharr(&s, &d, dx, dy, scale, static_cast<int>(border), detail::precision(),
+                              gpu::streamSetting()));
+    return out;
+}
+
+/// 3x3 Prewitt derivative (box-smoothed central difference); (dx, dy) is (1, 0) or (0, 1).
+template <class T>
+ImageF prewitt(const Image<T>& src, int dx, int dy, BorderMode border = BorderMode::Reflect101) {
+    ImageF out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_prewitt(&s, &d, dx, dy, static_cast<int>(border), detail::precision(), gpu::streamSetting()));
+    return out;
+}
+
+/// Laplacian: ksize 1 correlates with [0 1 0; 1 -4 1; 0 1 0] * scale; larger
+/// ksize adds the second-order Sobel derivatives in x and y (float sums).
+template <class T>
+ImageF laplacian(const Image<T>& src, int ksize = 1, double scale = 1.0, BorderMode border = BorderMode::Reflect101) {
+    ImageF out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_laplacian(&s, &d, ksize, scale, static_cast<int>(border), detail::precision(),
+                                 gpu::streamSetting()));
+    return out;
+}
+
+/// hypot(gx, gy) per sample.
+inline ImageF magnitude(const ImageF& gx, const ImageF& gy) {
+    detail::require(gx.sameShape(gy), "magnitude: shape mismatch");
+    ImageF out(gx.width(), gx.height(), gx.channels());
+    const ipfg_image a = detail::describe(gx), b = detail::describe(gy), d = detail::describe(out);
+    detail::check(ipfg_magnitude(&a, &b, &d, gpu::streamSetting()));
+    return out;
+}
+
+/// atan2(gy, gx) in degrees [0, 360) (radians [0, 2 pi) when degrees is false).
+inline ImageF phase(const ImageF& gx, const ImageF& gy, bool degrees = true) {
+    detail::require(gx.sameShape(gy), "phase: shape mismatch");
+    ImageF out(gx.width(), gx.height(), gx.channels());
+    const ipfg_image a = detail::describe(gx), b = detail::describe(gy), d = detail::describe(out);
+    detail::check(ipfg_phase(&a, &b, &d, degrees ? 1 : 0, gpu::streamSetting()));
+    return out;
+}
+
+/// Magnitude of the Sobel gradient (dx = 1 and dy = 1 images of ksize).
+template <class T>
+ImageF gradientMagnitude(const Image<T>& src, int ksize = 3) {
+    return magnitude(sobel(src, 1, 0, ksize), sobel(src, 0, 1, ksize));
+}
+
+/// out = src + amount * (src - gaussianBlur(src, sigma)) where
+/// |src - blurred| > threshold, src elsewhere; saturated to T.
+template <class T>
+Image<T> unsharpMask(const Image<T>& src, double sigma = 1.0, double amount = 1.0, double threshold = 0.0) {
+    Image<T> out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_unsharp_mask(&s, &d, sigma, amount, threshold, detail::precision(), gpu::streamSetting()));
+    return out;
+}
+
+/// 3x3 sharpening kernel {0 -s 0; -s 1+4s -s; 0 -s 0}, s = float(strength).
+template <class T>
+Image<T> sharpen(const Image<T>& src, double strength = 1.0) {
+    Image<T> out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_sharpen(&s, &d, strength, detail::precision(), gpu::streamSetting()));
+    return out;
+}
+
+/// 3x3 emboss kernel {-2 -1 0; -1 1 1; 0 1 2}.
+template <class T>
+Image<T> emboss(const Image<T>& src) {
+    Image<T> out(src.width(), src.height(), src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_emboss(&s, &d, detail::precision(), gpu::streamSetting()));
+    return out;
+}
+
+/// Normalised line of `length` pixels through the centre of a square kernel
+/// (odd side length or length + 1) at angleDegrees, counter-clockwise from
+/// horizontal, drawn from 4 * length points. Built on the host.
+inline Kernel motionBlurKernel(int length, double angleDegrees) {
+    detail::require(length >= 1, "motionBlurKernel: length must be >= 1");
+    const int n = length | 1;
+    Kernel k(n, n, 1, 0.f);
+    const double rad = angleDegrees * detail::kPi / 180.0, mid = 0.5 * (n - 1);
+    const double cx = std::cos(rad), sy = std::sin(rad);
+    const int points = 4 * length;
+    for (int i = 0; i < points; ++i) {
+        const double t = (length - 1) * (static_cast<double>(i) / (points - 1) - 0.5);
+        const int x = clamp(static_cast<int>(std::lround(mid + t * cx)), 0, n - 1);
+        const int y = clamp(static_cast<int>(std::lround(mid - t * sy)), 0, n - 1);
+        k(x, y) = 1.f;
+    }
+    float total = 0.f;
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) total += k(x, y);
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) k(x, y) /= total;
+    return k;
+}
+
+/// Threshold types (the reference declares them in pixel_ops.hpp:253-259);
+/// adaptiveThreshold takes Binary and BinaryInv.
+enum class ThresholdType { Binary, BinaryInv, Trunc, ToZero, ToZeroInv };
+
+/// Local threshold of adaptiveThreshold: the blockSize box mean or the
+/// Gaussian-weighted mean (Replicate border), minus C.
+enum class AdaptiveMethod { Mean, Gaussian };
+
+/// maxval where src - localMean > -C (Binary; BinaryInv swaps the outputs),
+/// 0 elsewhere; single-channel images, odd blockSize >= 3.
+template <class T>
+Image<T> adaptiveThreshold(const Image<T>& src, double maxval, AdaptiveMethod method, ThresholdType type, int blockSize,
+                           double C) {
+    detail::require(src.channels() == 1, "adaptiveThreshold: expected a single-channel image");
+    detail::require(blockSize >= 3 && blockSize % 2 == 1, "adaptiveThreshold: blockSize must be odd and >= 3");
+    detail::require(type == ThresholdType::Binary || type == ThresholdType::BinaryInv,
+                    "adaptiveThreshold: type must be Binary or BinaryInv");
+    Image<T> out(src.width(), src.height(), 1);
+    const ipfg_image s = detail::describe(src), d = detail::describe(out);
+    detail::check(ipfg_adaptive_threshold(&s, &d, maxval, static_cast<int>(method), static_cast<int>(type), blockSize,
+                                          C, detail::precision(), gpu::streamSetting()));
     return out;
 }
 

@@ -174,6 +174,41 @@ int ipfg_median(const ipfg_image* src, const ipfg_image* dst, int ksize, int bor
 int ipfg_bilateral(const ipfg_image* src, const ipfg_image* dst, int d, double sigma_color, double sigma_space,
                    int border, int precision, ipfg_stream stream);
 
+/* ---------------------------------------------------------------- derivatives and sharpening
+ * filter.hpp:388-567: compositions of the filters above with elementwise
+ * steps (run on the device). Derivative, Laplacian, magnitude and phase
+ * images are float32 (dst->dtype == IPFG_F32); unsharpMask, sharpen, emboss
+ * and adaptiveThreshold keep the source type. Argument errors carry the
+ * reference's messages. */
+/* sobel(src, dx, dy, ksize, scale, border), filter.hpp:392-406 */
+int ipfg_sobel(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, int ksize, double scale, int border,
+               int precision, ipfg_stream stream);
+/* scharr(src, dx, dy, scale, border), filter.hpp:408-413 */
+int ipfg_scharr(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, double scale, int border, int precision,
+                ipfg_stream stream);
+/* prewitt(src, dx, dy, border), filter.hpp:415-421 */
+int ipfg_prewitt(const ipfg_image* src, const ipfg_image* dst, int dx, int dy, int border, int precision,
+                 ipfg_stream stream);
+/* laplacian(src, ksize, scale, border), filter.hpp:423-436 */
+int ipfg_laplacian(const ipfg_image* src, const ipfg_image* dst, int ksize, double scale, int border, int precision,
+                   ipfg_stream stream);
+/* magnitude(gx, gy), filter.hpp:438-444; phase(gx, gy, degrees), 446-458.
+ * magnitude is bit-identical to the reference (FP64 sqrt(x^2 + y^2) rounded
+ * to float); phase takes atan2 in FP64 rounded to float, within one ulp of
+ * the host's atan2f before the degree scaling. */
+int ipfg_magnitude(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, ipfg_stream stream);
+int ipfg_phase(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, int degrees, ipfg_stream stream);
+/* unsharpMask(src, sigma, amount, threshold), filter.hpp:472-483 */
+int ipfg_unsharp_mask(const ipfg_image* src, const ipfg_image* dst, double sigma, double amount, double threshold,
+                      int precision, ipfg_stream stream);
+/* sharpen(src, strength), filter.hpp:485-490; emboss(src), 492-496 */
+int ipfg_sharpen(const ipfg_image* src, const ipfg_image* dst, double strength, int precision, ipfg_stream stream);
+int ipfg_emboss(const ipfg_image* src, const ipfg_image* dst, int precision, ipfg_stream stream);
+/* adaptiveThreshold(src, maxval, method (0 Mean, 1 Gaussian), type (0
+ * Binary, 1 BinaryInv), blockSize, C), filter.hpp:545-565 */
+int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type,
+                            int block_size, double c, int precision, ipfg_stream stream);
+
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference (its filters run one image per call).
  * A batch runs many uint8 images (1, 3 or 4 channels, any sizes) through

@@ -19,6 +19,13 @@
 //   box filter / blur ....... include/ipf/filter.hpp:205-259
 //   median .................. include/ipf/filter.hpp:269-332
 //   bilateral ............... include/ipf/filter.hpp:334-387
+//   derivative kernels ...... include/ipf/filter.hpp:93-114
+//   sobel/scharr/prewitt .... include/ipf/filter.hpp:392-421
+//   laplacian ............... include/ipf/filter.hpp:423-436
+//   magnitude / phase ....... include/ipf/filter.hpp:438-458
+//   unsharp/sharpen/emboss .. include/ipf/filter.hpp:472-496
+//   motion-blur kernel ...... include/ipf/filter.hpp:498-516
+//   adaptive threshold ...... include/ipf/filter.hpp:545-565
 //
 // Pinning: every function here is pinned by tests/test_oracle.py against the
 // reference-built fixtures in tests/golden/ and against closed forms, brute force
@@ -565,6 +572,271 @@ int ipfo_bilateral(const void* src, int w, int h, int cn, int dtype, void* dst, 
                          border);
             return 0;
         });
+    });
+}
+
+// ---------------------------------------------------------------- derivatives and sharpening
+}  // extern "C"
+
+namespace {
+
+constexpr double kPiO = 3.14159265358979323846;
+
+// derivKernel (filter.hpp:93-114): binomial smoothing (1, 1) convolved
+// ksize - 1 - order times, then the difference (-1, 1) order times; ksize 1
+// gives the unsmoothed central differences.
+std::vector<double> deriv_kernel(int order, int ksize) {
+    if (ksize == 1) {
+        if (order == 0) return {1.0};
+        if (order == 1) return {-1.0, 0.0, 1.0};
+        if (order == 2) return {1.0, -2.0, 1.0};
+        throw OracleError{"ipf: derivKernel: order > 2 requires ksize >= 3"};
+    }
+    check(ksize % 2 == 1 && ksize >= 3 && ksize <= 31, "derivKernel: ksize must be odd in [3, 31]");
+    check(order >= 0 && order < ksize, "derivKernel: invalid derivative order");
+    std::vector<double> k{1.0};
+    auto step = [&k](double a, double b) {
+        std::vector<double> r(k.size() + 1, 0.0);
+        for (size_t i = 0; i < k.size(); ++i) {
+            r[i] += k[i] * a;
+            r[i + 1] += k[i] * b;
+        }
+        k.swap(r);
+    };
+    for (int i = 0; i < ksize - 1 - order; ++i) step(1, 1);
+    for (int i = 0; i < order; ++i) step(-1, 1);
+    return k;
+}
+
+template <class Ti>
+void sep_float(const Ti* src, int w, int h, int cn, float* dst, const std::vector<double>& kx,
+               const std::vector<double>& ky, int border) {
+    const int kw = static_cast<int>(kx.size()), kh = static_cast<int>(ky.size());
+    sep_filter<Ti, float>(src, w, h, cn, dst, kx.data(), kw, ky.data(), kh, kw / 2, kh / 2, border, 0.0, 0.0);
+}
+
+// sobel (filter.hpp:392-406)
+template <class Ti>
+void sobel(const Ti* src, int w, int h, int cn, float* dst, int dx, int dy, int ksize, double scale, int border) {
+    check(dx >= 0 && dy >= 0 && dx + dy > 0, "sobel: need dx + dy > 0");
+    std::vector<double> kx = deriv_kernel(dx, ksize), ky = deriv_kernel(dy, ksize);
+    if (ksize == 1) {  // both kernels odd-length so the anchors stay centred
+        if (kx.size() == 1) kx = {0.0, 1.0, 0.0};
+        if (ky.size() == 1) ky = {0.0, 1.0, 0.0};
+    }
+    for (double& v : kx) v *= scale;
+    sep_float(src, w, h, cn, dst, kx, ky, border);
+}
+
+// filter2D with a 3 x 3 float kernel and the reference's defaults (Reflect101,
+// centre anchor, no delta)
+template <class Ti, class To>
+void filter3x3(const Ti* src, int w, int h, int cn, To* dst, const float (&k)[9], int border) {
+    filter2d<Ti, To>(src, w, h, cn, dst, k, 3, 3, 1, 1, border, 0.0, 0.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipfo_sobel(const void* src, int w, int h, int cn, int tin, float* dst, int dx, int dy, int ksize, double scale,
+               int border) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            sobel<Ti>(static_cast<const Ti*>(src), w, h, cn, dst, dx, dy, ksize, scale, border);
+            return 0;
+        });
+    });
+}
+
+// scharr, prewitt (filter.hpp:408-421): a central difference along the
+// derivative axis, a smoothing triple across it.
+int ipfo_scharr(const void* src, int w, int h, int cn, int tin, float* dst, int dx, int dy, double scale, int border) {
+    return guarded([&] {
+        check((dx == 1 && dy == 0) || (dx == 0 && dy == 1), "scharr: (dx, dy) must be (1,0) or (0,1)");
+        check_image(w, h, cn);
+        const std::vector<double> d = {-scale, 0.0, scale}, s = {3.0, 10.0, 3.0};
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            sep_float<Ti>(static_cast<const Ti*>(src), w, h, cn, dst, dx ? d : s, dx ? s : d, border);
+            return 0;
+        });
+    });
+}
+
+int ipfo_prewitt(const void* src, int w, int h, int cn, int tin, float* dst, int dx, int dy, int border) {
+    return guarded([&] {
+        check((dx == 1 && dy == 0) || (dx == 0 && dy == 1), "prewitt: (dx, dy) must be (1,0) or (0,1)");
+        check_image(w, h, cn);
+        const std::vector<double> d = {-1.0, 0.0, 1.0}, s = {1.0, 1.0, 1.0};
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            sep_float<Ti>(static_cast<const Ti*>(src), w, h, cn, dst, dx ? d : s, dx ? s : d, border);
+            return 0;
+        });
+    });
+}
+
+// laplacian (filter.hpp:423-436): ksize 1 is the 4-neighbour 3 x 3 kernel
+// (weights scaled in double, rounded to float); otherwise the float sum of the
+// second-order Sobel derivatives in x and in y.
+int ipfo_laplacian(const void* src, int w, int h, int cn, int tin, float* dst, int ksize, double scale, int border) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            const Ti* s = static_cast<const Ti*>(src);
+            if (ksize == 1) {
+                const float base[9] = {0, 1, 0, 1, -4, 1, 0, 1, 0};
+                float k[9];
+                for (int i = 0; i < 9; ++i) k[i] = static_cast<float>(base[i] * scale);
+                filter3x3<Ti, float>(s, w, h, cn, dst, k, border);
+                return 0;
+            }
+            const size_t n = static_cast<size_t>(w) * h * cn;
+            std::vector<float> b(n);
+            sobel<Ti>(s, w, h, cn, dst, 2, 0, ksize, scale, border);
+            sobel<Ti>(s, w, h, cn, b.data(), 0, 2, ksize, scale, border);
+            for (size_t i = 0; i < n; ++i) dst[i] += b[i];
+            return 0;
+        });
+    });
+}
+
+// magnitude, phase (filter.hpp:438-458)
+int ipfo_magnitude(const float* gx, const float* gy, int w, int h, int cn, float* dst) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const size_t n = static_cast<size_t>(w) * h * cn;
+        for (size_t i = 0; i < n; ++i) dst[i] = std::hypot(gx[i], gy[i]);
+        return 0;
+    });
+}
+
+int ipfo_phase(const float* gx, const float* gy, int w, int h, int cn, float* dst, int degrees) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const size_t n = static_cast<size_t>(w) * h * cn;
+        const double full = degrees ? 360.0 : 2 * kPiO;
+        for (size_t i = 0; i < n; ++i) {
+            double a = std::atan2(gy[i], gx[i]);
+            if (degrees) a *= 180.0 / kPiO;
+            if (a < 0) a += full;
+            dst[i] = static_cast<float>(a);
+        }
+        return 0;
+    });
+}
+
+// unsharpMask (filter.hpp:472-483): FP64 Gaussian blur (Reflect101), then
+// v + amount * (v - blurred) where |v - blurred| > threshold.
+int ipfo_unsharp(const void* src, int w, int h, int cn, int dtype, void* dst, double sigma, double amount,
+                 double threshold) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const int ks = ipfo_gaussian_kernel_1d(sigma, 0, nullptr, 0);
+        check(ks > 0, "gaussianKernel1D: need sigma > 0 or ksize > 0");
+        std::vector<double> k(static_cast<size_t>(ks));
+        ipfo_gaussian_kernel_1d(sigma, 0, k.data(), ks);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const T* s = static_cast<const T*>(src);
+            T* d = static_cast<T*>(dst);
+            const size_t n = static_cast<size_t>(w) * h * cn;
+            std::vector<double> blurred(n);
+            sep_filter<T, double>(s, w, h, cn, blurred.data(), k.data(), ks, k.data(), ks, ks / 2, ks / 2, B_REFLECT101,
+                                  0.0, 0.0);
+            for (size_t i = 0; i < n; ++i) {
+                const double v = s[i], diff = v - blurred[i];
+                d[i] = sat<T>(std::fabs(diff) > threshold ? v + amount * diff : v);
+            }
+            return 0;
+        });
+    });
+}
+
+// sharpen, emboss (filter.hpp:485-496): 3 x 3 filter2D to the input type.
+int ipfo_sharpen(const void* src, int w, int h, int cn, int dtype, void* dst, double strength) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const float s = static_cast<float>(strength);
+        const float k[9] = {0, -s, 0, -s, 1 + 4 * s, -s, 0, -s, 0};
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            filter3x3<T, T>(static_cast<const T*>(src), w, h, cn, static_cast<T*>(dst), k, B_REFLECT101);
+            return 0;
+        });
+    });
+}
+
+int ipfo_emboss(const void* src, int w, int h, int cn, int dtype, void* dst) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const float k[9] = {-2, -1, 0, -1, 1, 1, 0, 1, 2};
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            filter3x3<T, T>(static_cast<const T*>(src), w, h, cn, static_cast<T*>(dst), k, B_REFLECT101);
+            return 0;
+        });
+    });
+}
+
+// adaptiveThreshold (filter.hpp:545-565): local mean (box, or Gaussian of
+// sigma 0 = from the block size) with Replicate borders, in the input type;
+// maxval where (src - mean > -C) matches Binary, else 0. method 0 Mean, 1
+// Gaussian; type 0 Binary, 1 BinaryInv.
+int ipfo_adaptive_threshold(const void* src, int w, int h, int dtype, void* dst, double maxval, int method, int type,
+                            int block, double c) {
+    return guarded([&] {
+        check_image(w, h, 1);
+        check(block >= 3 && block % 2 == 1, "adaptiveThreshold: blockSize must be odd and >= 3");
+        check(type == 0 || type == 1, "adaptiveThreshold: type must be Binary or BinaryInv");
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const T* s = static_cast<const T*>(src);
+            T* d = static_cast<T*>(dst);
+            const size_t n = static_cast<size_t>(w) * h;
+            std::vector<T> mean(n);
+            if (method == 0) {
+                box_filter<T, T>(s, w, h, 1, mean.data(), block, block, true, B_REPLICATE, 0.0);
+            } else {
+                std::vector<double> k(static_cast<size_t>(block));
+                ipfo_gaussian_kernel_1d(0.0, block, k.data(), block);
+                sep_filter<T, T>(s, w, h, 1, mean.data(), k.data(), block, k.data(), block, block / 2, block / 2,
+                                 B_REPLICATE, 0.0, 0.0);
+            }
+            for (size_t i = 0; i < n; ++i) {
+                const bool above = static_cast<double>(s[i]) - static_cast<double>(mean[i]) > -c;
+                d[i] = sat<T>((above == (type == 0)) ? maxval : 0.0);
+            }
+            return 0;
+        });
+    });
+}
+
+// motionBlurKernel (filter.hpp:498-516): 4 * length samples along the line
+// through the centre at the angle, marked 1, then normalised by their float
+// sum. Returns the kernel side; writes at most cap weights (row-major).
+int ipfo_motion_kernel(int length, double angle, float* out, int cap) {
+    return guarded([&] {
+        check(length >= 1, "motionBlurKernel: length must be >= 1");
+        const int size = length % 2 ? length : length + 1;
+        std::vector<float> k(static_cast<size_t>(size) * size, 0.f);
+        const double a = angle * kPiO / 180.0, c = (size - 1) / 2.0;
+        const int samples = 4 * length;
+        for (int i = 0; i < samples; ++i) {
+            const double t = (length - 1) * (static_cast<double>(i) / (samples - 1) - 0.5);
+            const int x = static_cast<int>(std::lround(c + t * std::cos(a)));
+            const int y = static_cast<int>(std::lround(c - t * std::sin(a)));
+            k[static_cast<size_t>(std::clamp(y, 0, size - 1)) * size + std::clamp(x, 0, size - 1)] = 1.f;
+        }
+        float s = 0;
+        for (float v : k) s += v;
+        for (float& v : k) v /= s;
+        for (int i = 0; i < size * size && i < cap; ++i) out[i] = k[i];
+        return size;
     });
 }
 

@@ -74,6 +74,17 @@ class Oracle:
             "filter2d": (_I, [_V, _I, _I, _I, _I, _V, _I, _PF, _I, _I, _I, _I, _I, _D, _D]),
             "median": (_I, [_V, _I, _I, _I, _I, _V, _I, _I]),
             "bilateral": (_I, [_V, _I, _I, _I, _I, _V, _I, _D, _D, _I]),
+            "sobel": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _I, _D, _I]),
+            "scharr": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _D, _I]),
+            "prewitt": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _I]),
+            "laplacian": (_I, [_V, _I, _I, _I, _I, _V, _I, _D, _I]),
+            "magnitude": (_I, [_V, _V, _I, _I, _I, _V]),
+            "phase": (_I, [_V, _V, _I, _I, _I, _V, _I]),
+            "unsharp": (_I, [_V, _I, _I, _I, _I, _V, _D, _D, _D]),
+            "sharpen": (_I, [_V, _I, _I, _I, _I, _V, _D]),
+            "emboss": (_I, [_V, _I, _I, _I, _I, _V]),
+            "adaptive_threshold": (_I, [_V, _I, _I, _I, _V, _D, _I, _I, _I, _D]),
+            "motion_kernel": (_I, [_I, _D, _PF, _I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(self.lib, pre + name)
@@ -162,6 +173,70 @@ class Oracle:
         src, h, w, cn = self._shape(src)
         out = np.empty_like(src)
         self._check(self._f("median")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, ksize, border))
+        return out
+
+    # ---- derivative and sharpening filters (filter.hpp:388-567)
+    def _float_op(self, name, src, *args):
+        src, h, w, cn = self._shape(src)
+        out = np.empty(src.shape, np.float32)
+        self._check(self._f(name)(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, *args))
+        return out
+
+    def sobel(self, src, dx, dy, ksize=3, scale=1.0, border=REFLECT101):
+        return self._float_op("sobel", src, dx, dy, ksize, float(scale), border)
+
+    def scharr(self, src, dx, dy, scale=1.0, border=REFLECT101):
+        return self._float_op("scharr", src, dx, dy, float(scale), border)
+
+    def prewitt(self, src, dx, dy, border=REFLECT101):
+        return self._float_op("prewitt", src, dx, dy, border)
+
+    def laplacian(self, src, ksize=1, scale=1.0, border=REFLECT101):
+        return self._float_op("laplacian", src, ksize, float(scale), border)
+
+    def magnitude(self, gx, gy):
+        gx, h, w, cn = self._shape(np.asarray(gx, np.float32))
+        gy = np.ascontiguousarray(gy, np.float32)
+        out = np.empty(gx.shape, np.float32)
+        self._check(self._f("magnitude")(gx.ctypes.data, gy.ctypes.data, w, h, cn, out.ctypes.data))
+        return out
+
+    def phase(self, gx, gy, degrees=True):
+        gx, h, w, cn = self._shape(np.asarray(gx, np.float32))
+        gy = np.ascontiguousarray(gy, np.float32)
+        out = np.empty(gx.shape, np.float32)
+        self._check(self._f("phase")(gx.ctypes.data, gy.ctypes.data, w, h, cn, out.ctypes.data, int(bool(degrees))))
+        return out
+
+    def _same_op(self, name, src, *args):
+        src, h, w, cn = self._shape(src)
+        out = np.empty_like(src)
+        self._check(self._f(name)(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, *args))
+        return out
+
+    def unsharp_mask(self, src, sigma=1.0, amount=1.0, threshold=0.0):
+        return self._same_op("unsharp", src, float(sigma), float(amount), float(threshold))
+
+    def sharpen(self, src, strength=1.0):
+        return self._same_op("sharpen", src, float(strength))
+
+    def emboss(self, src):
+        return self._same_op("emboss", src)
+
+    def adaptive_threshold(self, src, maxval, method, ttype, block, c):
+        src = np.ascontiguousarray(src)
+        h, w = src.shape[:2]
+        out = np.empty_like(src)
+        self._check(self._f("adaptive_threshold")(src.ctypes.data, w, h, NP2DT[src.dtype], out.ctypes.data,
+                                                  float(maxval), int(method), int(ttype), int(block), float(c)))
+        return out
+
+    def motion_blur_kernel(self, length, angle):
+        n = self._f("motion_kernel")(int(length), float(angle), None, 0)
+        if n < 0:
+            self._check(n)
+        out = np.empty((n, n), np.float32)
+        self._f("motion_kernel")(int(length), float(angle), out.ctypes.data_as(_PF), n * n)
         return out
 
     def bilateral(self, src, d, sigma_color, sigma_space, border=REFLECT101):

@@ -166,4 +166,125 @@ int ipfr_bilateral(const void* src, int w, int h, int cn, int dtype, void* dst, 
     });
 }
 
+// ---- derivative and sharpening filters (filter.hpp:388-567)
+int ipfr_sobel(const void* src, int w, int h, int cn, int tin, float* dst, int dx, int dy, int ksize, double scale,
+               int border) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            const auto img = wrap<Ti>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::sobel(img, dx, dy, ksize, scale, bm(border)); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_scharr(const void* src, int w, int h, int cn, int tin, float* dst, int dx, int dy, double scale, int border) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            const auto img = wrap<Ti>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::scharr(img, dx, dy, scale, bm(border)); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_prewitt(const void* src, int w, int h, int cn, int tin, float* dst, int dx, int dy, int border) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            const auto img = wrap<Ti>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::prewitt(img, dx, dy, bm(border)); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_laplacian(const void* src, int w, int h, int cn, int tin, float* dst, int ksize, double scale, int border) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            const auto img = wrap<Ti>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::laplacian(img, ksize, scale, bm(border)); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_magnitude(const float* gx, const float* gy, int w, int h, int cn, float* dst) {
+    return guarded([&] {
+        const auto a = ipf::ImageF::fromData(w, h, cn, gx), b = ipf::ImageF::fromData(w, h, cn, gy);
+        unwrap(timed([&] { return ipf::magnitude(a, b); }), dst);
+        return 0;
+    });
+}
+
+int ipfr_phase(const float* gx, const float* gy, int w, int h, int cn, float* dst, int degrees) {
+    return guarded([&] {
+        const auto a = ipf::ImageF::fromData(w, h, cn, gx), b = ipf::ImageF::fromData(w, h, cn, gy);
+        unwrap(timed([&] { return ipf::phase(a, b, degrees != 0); }), dst);
+        return 0;
+    });
+}
+
+int ipfr_unsharp(const void* src, int w, int h, int cn, int dtype, void* dst, double sigma, double amount,
+                 double threshold) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::unsharpMask(img, sigma, amount, threshold); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_sharpen(const void* src, int w, int h, int cn, int dtype, void* dst, double strength) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::sharpen(img, strength); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_emboss(const void* src, int w, int h, int cn, int dtype, void* dst) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::emboss(img); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_adaptive_threshold(const void* src, int w, int h, int dtype, void* dst, double maxval, int method, int type,
+                            int block, double c) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, 1);
+            unwrap(timed([&] {
+                       return ipf::adaptiveThreshold(img, maxval, static_cast<ipf::AdaptiveMethod>(method),
+                                                     static_cast<ipf::ThresholdType>(type), block, c);
+                   }),
+                   dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_motion_kernel(int length, double angle, float* out, int cap) {
+    return guarded([&] {
+        const ipf::Kernel k = ipf::motionBlurKernel(length, angle);
+        const int n = static_cast<int>(k.sampleCount());
+        for (int i = 0; i < n && i < cap; ++i) out[i] = k.data()[i];
+        return k.width();
+    });
+}
+
 }  // extern "C"

@@ -4,9 +4,11 @@
 //
 //   frontend_check errors        argument checks only (no GPU needed)
 //   frontend_check run <dir>     writes inputs and outputs as raw files
+//   frontend_check motion n a..  prints motionBlurKernel(n, a) (no GPU needed)
 #include <ipf/ipf.hpp>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <functional>
@@ -49,6 +51,25 @@ int errors() {
                         "ipf: filter2D: kernel must be a non-empty single-channel image");
     bad += expect_error("gauss ksize", [&] { gaussianBlur(img, 1.0, 0.0, 4); }, "ipf: gaussianKernel1D: ksize must be odd");
     bad += expect_error("makeKernel", [&] { makeKernel(2, 2, {1, 2, 3}); }, "ipf: makeKernel: wrong number of values");
+    bad += expect_error("sobel order", [&] { sobel(img, 0, 0); }, "ipf: sobel: need dx + dy > 0");
+    bad += expect_error("sobel ksize", [&] { sobel(img, 1, 0, 4); },
+                        "ipf: derivKernel: ksize must be odd in [3, 31]");
+    bad += expect_error("sobel order 3", [&] { sobel(img, 3, 0, 1); },
+                        "ipf: derivKernel: order > 2 requires ksize >= 3");
+    bad += expect_error("scharr", [&] { scharr(img, 1, 1); }, "ipf: scharr: (dx, dy) must be (1,0) or (0,1)");
+    bad += expect_error("prewitt", [&] { prewitt(img, 0, 0); }, "ipf: prewitt: (dx, dy) must be (1,0) or (0,1)");
+    bad += expect_error("magnitude", [&] { magnitude(ImageF(3, 2), ImageF(2, 3)); }, "ipf: magnitude: shape mismatch");
+    bad += expect_error("phase", [&] { phase(ImageF(3, 2), ImageF(3, 2, 2)); }, "ipf: phase: shape mismatch");
+    bad += expect_error("adaptive channels",
+                        [&] { adaptiveThreshold(ImageU8(4, 4, 3), 255, AdaptiveMethod::Mean, ThresholdType::Binary, 3, 0); },
+                        "ipf: adaptiveThreshold: expected a single-channel image");
+    bad += expect_error("adaptive block",
+                        [&] { adaptiveThreshold(img, 255, AdaptiveMethod::Mean, ThresholdType::Binary, 4, 0); },
+                        "ipf: adaptiveThreshold: blockSize must be odd and >= 3");
+    bad += expect_error("adaptive type",
+                        [&] { adaptiveThreshold(img, 255, AdaptiveMethod::Gaussian, ThresholdType::Trunc, 5, 0); },
+                        "ipf: adaptiveThreshold: type must be Binary or BinaryInv");
+    bad += expect_error("motion", [&] { motionBlurKernel(0, 0.0); }, "ipf: motionBlurKernel: length must be >= 1");
     // empty images come back empty without device work; ksize 1 is a copy
     if (!gaussianBlur(ImageU8(0, 5, 3), 2.0).empty()) ++bad;
     if (!(medianBlur(img, 1) == img)) ++bad;
@@ -57,6 +78,17 @@ int errors() {
     if (borderIndex(-1, 4, BorderMode::Reflect101) != 1 || borderIndex(4, 4, BorderMode::Wrap) != 0) ++bad;
     std::printf(bad ? "ERRORS FAILED %d\n" : "ERRORS OK\n", bad);
     return bad ? 1 : 0;
+}
+
+// motionBlurKernel(length, angle) for each pair of arguments, one kernel a line.
+int motion(int argc, char** argv) {
+    for (int i = 0; i + 1 < argc; i += 2) {
+        const Kernel k = motionBlurKernel(std::atoi(argv[i]), std::atof(argv[i + 1]));
+        std::printf("%d", k.width());
+        for (float v : k) std::printf(" %.9g", static_cast<double>(v));
+        std::printf("\n");
+    }
+    return 0;
 }
 
 int run(const std::string& dir) {
@@ -85,6 +117,21 @@ int run(const std::string& dir) {
     gpu::setPrecision(gpu::Precision::Exact);
     save(dir + "/o10.bin", gaussianBlur(imgf, 3.0));
     gpu::setPrecision(gpu::Precision::Default);
+    ImageU8 gray(97, 61, 1);
+    for (int y = 0; y < gray.height(); ++y)
+        for (int x = 0; x < gray.width(); ++x) gray(x, y) = img(x, y, 1);
+    save(dir + "/o11.bin", sobel(img, 1, 0));
+    save(dir + "/o12.bin", scharr(imgf, 0, 1, 2.0, BorderMode::Replicate));
+    save(dir + "/o13.bin", laplacian(img, 3));
+    const ImageF gx = sobel(imgf, 1, 0), gy = sobel(imgf, 0, 1);
+    save(dir + "/o14.bin", magnitude(gx, gy));
+    save(dir + "/o15.bin", phase(gx, gy));
+    save(dir + "/o16.bin", unsharpMask(img, 1.5, 0.8, 2.0));
+    save(dir + "/o17.bin", sharpen(img, 0.5));
+    save(dir + "/o18.bin", emboss(img));
+    save(dir + "/o19.bin", adaptiveThreshold(gray, 255, AdaptiveMethod::Gaussian, ThresholdType::BinaryInv, 7, 3.0));
+    save(dir + "/o20.bin", prewitt(img, 1, 0, BorderMode::Wrap));
+    save(dir + "/o21.bin", gradientMagnitude(img));
     std::printf("RUN OK\n");
     return 0;
 }
@@ -93,6 +140,7 @@ int run(const std::string& dir) {
 int main(int argc, char** argv) {
     if (argc >= 2 && std::strcmp(argv[1], "errors") == 0) return errors();
     if (argc >= 3 && std::strcmp(argv[1], "run") == 0) return run(argv[2]);
-    std::printf("usage: frontend_check errors | run <dir>\n");
+    if (argc >= 2 && std::strcmp(argv[1], "motion") == 0) return motion(argc - 2, argv + 2);
+    std::printf("usage: frontend_check errors | run <dir> | motion (length angle)...\n");
     return 2;
 }

@@ -48,6 +48,35 @@ def run_oracle(o, case):
         return o.median(inp, p["k"], b)
     if c["op"] == "bilateral":
         return o.bilateral(inp, p["d"], p["sc"], p["ss"], b)
+    return _run_composite(o, case)
+
+
+def _run_composite(m, case, **kw):
+    """The derivative and sharpening cases; m is the oracle or the package,
+    which name these functions alike (kw: the package's precision)."""
+    c, inp, _, ex = case
+    p = c["params"]
+    b = p["border"]
+    if c["op"] == "sobel":
+        return m.sobel(inp, p["dx"], p["dy"], p["ksize"], p["scale"], b, **kw)
+    if c["op"] == "scharr":
+        return m.scharr(inp, p["dx"], p["dy"], p["scale"], b, **kw)
+    if c["op"] == "prewitt":
+        return m.prewitt(inp, p["dx"], p["dy"], b, **kw)
+    if c["op"] == "laplacian":
+        return m.laplacian(inp, p["ksize"], p["scale"], b, **kw)
+    if c["op"] == "unsharp":
+        return m.unsharp_mask(inp, p["sigma"], p["amount"], p["threshold"], **kw)
+    if c["op"] == "sharpen":
+        return m.sharpen(inp, p["strength"], **kw)
+    if c["op"] == "emboss":
+        return m.emboss(inp, **kw)
+    if c["op"] == "adaptive":
+        return m.adaptive_threshold(inp, p["maxval"], p["method"], p["type"], p["block"], p["c"], **kw)
+    if c["op"] == "magnitude":
+        return m.magnitude(inp, ex["gy"])
+    if c["op"] == "phase":
+        return m.phase(inp, ex["gy"], bool(p["degrees"]))
     raise ValueError(c["op"])
 
 
@@ -71,7 +100,7 @@ def run_gpu(ipf, case, precision=0):
         return ipf.median_blur(inp, p["k"], b)
     if c["op"] == "bilateral":
         return ipf.bilateral_filter(inp, p["d"], p["sc"], p["ss"], b, precision)
-    raise ValueError(c["op"])
+    return _run_composite(ipf, case, **({} if c["op"] in ("magnitude", "phase") else {"precision": precision}))
 
 
 def float_tolerance(case) -> np.ndarray | float:
@@ -81,6 +110,8 @@ def float_tolerance(case) -> np.ndarray | float:
     box         : FP64 window sums: 1e-12 * kw * kh * X + u |ref|
     filter2D    : (taps + 5) u * (sum|k| X + |delta|) + u |ref|
     bilateral   : (2 taps + 10) u * X (weights, two positive sums, one division)
+    phase       : one float ulp of atan2(gy, gx), times 180/pi in degrees,
+                  plus one ulp of the result
     where X bounds |input| (and the Constant border value), u = 2^-24.
     """
     c, inp, ref, ex = case
@@ -106,8 +137,19 @@ def float_tolerance(case) -> np.ndarray | float:
         k = ex["k"]
         taps = int((k != 0).sum())
         return (taps + 5) * U * (np.abs(k.astype(np.float64)).sum() * X + abs(p.get("delta", 0.0))) * 1.01 + U * r
+    if c["op"] == "phase":
+        return phase_tolerance(inp, ex["gy"], ref, bool(p["degrees"]))
     if c["op"] == "bilateral":
         r_ = p["d"] // 2 if p["d"] > 0 else int(round(p["ss"] * 1.5))
         taps = sum(1 for j in range(-r_, r_ + 1) for i in range(-r_, r_ + 1) if i * i + j * j <= r_ * r_)
         return (2 * taps + 10) * U * X * 1.01 + U * r
     return 0.0
+
+
+def phase_tolerance(gx, gy, ref, degrees) -> np.ndarray:
+    """One float ulp of the angle atan2(gy, gx) (scaled to degrees) plus one
+    ulp of the wrapped result: the bound for an atan2 that differs from the
+    host's atan2f by at most an ulp."""
+    t = np.abs(np.arctan2(gy.astype(np.float64), gx.astype(np.float64))).astype(np.float32)
+    scale = 180.0 / np.pi if degrees else 1.0
+    return (np.spacing(t).astype(np.float64) * scale + np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)) * 1.01

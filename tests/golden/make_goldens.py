@@ -9,7 +9,8 @@ expected outputs come from the reference compiled from its own headers
 Outputs:
   fixtures.npz     small cases: inputs, kernels and reference outputs
   manifest.json    the parameters of every case
-  kat.json         known answers: saturate_cast, borderIndex, gaussianKernel1D
+  kat.json         known answers: saturate_cast, borderIndex, gaussianKernel1D,
+                   motionBlurKernel
   fullsize.json    SHA-256 of reference outputs at the benchmark sizes
 """
 from __future__ import annotations
@@ -112,6 +113,39 @@ def main():
                 add("median", {"k": k, "border": b}, f32_1, ref.median(f32_1, k, b))
                 add("median", {"k": k, "border": b}, i16_1, ref.median(i16_1, k, b))
 
+    # Derivative and sharpening filters (filter.hpp:388-567), appended after
+    # the cases above so that their ids stay the same
+    i16_2 = (wl.uniform_f32((21, 26), seed=seed, stream=7) * 4000 - 2000).astype(np.int16)
+    for b in BORDERS:
+        for img, name in ((u8_3, "u8x3"), (f32_1, "f32x1"), (i16_2, "i16x1")):
+            for (dx, dy, k) in ((1, 0, 3), (0, 1, 3), (1, 1, 5), (2, 0, 1), (0, 2, 7), (1, 2, 31)):
+                add("sobel", {"dx": dx, "dy": dy, "ksize": k, "scale": 0.5, "border": b, "src": name}, img,
+                    ref.sobel(img, dx, dy, k, 0.5, b))
+            for (dx, dy) in ((1, 0), (0, 1)):
+                add("scharr", {"dx": dx, "dy": dy, "scale": 1.5, "border": b, "src": name}, img,
+                    ref.scharr(img, dx, dy, 1.5, b))
+                add("prewitt", {"dx": dx, "dy": dy, "border": b, "src": name}, img, ref.prewitt(img, dx, dy, b))
+            for k in (1, 3, 5):
+                add("laplacian", {"ksize": k, "scale": 2.0, "border": b, "src": name}, img,
+                    ref.laplacian(img, k, 2.0, b))
+    for img, name in ((u8_3, "u8x3"), (f32_1, "f32x1"), (i16_2, "i16x1"), (poly, "poly")):
+        for (s, a, th) in ((1.0, 1.0, 0.0), (2.5, 0.6, 4.0)):
+            add("unsharp", {"sigma": s, "amount": a, "threshold": th, "border": 3, "src": name}, img,
+                ref.unsharp_mask(img, s, a, th))
+        for st in (1.0, 0.35):
+            add("sharpen", {"strength": st, "border": 3, "src": name}, img, ref.sharpen(img, st))
+        add("emboss", {"border": 3, "src": name}, img, ref.emboss(img))
+    for img, name in ((u8_1, "u8x1"), (f32_1, "f32x1"), (i16_2, "i16x1")):
+        for m in (0, 1):
+            for tt in (0, 1):
+                for bs in (3, 9):
+                    add("adaptive", {"method": m, "type": tt, "block": bs, "maxval": 200.0, "c": 2.5, "border": 1,
+                                     "src": name}, img, ref.adaptive_threshold(img, 200.0, m, tt, bs, 2.5))
+    gx, gy = ref.sobel(u8_3, 1, 0, 3), ref.sobel(u8_3, 0, 1, 3)
+    add("magnitude", {"border": 3}, gx, ref.magnitude(gx, gy), gy=gy)
+    for deg in (1, 0):
+        add("phase", {"degrees": deg, "border": 3}, gx, ref.phase(gx, gy, bool(deg)), gy=gy)
+
     np.savez_compressed(OUT / "fixtures.npz", **arrays)
     (OUT / "manifest.json").write_text(json.dumps(cases, indent=0))
 
@@ -127,6 +161,11 @@ def main():
                 kat["border"].append([p, n, m, ref.border_index(p, n, m)])
     for s, k in ((0.5, 0), (1.0, 0), (2.0, 0), (3.0, 0), (8.0, 0), (0.0, 5), (1.5, 9), (-1.0, 7)):
         kat["gauss"].append([s, k, [float(x) for x in ref.gaussian_kernel_1d(s, k)]])
+    # motionBlurKernel (filter.hpp:498-516): float32 values as exact decimal strings
+    kat["motion"] = []
+    for n, a in ((1, 0.0), (2, 0.0), (5, 0.0), (5, 90.0), (7, 30.0), (8, 45.0), (9, -20.0), (12, 135.0),
+                 (15, 200.0), (21, 73.5)):
+        kat["motion"].append([n, a, [float(x) for x in ref.motion_blur_kernel(n, a).ravel()]])
     (OUT / "kat.json").write_text(json.dumps(kat))
 
     # Full-size reference outputs (SHA-256): integer paths are bit-exact on the GPU.

@@ -4,11 +4,15 @@ messages before touching a device. No kernel runs here."""
 from __future__ import annotations
 
 import ctypes as C
+import json
 import re
 from pathlib import Path
 
 import numpy as np
 import pytest
+
+import imageprocessingflex_amd as ipf
+from tests import fixtures
 
 ROOT = Path(__file__).resolve().parents[1]
 HEADER = ROOT / "include" / "ipfg.h"
@@ -46,6 +50,15 @@ def test_gaussian_kernel_host_matches_oracle(oracle):
         ipf.gaussian_kernel_1d(0.0, 0)
 
 
+def test_motion_blur_kernel_known_answers():
+    kat = json.loads((fixtures.GOLDEN / "kat.json").read_text())["motion"]
+    for n, a, want in kat:
+        got = ipf.motion_blur_kernel(n, a)
+        assert got.dtype == np.float32 and np.array_equal(got.ravel(), np.array(want, np.float32)), (n, a)
+    with pytest.raises(ipf.IpfError, match="length must be >= 1"):
+        ipf.motion_blur_kernel(0, 10.0)
+
+
 def test_argument_errors_precede_device_work():
     import imageprocessingflex_amd as ipf
     img = np.zeros((4, 5), np.uint8)
@@ -61,6 +74,19 @@ def test_argument_errors_precede_device_work():
         (lambda: ipf.filter2d(img, np.ones((0, 3))), "ipf: filter2D: kernel must be a non-empty single-channel image"),
         (lambda: ipf.gaussian_blur(img, 0.0), "ipf: gaussianKernel1D: need sigma > 0 or ksize > 0"),
         (lambda: ipf.gaussian_blur(img, 1.0, ksize=4), "ipf: gaussianKernel1D: ksize must be odd"),
+        (lambda: ipf.sobel(img, 0, 0), "ipf: sobel: need dx + dy > 0"),
+        (lambda: ipf.sobel(img, 1, 0, 33), "ipf: derivKernel: ksize must be odd in [3, 31]"),
+        (lambda: ipf.sobel(img, 0, 3, 1), "ipf: derivKernel: order > 2 requires ksize >= 3"),
+        (lambda: ipf.sobel(img, 3, 0, 3), "ipf: derivKernel: invalid derivative order"),
+        (lambda: ipf.scharr(img, 1, 1), "ipf: scharr: (dx, dy) must be (1,0) or (0,1)"),
+        (lambda: ipf.prewitt(img, 2, 0), "ipf: prewitt: (dx, dy) must be (1,0) or (0,1)"),
+        (lambda: ipf.laplacian(img, 2), "ipf: derivKernel: ksize must be odd in [3, 31]"),
+        (lambda: ipf.magnitude(np.zeros((2, 3)), np.zeros((3, 2))), "ipf: magnitude: shape mismatch"),
+        (lambda: ipf.phase(np.zeros((2, 3)), np.zeros((2, 3, 2))), "ipf: phase: shape mismatch"),
+        (lambda: ipf.adaptive_threshold(np.zeros((4, 4, 3), np.uint8), 255, 0, 0, 3, 0.0),
+         "ipf: adaptiveThreshold: expected a single-channel image"),
+        (lambda: ipf.adaptive_threshold(img, 255, 0, 0, 1, 0.0), "ipf: adaptiveThreshold: blockSize must be odd and >= 3"),
+        (lambda: ipf.adaptive_threshold(img, 255, 1, 2, 5, 0.0), "ipf: adaptiveThreshold: type must be Binary or BinaryInv"),
     ]
     for fn, msg in cases:
         with pytest.raises(ipf.IpfError) as e:

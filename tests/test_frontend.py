@@ -9,6 +9,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from tests.fixtures import phase_tolerance
+
 ROOT = Path(__file__).resolve().parents[1]
 EXE = ROOT / "tests" / "cpp" / "frontend_check"
 
@@ -51,3 +53,34 @@ def test_frontend_results_match_oracle(exe, oracle, tmp_path):
     assert np.array_equal(out(8, np.uint8, u8.shape), oracle.median(u8, 3, 3))
     assert np.array_equal(out(9, np.uint8, u8.shape), oracle.bilateral(u8, 9, 75.0, 75.0))
     assert np.array_equal(out(10, np.float32, f.shape), oracle.gaussian_blur(f, 3.0))
+    # derivative and sharpening filters
+    assert np.array_equal(out(11, np.float32, u8.shape), oracle.sobel(u8, 1, 0))
+    assert np.array_equal(out(12, np.float32, f.shape), oracle.scharr(f, 0, 1, 2.0, 1))
+    assert np.array_equal(out(13, np.float32, u8.shape), oracle.laplacian(u8, 3))
+    gx, gy = oracle.sobel(f, 1, 0), oracle.sobel(f, 0, 1)
+    assert np.array_equal(out(14, np.float32, f.shape), oracle.magnitude(gx, gy))
+    want = oracle.phase(gx, gy)
+    assert np.all(np.abs(out(15, np.float32, f.shape) - want.astype(np.float64)) <= phase_tolerance(gx, gy, want, True))
+    assert np.array_equal(out(16, np.uint8, u8.shape), oracle.unsharp_mask(u8, 1.5, 0.8, 2.0))
+    assert np.array_equal(out(17, np.uint8, u8.shape), oracle.sharpen(u8, 0.5))
+    assert np.array_equal(out(18, np.uint8, u8.shape), oracle.emboss(u8))
+    gray = np.ascontiguousarray(u8[:, :, 1])
+    assert np.array_equal(out(19, np.uint8, gray.shape), oracle.adaptive_threshold(gray, 255, 1, 1, 7, 3.0))
+    assert np.array_equal(out(20, np.float32, u8.shape), oracle.prewitt(u8, 1, 0, 4))
+    assert np.array_equal(out(21, np.float32, u8.shape),
+                          oracle.magnitude(oracle.sobel(u8, 1, 0), oracle.sobel(u8, 0, 1)))
+
+
+def test_frontend_motion_blur_kernel_known_answers(exe):
+    import json
+    from tests import fixtures
+    kat = json.loads((fixtures.GOLDEN / "kat.json").read_text())["motion"]
+    args = [str(x) for n, a, _ in kat for x in (n, repr(a))]
+    r = subprocess.run([str(exe), "motion", *args], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split("\n")
+    for (n, a, want), line in zip(kat, lines):
+        vals = line.split()
+        side = int(vals[0])
+        got = np.array([float(v) for v in vals[1:]], np.float32)
+        assert side * side == len(want) and np.array_equal(got, np.array(want, np.float32)), (n, a)

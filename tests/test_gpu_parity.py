@@ -28,11 +28,14 @@ def _exact_expected(case, precision) -> bool:
     output, EXACT replays the reference's operation order; the exceptions are
     box sums of float images in DEFAULT (FP64 window sums instead of the
     reference's left-to-right running sums) and the float bilateral filter,
-    whose range weights use the device exp()."""
+    whose range weights use the device exp(), and phase, whose angle comes
+    from an FP64 atan2 rounded to float in place of the host's atan2f."""
     c, inp, want, _ = case
     if c["op"] == "median":
         return True
     if c["op"] == "bilateral" and inp.dtype.kind == "f":
+        return False
+    if c["op"] == "phase":
         return False
     if precision == 1:
         return True
@@ -238,6 +241,65 @@ def test_certified_calls_reuse_stream_state(ipf, oracle):
 # ------------------------------------------------------------- full-size runs
 FULL = json.loads((fixtures.GOLDEN / "fullsize.json").read_text())
 BNAME = {0: "constant", 1: "replicate", 2: "reflect", 3: "reflect101", 4: "wrap"}
+
+
+# ------------------------------------------- derivative and sharpening filters
+def test_derivative_filters_vs_oracle(ipf, oracle):
+    rng = np.random.default_rng(41)
+    imgs = [rng.integers(0, 256, (300, 517, 3), dtype=np.uint8),
+            (rng.random((257, 411)) * 2 - 1).astype(np.float32),
+            rng.integers(-30000, 30000, (190, 263, 2)).astype(np.int16)]
+    for img in imgs:
+        for b in (0, 1, 3, 4):
+            for dx, dy, k in ((1, 0, 3), (0, 1, 5), (1, 1, 3), (2, 0, 1), (0, 2, 7), (1, 2, 31), (3, 0, 9)):
+                got, want = ipf.sobel(img, dx, dy, k, 0.75, b), oracle.sobel(img, dx, dy, k, 0.75, b)
+                assert np.array_equal(got, want), (img.dtype, b, dx, dy, k)
+            for dx, dy in ((1, 0), (0, 1)):
+                assert np.array_equal(ipf.scharr(img, dx, dy, 2.0, b), oracle.scharr(img, dx, dy, 2.0, b))
+                assert np.array_equal(ipf.prewitt(img, dx, dy, b), oracle.prewitt(img, dx, dy, b))
+            for k in (1, 3, 7):
+                assert np.array_equal(ipf.laplacian(img, k, 0.5, b), oracle.laplacian(img, k, 0.5, b)), (b, k)
+
+
+def test_magnitude_bit_exact_and_phase_within_an_ulp(ipf, oracle):
+    rng = np.random.default_rng(43)
+    n = 1 << 20
+    gx = (rng.standard_normal(n) * 10.0 ** rng.integers(-38, 38, n)).astype(np.float32).reshape(512, 2048)
+    gy = (rng.standard_normal(n) * 10.0 ** rng.integers(-38, 38, n)).astype(np.float32).reshape(512, 2048)
+    gy[0, :1024] = gx[0, :1024] * rng.random(1024).astype(np.float32)  # near-equal magnitudes
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 3.4e38, -3.4e38, 1.0, -1.0], np.float32)
+    sx, sy = np.meshgrid(special, special)
+    gx[1, :sx.size], gy[1, :sy.size] = sx.ravel(), sy.ravel()
+    assert np.array_equal(ipf.magnitude(gx, gy), oracle.magnitude(gx, gy), equal_nan=True)
+    for deg in (True, False):
+        got, want = ipf.phase(gx, gy, deg), oracle.phase(gx, gy, deg)
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        ok = ~np.isnan(want)
+        err = np.abs(got[ok].astype(np.float64) - want[ok])
+        assert np.all(err <= fixtures.phase_tolerance(gx, gy, want, deg)[ok]), err.max()
+        assert np.all((got[ok] >= 0) & (got[ok] <= (360.0 if deg else 2 * np.pi)))
+    img = rng.integers(0, 256, (211, 307, 3), dtype=np.uint8)
+    assert np.array_equal(ipf.gradient_magnitude(img, 5),
+                          oracle.magnitude(oracle.sobel(img, 1, 0, 5), oracle.sobel(img, 0, 1, 5)))
+
+
+def test_sharpening_and_adaptive_threshold_vs_oracle(ipf, oracle):
+    rng = np.random.default_rng(47)
+    imgs = [rng.integers(0, 256, (300, 517, 3), dtype=np.uint8),
+            rng.random((257, 411)).astype(np.float32) * 255,
+            rng.integers(0, 65536, (190, 263), dtype=np.uint16)]
+    for img in imgs:
+        for s, a, th in ((1.0, 1.0, 0.0), (2.2, 0.7, 3.0), (6.0, 1.5, 0.5)):
+            assert np.array_equal(ipf.unsharp_mask(img, s, a, th), oracle.unsharp_mask(img, s, a, th)), (s, a, th)
+        for st in (1.0, 0.3, 2.5):
+            assert np.array_equal(ipf.sharpen(img, st), oracle.sharpen(img, st)), st
+        assert np.array_equal(ipf.emboss(img), oracle.emboss(img))
+        if img.ndim == 2:
+            for m in (0, 1):
+                for tt in (0, 1):
+                    for bs in (3, 11, 31):
+                        got = ipf.adaptive_threshold(img, 177.0, m, tt, bs, -1.5)
+                        assert np.array_equal(got, oracle.adaptive_threshold(img, 177.0, m, tt, bs, -1.5)), (m, tt, bs)
 
 
 def _sha(a):

@@ -32,6 +32,9 @@ def test_oracle_known_answers(oracle):
         assert oracle.border_index(p, n, m) == want, (p, n, m)
     for s, k, want in kat["gauss"]:
         assert np.array_equal(oracle.gaussian_kernel_1d(s, k), np.array(want)), (s, k)
+    for n, a, want in kat["motion"]:
+        got = oracle.motion_blur_kernel(n, a)
+        assert got.size == len(want) and np.array_equal(got.ravel(), np.array(want, np.float32)), (n, a)
 
 
 # ------------------------------------------------ the reference's own test KATs
