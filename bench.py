@@ -1,6 +1,6 @@
 """Benchmark of the MI355X filtering path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|deriv]
                     [--no-cpu] [--cpu-budget S]
 
 Workloads (one step = every call listed once; inputs resident in HBM; enough
@@ -14,6 +14,10 @@ Cache):
                 pepper): medianBlur 3 and 5 (Replicate), bilateralFilter(9, 75, 75).
   c4            4096x4096x1 uint8 and float32 uniform: filter2D with a dense
                 31x31 N(0, 1) kernel normalised to sum 1, Reflect101.
+  deriv         3840x2160x3 uint8 uniform: sobel (dx 3x3, dy 5x5), scharr,
+                laplacian (ksize 1 and 3) with float32 output, unsharpMask,
+                sharpen, emboss; 3840x2160x1 uint8: adaptiveThreshold (mean and
+                Gaussian, block 11).
 All calls use the default precision: FP64 accumulation, certified bit-identical
 to the reference (median is compare-only).
 
@@ -184,7 +188,43 @@ def _c4():
             "keep": kp, "post": accuracy, "kernel_raw_sum": ksum}
 
 
-WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None}
+def _deriv():
+    from imageprocessingflex_amd import workloads as wl
+    rgb = wl.uniform_u8((2160, 3840, 3), seed=7, stream=0)
+    gray = wl.uniform_u8((2160, 3840), seed=7, stream=1)
+    f = np.float32
+    u8 = np.uint8
+    calls = [
+        Call("sobel_dx3", "sobel_dx3", "rgb", f, lambda lib, s, d, st: lib.ipfg_sobel(s, d, 1, 0, 3, 1.0, 3, 0, st),
+             lambda o, im: o.sobel(im, 1, 0, 3), "sep_plane_kernel<unsigned char, float, double, 3, 1, true, 3>"),
+        Call("sobel_dy5", "sobel_dy5", "rgb", f, lambda lib, s, d, st: lib.ipfg_sobel(s, d, 0, 1, 5, 1.0, 3, 0, st),
+             lambda o, im: o.sobel(im, 0, 1, 5), "sep_plane_kernel<unsigned char, float, double, 3, 1, true, 5>"),
+        Call("scharr_dx", "scharr_dx", "rgb", f, lambda lib, s, d, st: lib.ipfg_scharr(s, d, 1, 0, 1.0, 3, 0, st),
+             lambda o, im: o.scharr(im, 1, 0), "sep_plane_kernel<unsigned char, float, double, 3, 1, true, 3>"),
+        Call("laplacian1", "laplacian1", "rgb", f, lambda lib, s, d, st: lib.ipfg_laplacian(s, d, 1, 1.0, 3, 0, st),
+             lambda o, im: o.laplacian(im, 1), "f2d_fast_kernel<unsigned char, double, false, true, 4>"),
+        Call("laplacian3", "laplacian3", "rgb", f, lambda lib, s, d, st: lib.ipfg_laplacian(s, d, 3, 1.0, 3, 0, st),
+             lambda o, im: o.laplacian(im, 3), "two sobel passes + add_f32_kernel"),
+        Call("unsharp_s1", "unsharp_s1", "rgb", u8,
+             lambda lib, s, d, st: lib.ipfg_unsharp_mask(s, d, 1.0, 1.0, 0.0, 0, st),
+             lambda o, im: o.unsharp_mask(im, 1.0, 1.0, 0.0), "FP64 gaussian pass + unsharp_kernel"),
+        Call("sharpen", "sharpen", "rgb", u8, lambda lib, s, d, st: lib.ipfg_sharpen(s, d, 1.0, 0, st),
+             lambda o, im: o.sharpen(im, 1.0), "f2d_fast_kernel<unsigned char, double, false, true, 4>"),
+        Call("emboss", "emboss", "rgb", u8, lambda lib, s, d, st: lib.ipfg_emboss(s, d, 0, st),
+             lambda o, im: o.emboss(im), "f2d_fast_kernel<unsigned char, double, false, true, 4>"),
+        Call("adaptive_mean11", "adaptive_mean11", "gray", u8,
+             lambda lib, s, d, st: lib.ipfg_adaptive_threshold(s, d, 255.0, 0, 0, 11, 2.0, 0, st),
+             lambda o, im: o.adaptive_threshold(im, 255.0, 0, 0, 11, 2.0), "box pass + adaptive_kernel"),
+        Call("adaptive_gauss11", "adaptive_gauss11", "gray", u8,
+             lambda lib, s, d, st: lib.ipfg_adaptive_threshold(s, d, 255.0, 1, 0, 11, 2.0, 0, st),
+             lambda o, im: o.adaptive_threshold(im, 255.0, 1, 0, 11, 2.0), "gaussian pass + adaptive_kernel"),
+    ]
+    return {"images": {"rgb": rgb, "gray": gray}, "calls": calls, "dtype": "f64",
+            "workload": "sobel/scharr/laplacian (float out), unsharpMask/sharpen/emboss on 3840x2160x3 uint8; "
+                        "adaptiveThreshold mean/gaussian 11 on 3840x2160x1 uint8"}
+
+
+WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None, "deriv": _deriv}
 
 C5_BUDGET = 256 << 20  # device-memory budget of the batched server (>= the largest request, 2 x 67 MB)
 C5_MAX_BATCH = 256

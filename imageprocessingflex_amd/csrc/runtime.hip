@@ -434,6 +434,50 @@ double input_bound(int dt, double bval, int border) {
     return x;
 }
 
+bool is_integral_value(double v) { return std::isfinite(v) && std::floor(v) == v; }
+
+// Weights m_i * 2^-frac with integer m_i (frac <= 64); msum = sum |m_i|.
+struct Dyadic {
+    bool ok = false;
+    int frac = 0;
+    double msum = 0.0;
+};
+
+Dyadic dyadic_weights(const double* k, int n) {
+    Dyadic d;
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(k[i])) return d;
+        int f = 0;
+        for (double v = k[i]; v != std::floor(v); v *= 2.0)
+            if (++f > 64) return d;
+        d.frac = std::max(d.frac, f);
+    }
+    for (int i = 0; i < n; ++i) d.msum += std::fabs(std::ldexp(k[i], d.frac));
+    d.ok = true;
+    return d;
+}
+
+// Exactness of a linear filter on an integer image: with integer samples
+// (and an integral Constant value) bounded by X and dyadic weights, every
+// product, partial sum and the delta are integer multiples of 2^-(fx + fy).
+// When X * sum|m| (+ |delta| in those units) stays below 2^52, FP64 holds
+// every intermediate exactly, in any operation order, so the FP64 result is
+// the reference's bit for bit (returns 2); below 2^23 in every pass FP32
+// does too (returns 3). Otherwise 0.
+int exact_sum_level(int src_dt, int border, double bval, double delta, const Dyadic& x, const Dyadic& y) {
+    if (!dtype_is_int(src_dt) || !x.ok || !y.ok) return 0;
+    if (border == IPFG_BORDER_CONSTANT && !is_integral_value(bval)) return 0;
+    const double X = input_bound(src_dt, bval, border);
+    const double unit = std::ldexp(1.0, x.frac + y.frac);
+    const double dm = delta * unit;
+    if (!is_integral_value(dm)) return 0;
+    const double total = X * x.msum * y.msum + std::fabs(dm);
+    const double hx = X * x.msum;
+    if (total <= 8388608.0 && hx <= 8388608.0) return 3;
+    if (total <= 4503599627370496.0 && hx <= 8388608.0) return 2;  // the float ring of uint8 outputs stays exact
+    return 0;
+}
+
 // Kernel family of a linear filter call. DEFAULT accumulates in FP64 and
 // certifies every integer or float32 output (bit-identical to the
 // reference); FP32 trades that for FP32 arithmetic; EXACT and the cases the
@@ -489,6 +533,11 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
     a.delta = delta;
     a.bval = bval;
     a.mode = resolve_mode(precision, src->dtype, dst->dtype);
+    // Exact sums need no certification; exact FP32 sums take the FP32 kernels.
+    const int exact_level = a.mode == MODE_CERTIFY ? exact_sum_level(src->dtype, border, bval, delta,
+                                                                     dyadic_weights(kx, kw), dyadic_weights(ky, kh))
+                                                   : 0;
+    if (exact_level == 3 && sep_fast_fits(src->channels, kw, kh, src->dtype, dst->dtype, false)) a.mode = MODE_FAST;
     if (a.mode != MODE_EXACT &&
         !sep_fast_fits(src->channels, kw, kh, src->dtype, dst->dtype, a.mode == MODE_CERTIFY))
         a.mode = MODE_EXACT;
@@ -526,6 +575,12 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
             a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sx * sy;
         a.xbound = input_bound(src->dtype, bval, border);
+        if (exact_level >= 2) {  // exact sums: only results exactly on a rounding tie are recomputed
+            a.bcoef = 0.0;
+            a.kabs = 0.0;
+            a.bdelta = 1e-300;
+            a.xbound = 1.0;
+        }
         a.fix = sc.fixlist();
     }
     if (a.mode == MODE_EXACT) {
@@ -546,8 +601,6 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
     io.finish();
 }
 
-
-bool is_integral_value(double v) { return std::isfinite(v) && std::floor(v) == v; }
 
 void box_impl(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int normalize, int border, double bval,
               int precision, ipfg_stream stream) {
@@ -723,6 +776,15 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
     a.kd = reinterpret_cast<const double*>(dev + sizeof(double) * tw.size());
     a.kf = reinterpret_cast<const float*>(dev + sizeof(double) * (tw.size() + nk));
     a.taps = reinterpret_cast<const int*>(dev + sizeof(double) * (tw.size() + nk) + sizeof(float) * nk);
+    // Exact sums need no certification; exact FP32 sums take the FP32 kernels.
+    int exact_level = 0;
+    if (a.mode == MODE_CERTIFY) {
+        Dyadic one;
+        one.ok = true;
+        one.msum = 1.0;
+        exact_level = exact_sum_level(src->dtype, border, bval, delta, dyadic_weights(kd, static_cast<int>(nk)), one);
+        if (exact_level == 3) a.mode = MODE_FAST;
+    }
     if (a.mode == MODE_CERTIFY) {
         // FP64 FMA chain and the reference's multiply/add chain each within
         // (taps + 1) u (sum|k| X + |delta|) of the exact sum.
@@ -731,6 +793,11 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
         if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
             a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sk;
         a.xbound = input_bound(src->dtype, bval, border);
+        if (exact_level >= 2) {  // exact sums: only results exactly on a rounding tie are recomputed
+            a.bcoef = 0.0;
+            a.bdelta = 1e-300;
+            a.xbound = 1.0;
+        }
         a.fix = sc.fixlist();
     }
     HIPCALL(launch_filter2d(a, s));

@@ -182,12 +182,13 @@ def test_concurrent_host_threads(ipf, oracle):
 def test_certification_heavy_fixups_sep(ipf, oracle, shape):
     # kernel [0.5, 0.5]: every sample whose 2x2 sum is 2 mod 4 sits exactly on
     # a rounding tie, so about a quarter of the samples are recomputed in the
-    # reference's order inside the launch.
+    # reference's order inside the launch. The fractional Constant value keeps
+    # the call off the exact-integer-sum route (which certifies nothing).
     rng = np.random.default_rng(14)
     img = rng.integers(0, 256, shape, dtype=np.uint8)
     k = np.array([0.5, 0.5])
-    got = ipf.sep_filter2d(img, k, k, 1)
-    assert np.array_equal(got, oracle.sep_filter(img, k, k, border=1))
+    got = ipf.sep_filter2d(img, k, k, 0, 0.0, 0.5)
+    assert np.array_equal(got, oracle.sep_filter(img, k, k, border=0, bval=0.5))
     assert ipf.last_fixups() > img.size // 8
 
 
@@ -196,8 +197,8 @@ def test_certification_heavy_fixups_filter2d(ipf, oracle, shape):
     rng = np.random.default_rng(16)
     img = rng.integers(0, 256, shape, dtype=np.uint8)
     kern = np.array([[0.5, 0.5], [0.5, 0.5]], dtype=np.float32) * 0.5
-    got = ipf.filter2d(img, kern, 2)
-    assert np.array_equal(got, oracle.filter2d(img, kern, border=2))
+    got = ipf.filter2d(img, kern, 0, border_value=0.5)
+    assert np.array_equal(got, oracle.filter2d(img, kern, border=0, bval=0.5))
     assert ipf.last_fixups() > img.size // 8
 
 
@@ -231,16 +232,44 @@ def test_certified_calls_reuse_stream_state(ipf, oracle):
     k = np.array([0.5, 0.5])
     outs = []
     for i in range(6):
-        outs.append(ipf.sep_filter2d(d, k, k, 1, stream=s) if i % 2 == 0 else ipf.gaussian_blur(d, 1.3, stream=s))
+        outs.append(ipf.sep_filter2d(d, k, k, 0, 0.0, 0.5, stream=s) if i % 2 == 0
+                    else ipf.gaussian_blur(d, 1.3, stream=s))
     s.synchronize()
     for i, o in enumerate(outs):
-        want = oracle.sep_filter(img, k, k, border=1) if i % 2 == 0 else oracle.gaussian_blur(img, 1.3)
+        want = oracle.sep_filter(img, k, k, border=0, bval=0.5) if i % 2 == 0 else oracle.gaussian_blur(img, 1.3)
         assert np.array_equal(o.download(s), want), i
 
 
 # ------------------------------------------------------------- full-size runs
 FULL = json.loads((fixtures.GOLDEN / "fullsize.json").read_text())
 BNAME = {0: "constant", 1: "replicate", 2: "reflect", 3: "reflect101", 4: "wrap"}
+
+
+# ------------------------------------------------ exact integer (dyadic) sums
+def test_dyadic_kernels_on_integer_images_bit_exact(ipf, oracle):
+    # integer images with weights m * 2^-f have exact sums in FP64 (and, when
+    # small, in FP32): those calls skip certification and must still match
+    rng = np.random.default_rng(53)
+    imgs = [rng.integers(0, 256, (203, 311, 3), dtype=np.uint8),
+            rng.integers(0, 65536, (170, 250), dtype=np.uint16),
+            rng.integers(-32768, 32768, (150, 190, 4)).astype(np.int16)]
+    seps = [([0.25, 0.5, 0.25], [0.25, 0.5, 0.25], 0.0), ([-1, 0, 1], [1, 2, 1], 0.5),
+            ([1, 4, 6, 4, 1], [0.0625, 0.25, 0.375, 0.25, 0.0625], -3.0), ([3, -10.5, 3], [1, 2, 1], 1.25),
+            ([2.0 ** 20, 1, -2.0 ** 20], [1, 2, 1], 0.0)]
+    for img in imgs:
+        for to in (None, np.float32):
+            for kx, ky, delta in seps:
+                for b, bval in ((0, 7.0), (0, 2.5), (1, 0.0), (3, 0.0), (4, 0.0)):
+                    got = ipf.sep_filter2d(img, kx, ky, b, delta, bval, to)
+                    want = oracle.sep_filter(img, kx, ky, None, b, delta, bval, to)
+                    assert np.array_equal(got, want), (img.dtype, to, kx, ky, delta, b, bval)
+        for k in (np.array([[0, 1, 0], [1, -4, 1], [0, 1, 0]], np.float32),
+                  np.array([[0.5, 0, -0.5], [0.25, 0.125, 0.25], [-1, 0, 1], [2, 0, 2]], np.float32)):
+            for to in (None, np.float32):
+                for b, bval in ((0, 3.0), (3, 0.0), (2, 0.0)):
+                    got = ipf.filter2d(img, k, b, 0.75, (-1, -1), bval, to)
+                    want = oracle.filter2d(img, k, b, 0.75, (-1, -1), bval, to)
+                    assert np.array_equal(got, want), (img.dtype, to, k.shape, b)
 
 
 # ------------------------------------------- derivative and sharpening filters
