@@ -65,6 +65,11 @@ struct F2dLaunch {
     double mfma_wscale;   // power of two the fragments' weights were scaled by
 };
 hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s);
+// 3x3 / 5x5 kernels with a centred anchor on uint8 images (uint8 or float32
+// out) whose sums are exact in FP32; k is row-major on the host.
+// hipErrorNotSupported when the call does not fit (types, sizes, alignment).
+hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const float* k, int kw, int kh, int ay,
+                             int border, double delta, double bval, hipStream_t s);
 // MFMA implicit-GEMM filter2D (precision BF16X3 / FP16X2).
 hipError_t launch_filter2d_mfma(const F2dLaunch& a, hipStream_t s);
 size_t f2d_mfma_lds_bytes(int kw, int kh, bool split_input);

@@ -399,11 +399,12 @@ private:
             d.stride = im.row_stride_bytes ? im.row_stride_bytes : row;
             return d;
         }
-        d.stride = row;
-        d.data = sc_.alloc(row * static_cast<size_t>(im.height));
+        d.stride = (row + 255) & ~static_cast<size_t>(255);  // aligned rows for the vector loads and stores
+        d.data = sc_.alloc(d.stride * static_cast<size_t>(im.height));
         if (copy_in && im.height > 0 && row > 0) {
             const size_t hs = im.row_stride_bytes ? im.row_stride_bytes : row;
-            HIPCALL(hipMemcpy2DAsync(d.data, row, im.data, hs, row, im.height, hipMemcpyHostToDevice, sc_.stream()));
+            HIPCALL(hipMemcpy2DAsync(d.data, d.stride, im.data, hs, row, im.height, hipMemcpyHostToDevice,
+                                     sc_.stream()));
         }
         return d;
     }
@@ -595,7 +596,18 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
                                 return launch_sep(t, s);
                             });
     }
-    HIPCALL(launch_sep(a, s));
+    hipError_t e = hipErrorNotSupported;
+    if (exact_level == 3 && a.mode == MODE_FAST && kw == kh && ax == kw / 2 && ay == kh / 2 && (kw == 3 || kw == 5)) {
+        float k2[25];  // kx[i] * ky[j]: exact in FP32 at this level
+        for (int j = 0; j < kh; ++j)
+            for (int i = 0; i < kw; ++i) k2[j * kw + i] = static_cast<float>(ky[j] * kx[i]);
+        e = launch_stencil_u8(a.src, a.dst, k2, kw, kh, ay, border, delta, bval, s);
+    }
+    if (e == hipErrorNotSupported) {
+        (void)hipGetLastError();
+        e = launch_sep(a, s);
+    }
+    HIPCALL(e);
     if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
     io.finish();
@@ -800,7 +812,14 @@ void f2d_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel,
         }
         a.fix = sc.fixlist();
     }
-    HIPCALL(launch_filter2d(a, s));
+    hipError_t e = hipErrorNotSupported;
+    if (exact_level == 3 && ax == kw / 2 && ay == kh / 2)
+        e = launch_stencil_u8(a.src, a.dst, kernel, kw, kh, ay, border, delta, bval, s);
+    if (e == hipErrorNotSupported) {
+        (void)hipGetLastError();
+        e = launch_filter2d(a, s);
+    }
+    HIPCALL(e);
     if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
     io.finish();
@@ -1100,12 +1119,34 @@ void laplacian_impl(const ipfg_image* src, const ipfg_image* dst, int ksize, dou
         f2d_impl(src, dst, k, 3, 3, 1, 1, border, 0.0, 0.0, precision, stream);
         return;
     }
-    (void)deriv_kernel(2, ksize);  // the argument errors of sobel(src, 2, 0, ksize), before any device work
+    const std::vector<double> d2 = deriv_kernel(2, ksize);  // sobel's argument errors, before any device work
+    const std::vector<double> d0 = deriv_kernel(0, ksize);
     ensure_init();
     if (static_cast<long long>(src->width) * src->height == 0) return;
     const hipStream_t s = pick_stream(stream);
     CallScratch sc(s);
     Staged io(src, dst, sc);
+    // When both second derivatives are exact float sums, so is their float
+    // sum: one stencil with the combined kernel gives the same bits.
+    std::vector<double> d2s = d2, d0s = d0;
+    for (double& v : d2s) v *= scale;
+    for (double& v : d0s) v *= scale;
+    const int n = static_cast<int>(d2.size());
+    if (precision != IPFG_PREC_EXACT && n <= 5 &&
+        exact_sum_level(src->dtype, border, 0.0, 0.0, dyadic_weights(d2s.data(), n), dyadic_weights(d0.data(), n)) == 3 &&
+        exact_sum_level(src->dtype, border, 0.0, 0.0, dyadic_weights(d0s.data(), n), dyadic_weights(d2.data(), n)) == 3) {
+        float k2[25];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) k2[j * n + i] = static_cast<float>(d0[j] * d2s[i] + d2[j] * d0s[i]);
+        const hipError_t e = launch_stencil_u8(io.src(), io.dst(), k2, n, n, n / 2, border, 0.0, 0.0, s);
+        if (e != hipErrorNotSupported) {
+            HIPCALL(e);
+            t_ctx.fix_valid = false;
+            io.finish();
+            return;
+        }
+        (void)hipGetLastError();
+    }
     const ipfg_image ds = dev_view(io.src()), dd = dev_view(io.dst());
     const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F32);
     const ipfg_image db = dev_view(b);

@@ -272,6 +272,36 @@ def test_dyadic_kernels_on_integer_images_bit_exact(ipf, oracle):
                     assert np.array_equal(got, want), (img.dtype, to, k.shape, b)
 
 
+@pytest.mark.parametrize("shape", [(150, 256, 3), (97, 64), (70, 128, 4), (5, 48, 3), (33, 16), (1, 32, 4),
+                                   (2, 1024, 1), (64, 80, 3)], ids=lambda s: "x".join(map(str, s)))
+def test_small_exact_stencils_vs_oracle(ipf, oracle, shape):
+    # 3x3 / 5x5 kernels with exact FP32 sums on uint8 rows that are whole
+    # 16-byte groups take the register-window stencil: interior groups, image
+    # edges and every border mode against the reference order
+    rng = np.random.default_rng(sum(shape) * 7)
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
+    ks = [np.array([[1, 2, 1], [2, 4, 2], [1, 2, 1]], np.float32) / 16,
+          np.array([[0, -1, 0], [-1, 5, -1], [0, -1, 0]], np.float32),
+          np.arange(25, dtype=np.float32).reshape(5, 5) - 12.0,
+          np.ones((5, 5), np.float32) / 32]
+    for k in ks:
+        for to in (None, np.float32):
+            for b, bval in ((0, 9.0), (1, 0.0), (2, 0.0), (3, 0.0), (4, 0.0)):
+                got = ipf.filter2d(img, k, b, -2.5, (-1, -1), bval, to)
+                assert np.array_equal(got, oracle.filter2d(img, k, b, -2.5, (-1, -1), bval, to)), (k.shape, to, b)
+        # an off-centre anchor and a fractional delta at the FP32 limit stay on the other kernels
+        got = ipf.filter2d(img, k, 3, 0.0, (0, 1))
+        assert np.array_equal(got, oracle.filter2d(img, k, 3, 0.0, (0, 1)))
+    for dx, dy, kk in ((1, 0, 3), (0, 1, 5), (2, 0, 1), (1, 1, 3), (0, 2, 5)):
+        assert np.array_equal(ipf.sobel(img, dx, dy, kk, 0.5, 4), oracle.sobel(img, dx, dy, kk, 0.5, 4))
+    for kk in (3, 5):
+        for b in (0, 3):
+            assert np.array_equal(ipf.laplacian(img, kk, 1.5, b), oracle.laplacian(img, kk, 1.5, b)), (kk, b)
+    d = ipf.DeviceImage.from_array(img)
+    out = ipf.sharpen(d, 0.75)
+    assert np.array_equal(out.download().reshape(img.shape), oracle.sharpen(img, 0.75))
+
+
 # ------------------------------------------- derivative and sharpening filters
 def test_derivative_filters_vs_oracle(ipf, oracle):
     rng = np.random.default_rng(41)
