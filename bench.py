@@ -21,11 +21,13 @@ Cache):
 All calls use the default precision: FP64 accumulation, certified bit-identical
 to the reference (median is compare-only).
 
-Prints ONE JSON line: value = Gpix/s over the whole step, the roofline of the
-dominant kernel (algorithmic bytes per launch / mean launch time from HIP
-events on the launch stream), per-filter Gpix/s and roofline fraction, call
-latency percentiles, and the reference's multithreaded CPU path timed on a
-bounded sample of the same calls on this host.
+Prints ONE JSON line: value = Gpix/s over the whole step (the step's calls
+back to back, wall clock between two device synchronisations), the roofline
+of the dominant kernel (algorithmic bytes per launch / mean launch time from
+HIP events around 4 back-to-back launches on the launch stream), per-filter
+Gpix/s and roofline fraction, the latency of one call on an idle stream, and
+the reference's multithreaded CPU path timed on a bounded sample of the same
+calls on this host.
 """
 from __future__ import annotations
 
@@ -508,26 +510,45 @@ def main():
         ipf._lib.check(c.launch(lib, C.byref(s), C.byref(d), stream.handle))
         k += 1
 
-    per_call = {c.name: [] for c in calls}
     for _ in range(args.warmup):
         for c in calls:
             run(c)
     stream.synchronize()
     ipf.synchronize()
+    # timed region: the step's calls back to back, nothing recorded between them
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        for i, c in enumerate(calls):
-            ev.record(evs[2 * i], stream.handle)
+        for c in calls:
             run(c)
-            ev.record(evs[2 * i + 1], stream.handle)
-        for i, c in enumerate(calls):
-            per_call[c.name].append(ev.ms(evs[2 * i], evs[2 * i + 1]))
     stream.synchronize()
     ipf.synchronize()
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1e3 / args.steps
     step_px = sum(wd["images"][c.image].shape[0] * wd["images"][c.image].shape[1] for c in calls)
     value = step_px / (ms_per_step * 1e-3) / 1e9
+
+    # launch durations: REP launches of each call (rotating sets) between one
+    # pair of HIP events on the launch stream, repeated `steps` times
+    REP = 4
+    per_call = {c.name: [] for c in calls}
+    for _ in range(args.steps):
+        for i, c in enumerate(calls):
+            ev.record(evs[2 * i], stream.handle)
+            for _r in range(REP):
+                run(c)
+            ev.record(evs[2 * i + 1], stream.handle)
+        for i, c in enumerate(calls):
+            per_call[c.name].append(ev.ms(evs[2 * i], evs[2 * i + 1]) / REP)
+    # call latency: one call between a pair of events, the device idle before it
+    lat = []
+    for _ in range(args.steps):
+        for i, c in enumerate(calls):
+            stream.synchronize()
+            ev.record(evs[0], stream.handle)
+            run(c)
+            ev.record(evs[1], stream.handle)
+            lat.append(ev.ms(evs[0], evs[1]))
+    stream.synchronize()
 
     per_filter = {}
     for c in calls:
@@ -544,7 +565,7 @@ def main():
         if g["flops"]:
             summary[name]["tflop_s"] = round(g["flops"] / (mean * 1e-3) / 1e12, 2)
     dom = max(summary, key=lambda n: summary[n]["mean_ms"])
-    all_ms = sorted(m for v in per_call.values() for m in v)
+    all_ms = sorted(lat)
     p50 = all_ms[len(all_ms) // 2]
     p99 = all_ms[min(len(all_ms) - 1, int(round(0.99 * (len(all_ms) - 1))))]
     achieved = summary[dom]["gb_s"]
@@ -605,7 +626,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "per_filter": summary,
-        "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4), "what": "single filter call, HIP events"},
+        "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4),
+                       "what": "one filter call on an idle stream, HIP events around the call"},
+        "launch_timing": f"mean of {REP} back-to-back launches between two HIP events, x {args.steps}",
         "device_copy": copy,
         **extra,
     }

@@ -19,6 +19,7 @@
 
 #include "batch.hpp"
 #include "kernels.hpp"
+#include "sep_kernels.hpp"
 #include "tuner.hpp"
 
 namespace ipfg {
@@ -1032,15 +1033,101 @@ __global__ __launch_bounds__(kBlock) void unsharp_kernel(DevImage src, DevImage 
     store_sample(dst.data, od, dst.dt, fabs(diff) > threshold ? v + amount * diff : v);
 }
 
-// adaptiveThreshold's decision (filter.hpp:559-563).
-__global__ __launch_bounds__(kBlock) void adaptive_kernel(DevImage src, DevImage mean, DevImage dst, double maxval,
-                                                          double c, int binary) {
-    const size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
-    size_t os, om, od;
-    if (!sample_at(src, i, &os) || !sample_at(mean, i, &om) || !sample_at(dst, i, &od)) return;
-    const bool above = load_sample(src.data, os, src.dt) - load_sample(mean.data, om, mean.dt) > -c;
-    store_sample(dst.data, od, dst.dt, (above == (binary != 0)) ? maxval : 0.0);
+// Row-wise elementwise launches: blockIdx.y walks rows (grid-stride), each
+// thread takes 4 consecutive samples of the row.
+constexpr int kEltPer = 4;
+
+dim3 rows_grid(const DevImage& d) {
+    const long long L = static_cast<long long>(d.w) * d.cn;
+    const unsigned gx = static_cast<unsigned>((L + kBlock * kEltPer - 1) / (kBlock * kEltPer));
+    return dim3(gx ? gx : 1, static_cast<unsigned>(std::min(d.h, 65535)));
 }
+
+// unsharpMask's combine (filter.hpp:476-482) from an FP32 blur b32 with
+// |b32 - blurred| <= bnd, for integer images. Every diff in
+// [v - b32 - bnd', v - b32 + bnd'] is possible (bnd' adds the FP64 rounding of
+// the difference); when the threshold branch and the rounded result agree
+// over that interval, the output is the reference's. Otherwise the wave
+// recomputes the sample's blur in the reference's order (sep_exact_wave:
+// gaussianBlur<double>, Reflect101), all 64 lanes on one sample at a time.
+template <class T>
+__global__ __launch_bounds__(kBlock) void unsharp_cert_kernel(DevImage src, DevImage b32, DevImage dst, double amount,
+                                                              double threshold, double bnd,
+                                                              const double* __restrict__ k, int ks) {
+    const int L = src.w * src.cn;
+    const int s0 = (blockIdx.x * kBlock + threadIdx.x) * kEltPer;
+    const bool active = s0 < L;  // every lane stays for the wave-wide recomputations
+    for (int y = blockIdx.y; y < src.h; y += gridDim.y) {
+        const T* srow = reinterpret_cast<const T*>(static_cast<const char*>(src.data) + static_cast<size_t>(y) * src.stride);
+        const float* brow = reinterpret_cast<const float*>(static_cast<const char*>(b32.data) + static_cast<size_t>(y) * b32.stride);
+        T* drow = reinterpret_cast<T*>(static_cast<char*>(dst.data) + static_cast<size_t>(y) * dst.stride);
+        unsigned doubt = 0u;
+#pragma unroll
+        for (int e = 0; e < kEltPer; ++e) {
+            const int sidx = s0 + e;
+            if (!active || sidx >= L) break;
+            const double v = static_cast<double>(srow[sidx]);
+            const double d = v - static_cast<double>(brow[sidx]);
+            const double w = bnd + 2.3e-16 * fabs(d);
+            const double lo = d - w, hi = d + w;
+            const double mina = (lo <= 0.0 && hi >= 0.0) ? 0.0 : fmin(fabs(lo), fabs(hi));
+            const double maxa = fmax(fabs(lo), fabs(hi));
+            const bool may_keep = !(mina > threshold), may_sharpen = maxa > threshold;
+            T out = saturate_from_double<T>(v);
+            bool sure = true;
+            if (may_sharpen) {
+                const double r1 = v + amount * lo, r2 = v + amount * hi;
+                const double eps = 4.5e-16 * (fabs(v) + fabs(amount) * maxa) + 1e-300;
+                const T o1 = saturate_from_double<T>(fmin(r1, r2) - eps);
+                const T o2 = saturate_from_double<T>(fmax(r1, r2) + eps);
+                sure = o1 == o2 && (!may_keep || o1 == out);
+                out = o1;
+            }
+            if (sure) drow[sidx] = out;
+            else doubt |= 1u << e;
+        }
+        unsigned long long bal = __ballot(doubt != 0u);
+        while (bal) {
+            const int l = __builtin_ctzll(bal);
+            bal &= bal - 1ull;
+            unsigned mm = static_cast<unsigned>(bcast_i(static_cast<int>(doubt), l));
+            const int ls0 = bcast_i(s0, l);
+            while (mm) {
+                const int e = __builtin_ctz(mm);
+                mm &= mm - 1u;
+                const int sidx = ls0 + e, px = sidx / src.cn, c = sidx - px * src.cn;
+                const double b = sepk::sep_exact_wave<T>(src, px, y, c, k, ks, k, ks, ks / 2, ks / 2,
+                                                         IPFG_BORDER_REFLECT101, 0.0, 0.0);
+                if (lane_id() == l) {
+                    const double v = static_cast<double>(srow[sidx]);
+                    const double diff = v - b;
+                    drow[sidx] = saturate_from_double<T>(fabs(diff) > threshold ? v + amount * diff : v);
+                }
+            }
+        }
+    }
+}
+
+// adaptiveThreshold's decision (filter.hpp:559-563) on row-wise launches.
+template <class T>
+__global__ __launch_bounds__(kBlock) void adaptive_rows_kernel(DevImage src, DevImage mean, DevImage dst, T hi, T lo,
+                                                               double c) {
+    const int L = src.w * src.cn;
+    const int s0 = (blockIdx.x * kBlock + threadIdx.x) * kEltPer;
+    if (s0 >= L) return;
+    for (int y = blockIdx.y; y < src.h; y += gridDim.y) {
+        const T* srow = reinterpret_cast<const T*>(static_cast<const char*>(src.data) + static_cast<size_t>(y) * src.stride);
+        const T* mrow = reinterpret_cast<const T*>(static_cast<const char*>(mean.data) + static_cast<size_t>(y) * mean.stride);
+        T* drow = reinterpret_cast<T*>(static_cast<char*>(dst.data) + static_cast<size_t>(y) * dst.stride);
+#pragma unroll
+        for (int e = 0; e < kEltPer; ++e) {
+            if (s0 + e >= L) break;
+            const bool above = static_cast<double>(srow[s0 + e]) - static_cast<double>(mrow[s0 + e]) > -c;
+            drow[s0 + e] = above ? hi : lo;
+        }
+    }
+}
+
 
 // magnitude / phase of two float images (filter.hpp:438-458).
 // magnitude: sqrt(x^2 + y^2) evaluated in FP64 (the squares and their sum of
@@ -1200,9 +1287,47 @@ void unsharp_impl(const ipfg_image* src, const ipfg_image* dst, double sigma, do
     CallScratch sc(s);
     Staged io(src, dst, sc);
     const ipfg_image ds = dev_view(io.src());
+    const int ks = static_cast<int>(k.size());
+    const int dt = src->dtype;
+    if (precision != IPFG_PREC_EXACT && (dt == IPFG_U8 || dt == IPFG_U16 || dt == IPFG_I16)) {
+        // FP32 blur, then the certified combine (recomputing the rare samples
+        // whose result the FP32 error could change). FP32 chains of kw + kh
+        // taps with weights rounded to float stay within
+        // (2 ks + 4) 2^-24 (sum|k|)^2 X of the exact blur, as does the
+        // reference's FP64 blur (with room to spare); X = the type's bound.
+        const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F32);
+        const ipfg_image db = dev_view(b);
+        sep_impl(&ds, &db, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0,
+                 IPFG_PREC_FP32, reinterpret_cast<ipfg_stream>(s));
+        double sk = 0.0;
+        for (double v : k) sk += std::fabs(v);
+        const double X = input_bound(dt, 0.0, IPFG_BORDER_REFLECT101);
+        const double bnd = 1.02 * ((2.0 * ks + 6) * 5.9604644775390625e-08 * sk * sk * X + 1e-12 * X);
+        std::vector<char> blob(sizeof(double) * k.size());
+        std::memcpy(blob.data(), k.data(), blob.size());
+        const double* kd = static_cast<const double*>(sc.params(blob));
+        const dim3 grid = rows_grid(io.dst());
+        switch (dt) {
+            case IPFG_U8:
+                hipLaunchKernelGGL(unsharp_cert_kernel<uint8_t>, grid, dim3(kBlock), 0, s, io.src(), b, io.dst(), amount,
+                                   threshold, bnd, kd, ks);
+                break;
+            case IPFG_U16:
+                hipLaunchKernelGGL(unsharp_cert_kernel<uint16_t>, grid, dim3(kBlock), 0, s, io.src(), b, io.dst(),
+                                   amount, threshold, bnd, kd, ks);
+                break;
+            default:
+                hipLaunchKernelGGL(unsharp_cert_kernel<int16_t>, grid, dim3(kBlock), 0, s, io.src(), b, io.dst(), amount,
+                                   threshold, bnd, kd, ks);
+                break;
+        }
+        HIPCALL(hipGetLastError());
+        t_ctx.fix_valid = false;
+        io.finish();
+        return;
+    }
     const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F64);
     const ipfg_image db = dev_view(b);
-    const int ks = static_cast<int>(k.size());
     sep_impl(&ds, &db, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0, precision,
              reinterpret_cast<ipfg_stream>(s));
     hipLaunchKernelGGL(unsharp_kernel, dim3(elementwise_grid(io.dst())), dim3(kBlock), 0, s, io.src(), b, io.dst(),
@@ -1251,8 +1376,35 @@ void adaptive_impl(const ipfg_image* src, const ipfg_image* dst, double maxval, 
         sep_impl(&ds, &dm, k.data(), block, k.data(), block, block / 2, block / 2, IPFG_BORDER_REPLICATE, 0.0, 0.0,
                  precision, st);
     }
-    hipLaunchKernelGGL(adaptive_kernel, dim3(elementwise_grid(io.dst())), dim3(kBlock), 0, s, io.src(), m, io.dst(),
-                       maxval, c, type == 0 ? 1 : 0);
+    // the two possible outputs, saturated like the reference's per-sample cast
+    const double vhi = type == 0 ? maxval : 0.0, vlo = type == 0 ? 0.0 : maxval;
+    const dim3 grid = rows_grid(io.dst());
+    switch (src->dtype) {
+        case IPFG_U8:
+            hipLaunchKernelGGL(adaptive_rows_kernel<uint8_t>, grid, dim3(kBlock), 0, s, io.src(), m, io.dst(),
+                               saturate_from_double<uint8_t>(vhi), saturate_from_double<uint8_t>(vlo), c);
+            break;
+        case IPFG_U16:
+            hipLaunchKernelGGL(adaptive_rows_kernel<uint16_t>, grid, dim3(kBlock), 0, s, io.src(), m, io.dst(),
+                               saturate_from_double<uint16_t>(vhi), saturate_from_double<uint16_t>(vlo), c);
+            break;
+        case IPFG_I16:
+            hipLaunchKernelGGL(adaptive_rows_kernel<int16_t>, grid, dim3(kBlock), 0, s, io.src(), m, io.dst(),
+                               saturate_from_double<int16_t>(vhi), saturate_from_double<int16_t>(vlo), c);
+            break;
+        case IPFG_I32:
+            hipLaunchKernelGGL(adaptive_rows_kernel<int32_t>, grid, dim3(kBlock), 0, s, io.src(), m, io.dst(),
+                               saturate_from_double<int32_t>(vhi), saturate_from_double<int32_t>(vlo), c);
+            break;
+        case IPFG_F32:
+            hipLaunchKernelGGL(adaptive_rows_kernel<float>, grid, dim3(kBlock), 0, s, io.src(), m, io.dst(),
+                               static_cast<float>(vhi), static_cast<float>(vlo), c);
+            break;
+        default:
+            hipLaunchKernelGGL(adaptive_rows_kernel<double>, grid, dim3(kBlock), 0, s, io.src(), m, io.dst(), vhi, vlo,
+                               c);
+            break;
+    }
     HIPCALL(hipGetLastError());
     t_ctx.fix_valid = false;
     io.finish();
