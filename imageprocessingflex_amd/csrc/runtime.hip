@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <list>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -298,21 +299,99 @@ void forget_stream(hipStream_t s) {
     g_fix.erase(it);
 }
 
+// ------------------------------------------------------------------ scratch
+// Cache of device blocks for per-call scratch (staging copies, intermediate
+// images). A block returned by a call records an event on that call's stream;
+// the next call to take it makes its own stream wait for that event, so reuse
+// is ordered on any stream without the host waiting. Allocation and release
+// through the stream-ordered allocator made the host wait for the device on
+// every call.
+class ScratchPool {
+public:
+    struct Block {
+        void* ptr;
+        size_t size;
+        hipEvent_t ev;
+    };
+    static constexpr size_t kGranule = size_t(2) << 20;
+    static constexpr size_t kKeepBytes = size_t(8) << 30;  // cached beyond this is freed
+
+    Block take(size_t bytes, hipStream_t s) {
+        const size_t need = (std::max<size_t>(bytes, 1) + kGranule - 1) / kGranule * kGranule;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            auto it = free_.lower_bound(need);
+            if (it != free_.end() && it->first <= 2 * need) {
+                Block b = it->second;
+                free_.erase(it);
+                cached_ -= b.size;
+                HIPCALL(hipStreamWaitEvent(s, b.ev, 0));
+                return b;
+            }
+        }
+        Block b{nullptr, need, nullptr};
+        HIPCALL(hipMalloc(&b.ptr, need));
+        HIPCALL(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
+        return b;
+    }
+
+    void give(const Block& b, hipStream_t s) {
+        if (hipEventRecord(b.ev, s) != hipSuccess) {  // cannot order a reuse: drop the block
+            (void)hipStreamSynchronize(s);
+            (void)hipEventDestroy(b.ev);
+            (void)hipFree(b.ptr);
+            return;
+        }
+        std::vector<Block> drop;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            free_.emplace(b.size, b);
+            cached_ += b.size;
+            while (cached_ > kKeepBytes && !free_.empty()) {
+                auto big = std::prev(free_.end());
+                drop.push_back(big->second);
+                cached_ -= big->first;
+                free_.erase(big);
+            }
+        }
+        for (const Block& d : drop) release(d);
+    }
+
+    void clear() {
+        std::multimap<size_t, Block> all;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            all.swap(free_);
+            cached_ = 0;
+        }
+        for (auto& kv : all) release(kv.second);
+    }
+
+private:
+    static void release(const Block& b) {
+        (void)hipEventSynchronize(b.ev);
+        (void)hipEventDestroy(b.ev);
+        (void)hipFree(b.ptr);
+    }
+    std::mutex mu_;
+    std::multimap<size_t, Block> free_;
+    size_t cached_ = 0;
+};
+ScratchPool g_scratch;
+
 // ------------------------------------------------------------------ per call
-// Device memory for one call, allocated and freed in stream order, plus the
-// cached parameter blocks it reads.
+// Device memory for one call (from the scratch cache, ordered on the call's
+// stream) plus the cached parameter blocks it reads.
 class CallScratch {
 public:
     explicit CallScratch(hipStream_t s) : s_(s) {}
     ~CallScratch() {
         for (auto* e : held_) g_params.release(e, s_);
-        for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
+        for (const auto& b : blocks_) g_scratch.give(b, s_);
     }
     void* alloc(size_t bytes) {
-        void* p = nullptr;
-        HIPCALL(hipMallocAsync(&p, bytes ? bytes : 16, s_));
-        ptrs_.push_back(p);
-        return p;
+        blocks_.push_back(g_scratch.take(bytes, s_));
+        return blocks_.back().ptr;
     }
     // Device copy of a parameter block (cached across calls).
     void* params(const std::vector<char>& blob) {
@@ -334,7 +413,7 @@ public:
 
 private:
     hipStream_t s_;
-    std::vector<void*> ptrs_;
+    std::vector<ScratchPool::Block> blocks_;
     std::vector<ParamCache::Entry*> held_;
 };
 
@@ -1952,7 +2031,10 @@ int ipfg_init(int device) {
 
 int ipfg_shutdown(void) {
     return guarded([&] {
-        if (g_device.load() >= 0) HIPCALL(hipDeviceSynchronize());
+        if (g_device.load() >= 0) {
+            HIPCALL(hipDeviceSynchronize());
+            g_scratch.clear();
+        }
     });
 }
 
