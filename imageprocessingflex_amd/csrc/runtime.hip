@@ -1848,6 +1848,7 @@ private:
         size_t need = 0;
         void* dsrc = nullptr;
         void* ddst = nullptr;
+        ScratchPool::Block bsrc{}, bdst{};
     };
     struct Lane {
         hipStream_t stream = nullptr;
@@ -1933,7 +1934,8 @@ private:
         for (Req& q : reqs) {
             ipfg_job j = q.job;
             if (!j.src.on_device) {
-                HIPCALL(hipMallocAsync(&q.dsrc, image_bytes(j.src), L.stream));
+                q.bsrc = g_scratch.take(image_bytes(j.src), L.stream);
+                q.dsrc = q.bsrc.ptr;
                 HIPCALL(hipMemcpy2DAsync(q.dsrc, packed_stride(j.src), j.src.data, host_stride(j.src), packed_stride(j.src),
                                          j.src.height, hipMemcpyHostToDevice, L.stream));
                 j.src.data = q.dsrc;
@@ -1941,7 +1943,8 @@ private:
                 j.src.on_device = 1;
             }
             if (!j.dst.on_device || sink_output(j)) {
-                HIPCALL(hipMallocAsync(&q.ddst, image_bytes(j.dst), L.stream));
+                q.bdst = g_scratch.take(image_bytes(j.dst), L.stream);
+                q.ddst = q.bdst.ptr;
                 j.dst.data = q.ddst;
                 j.dst.row_stride_bytes = packed_stride(q.job.dst);
                 j.dst.on_device = 1;
@@ -1953,8 +1956,8 @@ private:
             if (q.ddst && !q.job.dst.on_device)
                 HIPCALL(hipMemcpy2DAsync(q.job.dst.data, host_stride(q.job.dst), q.ddst, packed_stride(q.job.dst),
                                          packed_stride(q.job.dst), q.job.dst.height, hipMemcpyDeviceToHost, L.stream));
-            if (q.dsrc) HIPCALL(hipFreeAsync(q.dsrc, L.stream));
-            if (q.ddst) HIPCALL(hipFreeAsync(q.ddst, L.stream));
+            if (q.dsrc) g_scratch.give(q.bsrc, L.stream);
+            if (q.ddst) g_scratch.give(q.bdst, L.stream);
         }
         HIPCALL(hipEventRecord(L.done, L.stream));
         L.reqs = std::move(reqs);
