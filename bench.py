@@ -198,28 +198,31 @@ def _deriv():
     u8 = np.uint8
     calls = [
         Call("sobel_dx3", "sobel_dx3", "rgb", f, lambda lib, s, d, st: lib.ipfg_sobel(s, d, 1, 0, 3, 1.0, 3, 0, st),
-             lambda o, im: o.sobel(im, 1, 0, 3), "sep_plane_kernel<unsigned char, float, double, 3, 1, true, 3>"),
+             lambda o, im: o.sobel(im, 1, 0, 3), "stencil_u8_kernel<float, 3, 3, 3>"),
         Call("sobel_dy5", "sobel_dy5", "rgb", f, lambda lib, s, d, st: lib.ipfg_sobel(s, d, 0, 1, 5, 1.0, 3, 0, st),
-             lambda o, im: o.sobel(im, 0, 1, 5), "sep_plane_kernel<unsigned char, float, double, 3, 1, true, 5>"),
+             lambda o, im: o.sobel(im, 0, 1, 5), "stencil_u8_kernel<float, 5, 5, 3>"),
         Call("scharr_dx", "scharr_dx", "rgb", f, lambda lib, s, d, st: lib.ipfg_scharr(s, d, 1, 0, 1.0, 3, 0, st),
-             lambda o, im: o.scharr(im, 1, 0), "sep_plane_kernel<unsigned char, float, double, 3, 1, true, 3>"),
+             lambda o, im: o.scharr(im, 1, 0), "stencil_u8_kernel<float, 3, 3, 3>"),
         Call("laplacian1", "laplacian1", "rgb", f, lambda lib, s, d, st: lib.ipfg_laplacian(s, d, 1, 1.0, 3, 0, st),
-             lambda o, im: o.laplacian(im, 1), "f2d_fast_kernel<unsigned char, double, false, true, 4>"),
+             lambda o, im: o.laplacian(im, 1), "stencil_u8_kernel<float, 3, 3, 3>"),
         Call("laplacian3", "laplacian3", "rgb", f, lambda lib, s, d, st: lib.ipfg_laplacian(s, d, 3, 1.0, 3, 0, st),
-             lambda o, im: o.laplacian(im, 3), "two sobel passes + add_f32_kernel"),
+             lambda o, im: o.laplacian(im, 3), "stencil_u8_kernel<float, 3, 3, 3> (combined kernel)"),
         Call("unsharp_s1", "unsharp_s1", "rgb", u8,
              lambda lib, s, d, st: lib.ipfg_unsharp_mask(s, d, 1.0, 1.0, 0.0, 0, st),
-             lambda o, im: o.unsharp_mask(im, 1.0, 1.0, 0.0), "FP64 gaussian pass + unsharp_kernel"),
+             lambda o, im: o.unsharp_mask(im, 1.0, 1.0, 0.0),
+             "sep_plane_kernel<unsigned char, float, float, float, 3, 8, false, 0> + unsharp_cert_kernel<unsigned char>"),
         Call("sharpen", "sharpen", "rgb", u8, lambda lib, s, d, st: lib.ipfg_sharpen(s, d, 1.0, 0, st),
-             lambda o, im: o.sharpen(im, 1.0), "f2d_fast_kernel<unsigned char, double, false, true, 4>"),
+             lambda o, im: o.sharpen(im, 1.0), "stencil_u8_kernel<unsigned char, 3, 3, 3>"),
         Call("emboss", "emboss", "rgb", u8, lambda lib, s, d, st: lib.ipfg_emboss(s, d, 0, st),
-             lambda o, im: o.emboss(im), "f2d_fast_kernel<unsigned char, double, false, true, 4>"),
+             lambda o, im: o.emboss(im), "stencil_u8_kernel<unsigned char, 3, 3, 3>"),
         Call("adaptive_mean11", "adaptive_mean11", "gray", u8,
              lambda lib, s, d, st: lib.ipfg_adaptive_threshold(s, d, 255.0, 0, 0, 11, 2.0, 0, st),
-             lambda o, im: o.adaptive_threshold(im, 255.0, 0, 0, 11, 2.0), "box pass + adaptive_kernel"),
+             lambda o, im: o.adaptive_threshold(im, 255.0, 0, 0, 11, 2.0),
+             "sep_plane_kernel<unsigned char, unsigned char, double, float, 1, 2, true, 11> + adaptive_rows_kernel"),
         Call("adaptive_gauss11", "adaptive_gauss11", "gray", u8,
              lambda lib, s, d, st: lib.ipfg_adaptive_threshold(s, d, 255.0, 1, 0, 11, 2.0, 0, st),
-             lambda o, im: o.adaptive_threshold(im, 255.0, 1, 0, 11, 2.0), "gaussian pass + adaptive_kernel"),
+             lambda o, im: o.adaptive_threshold(im, 255.0, 1, 0, 11, 2.0),
+             "sep_plane_kernel<unsigned char, unsigned char, double, float, 1, 2, true, 11> + adaptive_rows_kernel"),
     ]
     return {"images": {"rgb": rgb, "gray": gray}, "calls": calls, "dtype": "f64",
             "workload": "sobel/scharr/laplacian (float out), unsharpMask/sharpen/emboss on 3840x2160x3 uint8; "
