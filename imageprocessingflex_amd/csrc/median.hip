@@ -31,6 +31,18 @@ __device__ __forceinline__ K kmax(K a, K b) {
     return a < b ? b : a;
 }
 
+// Two 16-bit keys in one register: the comparators of two independent
+// windows run as one v_pk_min_u16 / v_pk_max_u16 each.
+typedef unsigned short KeyPair __attribute__((ext_vector_type(2)));
+template <>
+__device__ __forceinline__ KeyPair kmin<KeyPair>(KeyPair a, KeyPair b) {
+    return __builtin_elementwise_min(a, b);
+}
+template <>
+__device__ __forceinline__ KeyPair kmax<KeyPair>(KeyPair a, KeyPair b) {
+    return __builtin_elementwise_max(a, b);
+}
+
 }  // namespace
 }  // namespace ipfg
 
@@ -164,7 +176,78 @@ __device__ __forceinline__ void median_tile(const T* __restrict__ src, size_t ss
     }
     __syncthreads();
 
-    if constexpr (KS == 3 || KS == 5) {
+    if constexpr ((KS == 3 || KS == 5) && KT::bits <= 16) {
+        // Rows row and row + TH / 2 of the tile share every comparator (keys of
+        // at most 16 bits packed in one register).
+        constexpr int G = TX / P;
+        static_assert(kBlock % G == 0, "a thread keeps its column group");
+        const int HT = TH / 2;  // TH is an even power of two here
+        const int ntask = HT * G * cn;
+        const int g = tid % G;
+        int row = (tid / G) % HT, c = (tid / G) / HT;
+        for (int t = tid; t < ntask; t += kBlock, row += kBlock / G) {
+            while (row >= HT) {
+                row -= HT;
+                ++c;
+            }
+            const K* base = tile + c * plane + row * XP + g * P;
+            const K* base2 = base + HT * XP;
+            auto ld = [&](int off) {
+                KeyPair v;
+                v.x = static_cast<unsigned short>(base[off]);
+                v.y = static_cast<unsigned short>(base2[off]);
+                return v;
+            };
+            KeyPair res[P];
+            if constexpr (KS == 3) {
+                KeyPair lo[P + 2], md[P + 2], hi[P + 2];
+#pragma unroll
+                for (int cc = 0; cc < P + 2; ++cc) {
+                    const KeyPair a = ld(cc), b = ld(XP + cc), d = ld(2 * XP + cc);
+                    const KeyPair mn = kmin(a, b), mx = kmax(a, b);
+                    lo[cc] = kmin(mn, d);
+                    hi[cc] = kmax(mx, d);
+                    md[cc] = kmax(mn, kmin(mx, d));
+                }
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    const KeyPair a = kmax(kmax(lo[m], lo[m + 1]), lo[m + 2]);
+                    const KeyPair b = med3(md[m], md[m + 1], md[m + 2]);
+                    const KeyPair d = kmin(kmin(hi[m], hi[m + 1]), hi[m + 2]);
+                    res[m] = med3(a, b, d);
+                }
+            } else {
+                KeyPair col[P + 4][5];
+#pragma unroll
+                for (int cc = 0; cc < P + 4; ++cc) {
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) col[cc][j] = ld(j * XP + cc);
+                    sort5(col[cc]);
+                }
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    KeyPair win[25];
+#pragma unroll
+                    for (int cc = 0; cc < 5; ++cc)
+#pragma unroll
+                        for (int j = 0; j < 5; ++j) win[cc * 5 + j] = col[m + cc][j];
+                    res[m] = med25_cols(win);
+                }
+            }
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int y = y0 + row + half * HT;
+                if (y >= H) continue;
+                T* drow = reinterpret_cast<T*>(reinterpret_cast<char*>(dst) + static_cast<size_t>(y) * dstride);
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    const int x = x0 + g * P + m;
+                    if (x >= W) break;
+                    drow[static_cast<size_t>(x) * cn + c] = KT::from(half ? res[m].y : res[m].x);
+                }
+            }
+        }
+    } else if constexpr (KS == 3 || KS == 5) {
         constexpr int G = TX / P;
         static_assert(kBlock % G == 0, "a thread keeps its column group");
         const int ntask = TH * G * cn;
@@ -295,7 +378,7 @@ template <class T, int KS>
 hipError_t median_launch(const MedianLaunch& a, hipStream_t s) {
     const int cn = a.src.cn;
     int TH = 32;
-    while (TH > 1 && med_lds_bytes<T>(cn, TH, a.k) > 64 * 1024) TH /= 2;
+    while (TH > 2 && med_lds_bytes<T>(cn, TH, a.k) > 64 * 1024) TH /= 2;  // even: the 16-bit path pairs rows
     const size_t lds = med_lds_bytes<T>(cn, TH, a.k);
     if (lds > 64 * 1024) {
         const size_t total = static_cast<size_t>(a.src.w) * a.src.h * cn;
