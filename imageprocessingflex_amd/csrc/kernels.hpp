@@ -70,6 +70,11 @@ hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s);
 // hipErrorNotSupported when the call does not fit (types, sizes, alignment).
 hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const float* k, int kw, int kh, int ay,
                              int border, double delta, double bval, hipStream_t s);
+// unsharpMask on uint8 images (ks <= 13, 1/3/4 channels, 16-byte aligned
+// rows) in one launch; k_host / k_dev hold the ks FP64 Gaussian weights and
+// bnd bounds |FP32 blur - reference blur|. hipErrorNotSupported otherwise.
+hipError_t launch_unsharp_u8(const DevImage& src, const DevImage& dst, int ks, const double* k_host,
+                             const double* k_dev, double amount, double threshold, double bnd, hipStream_t s);
 // MFMA implicit-GEMM filter2D (precision BF16X3 / FP16X2).
 hipError_t launch_filter2d_mfma(const F2dLaunch& a, hipStream_t s);
 size_t f2d_mfma_lds_bytes(int kw, int kh, bool split_input);

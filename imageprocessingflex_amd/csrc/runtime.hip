@@ -1374,10 +1374,6 @@ void unsharp_impl(const ipfg_image* src, const ipfg_image* dst, double sigma, do
         // taps with weights rounded to float stay within
         // (2 ks + 4) 2^-24 (sum|k|)^2 X of the exact blur, as does the
         // reference's FP64 blur (with room to spare); X = the type's bound.
-        const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F32);
-        const ipfg_image db = dev_view(b);
-        sep_impl(&ds, &db, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0,
-                 IPFG_PREC_FP32, reinterpret_cast<ipfg_stream>(s));
         double sk = 0.0;
         for (double v : k) sk += std::fabs(v);
         const double X = input_bound(dt, 0.0, IPFG_BORDER_REFLECT101);
@@ -1385,6 +1381,20 @@ void unsharp_impl(const ipfg_image* src, const ipfg_image* dst, double sigma, do
         std::vector<char> blob(sizeof(double) * k.size());
         std::memcpy(blob.data(), k.data(), blob.size());
         const double* kd = static_cast<const double*>(sc.params(blob));
+        if (dt == IPFG_U8) {  // small windows: blur and combine in one launch
+            const hipError_t e = launch_unsharp_u8(io.src(), io.dst(), ks, k.data(), kd, amount, threshold, bnd, s);
+            if (e != hipErrorNotSupported) {
+                HIPCALL(e);
+                t_ctx.fix_valid = false;
+                io.finish();
+                return;
+            }
+            (void)hipGetLastError();
+        }
+        const DevImage b = scratch_image(sc, src->width, src->height, src->channels, IPFG_F32);
+        const ipfg_image db = dev_view(b);
+        sep_impl(&ds, &db, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0,
+                 IPFG_PREC_FP32, reinterpret_cast<ipfg_stream>(s));
         const dim3 grid = rows_grid(io.dst());
         switch (dt) {
             case IPFG_U8:

@@ -302,6 +302,21 @@ def test_small_exact_stencils_vs_oracle(ipf, oracle, shape):
     assert np.array_equal(out.download().reshape(img.shape), oracle.sharpen(img, 0.75))
 
 
+@pytest.mark.parametrize("shape", [(211, 320, 3), (97, 256), (64, 96, 4), (7, 48, 3), (40, 20, 1), (3, 400, 4)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_unsharp_u8_single_launch_vs_oracle(ipf, oracle, shape):
+    # uint8 unsharpMask with windows up to 13 taps: FP32 blur and certified
+    # combine in one launch (interior, frame rows and columns, recomputed
+    # doubtful samples); larger sigmas take the two-pass route
+    rng = np.random.default_rng(sum(shape) * 11)
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
+    if img.ndim == 3 and img.shape[0] > 20:
+        img[5:15, 8:30] = 128  # a flat patch: blur == sample up to rounding
+    for sigma, amount, th in ((0.5, 1.0, 0.0), (1.0, 1.5, 0.0), (1.5, -0.6, 3.0), (2.0, 2.0, 1.0), (2.6, 0.8, 0.0)):
+        got = ipf.unsharp_mask(img, sigma, amount, th)
+        assert np.array_equal(got, oracle.unsharp_mask(img, sigma, amount, th)), (sigma, amount, th)
+
+
 # ------------------------------------------- derivative and sharpening filters
 def test_derivative_filters_vs_oracle(ipf, oracle):
     rng = np.random.default_rng(41)
