@@ -210,7 +210,7 @@ def _deriv():
         Call("unsharp_s1", "unsharp_s1", "rgb", u8,
              lambda lib, s, d, st: lib.ipfg_unsharp_mask(s, d, 1.0, 1.0, 0.0, 0, st),
              lambda o, im: o.unsharp_mask(im, 1.0, 1.0, 0.0),
-             "sep_plane_kernel<unsigned char, float, float, float, 3, 8, false, 0> + unsharp_cert_kernel<unsigned char>"),
+             "unsharp_u8_kernel<7, 3>"),
         Call("sharpen", "sharpen", "rgb", u8, lambda lib, s, d, st: lib.ipfg_sharpen(s, d, 1.0, 0, st),
              lambda o, im: o.sharpen(im, 1.0), "stencil_u8_kernel<unsigned char, 3, 3, 3>"),
         Call("emboss", "emboss", "rgb", u8, lambda lib, s, d, st: lib.ipfg_emboss(s, d, 0, st),
