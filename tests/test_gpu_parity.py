@@ -129,6 +129,27 @@ def test_row_stride_views(ipf, oracle):
     want = oracle.gaussian_blur(np.ascontiguousarray(view), 2.0)
     assert np.array_equal(ipf.gaussian_blur(view, 2.0), want)
     assert np.array_equal(ipf.median_blur(view, 5), oracle.median(np.ascontiguousarray(view), 5))
+    cv = np.ascontiguousarray(view)
+    assert np.array_equal(ipf.sobel(view, 1, 0), oracle.sobel(cv, 1, 0))
+    assert np.array_equal(ipf.sharpen(view, 0.5), oracle.sharpen(cv, 0.5))
+    assert np.array_equal(ipf.unsharp_mask(view, 1.0, 1.2, 1.0), oracle.unsharp_mask(cv, 1.0, 1.2, 1.0))
+
+
+def test_in_place_device_calls_of_the_register_kernels(ipf, oracle):
+    # src and dst the same device buffer: the call reads a copy of the source
+    import ctypes as C
+    from imageprocessingflex_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(12)
+    img = rng.integers(0, 256, (120, 256, 3), dtype=np.uint8)
+    calls = ((lambda dsc, h: lib.ipfg_sharpen(dsc, dsc, 1.0, 0, h), oracle.sharpen(img, 1.0)),
+             (lambda dsc, h: lib.ipfg_unsharp_mask(dsc, dsc, 1.0, 1.5, 0.0, 0, h), oracle.unsharp_mask(img, 1.0, 1.5, 0.0)))
+    for fn, ref in calls:
+        s = ipf.Stream()
+        d = ipf.DeviceImage.from_array(img, s)
+        desc = d.descriptor()
+        _lib.check(fn(C.byref(desc), s.handle))
+        assert np.array_equal(d.download(s).reshape(img.shape), ref)
 
 
 def test_device_resident_async_and_in_place(ipf, oracle):
