@@ -24,6 +24,7 @@ struct SepLaunch {
     double bcoef, bdelta, xbound;
     double kabs;  // sum|kx| * sum|ky| (0: the ring holds exact sums)
     double oscale;  // results are multiplied by this (0 means 1)
+    int unit;       // every weight is 1 (box filters)
     FixList fix;
     double* tmp;  // MODE_EXACT: (H + kh - 1) * W * cn doubles
     int sh;       // strip height of the fast kernels (0: built-in choice)

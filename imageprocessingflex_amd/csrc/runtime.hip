@@ -739,6 +739,7 @@ void box_impl(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int 
         l.delta = 0.0;
         l.bval = bval;
         l.oscale = scale;
+        l.unit = 1;
         l.mode = precision == IPFG_PREC_FP32 ? MODE_FAST : MODE_CERTIFY;
         constexpr int kPad = 16;
         const int nx = kw + kPad, ny = kh + kPad;

@@ -152,6 +152,7 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     a.fix = l.fix;
     a.SH = l.sh;
     a.oscale = l.oscale != 0.0 ? l.oscale : 1.0;
+    a.unit = l.unit;
     const int cn = l.src.cn;
     const bool planar = sep_planar(cn, l.src.dt, l.dst.dt);
     if (planar && l.src.dt == IPFG_U8) {

@@ -118,6 +118,7 @@ struct SepArgs {
     double bdelta;
     double xbound;       // fixed X for integer images (< 0: measure the tile)
     double oscale;       // results are multiplied by this (1 for sepFilter2D; 1 / (kw kh) for a mean box)
+    int unit;            // every weight is 1 (box): the strip kernel slides its sums
     int SH;
     FixList fix;
 };
@@ -588,6 +589,19 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
             for (int c = 0; c < CN; ++c) {
                 const float* base = in_buf + c * plane_in + hr * T.XPP + hg * P;
                 Acc acc[P], win[P];
+                if (a.unit) {  // unit weights: one window sum, then slide it across the P outputs
+                    Acc sum = Acc(0);
+#pragma unroll U
+                    for (int i = 0; i < kw; ++i) sum += static_cast<Acc>(base[i]);
+                    acc[0] = sum;
+#pragma unroll
+                    for (int m = 1; m < P; ++m) {
+                        sum += static_cast<Acc>(base[m - 1 + kw]) - static_cast<Acc>(base[m - 1]);
+                        acc[m] = sum;
+                    }
+                    store_h4(ring + c * T.RPL + ((r0 + hr) & msk) * PTX + hg * P, acc);
+                    continue;
+                }
                 {
                     const float4 f = *reinterpret_cast<const float4*>(base);
                     win[0] = f.x;
@@ -653,6 +667,18 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
         for (int c = 0; c < CN; ++c) {
             const Hr* plane = ring + c * T.RPL + vx;
             Acc acc[M], win[M];
+            if (a.unit) {  // unit weights: one window sum, then slide it down the M rows
+                Acc sum = static_cast<Acc>(a.delta);
+#pragma unroll U
+                for (int j = 0; j < kh; ++j) sum += static_cast<Acc>(plane[((yr + j) & msk) * PTX]);
+                acc[0] = sum;
+#pragma unroll
+                for (int m = 1; m < M; ++m) {
+                    sum += static_cast<Acc>(plane[((yr + m - 1 + kh) & msk) * PTX]) -
+                           static_cast<Acc>(plane[((yr + m - 1) & msk) * PTX]);
+                    acc[m] = sum;
+                }
+            } else {
             {
                 const Hr* g0 = plane + (yr & msk) * PTX;  // rows yr .. yr + M - 1 do not wrap
 #pragma unroll
@@ -691,6 +717,7 @@ __device__ __forceinline__ void sep_plane_strip(const SepArgs& a, const Acc* __r
                         win[jj] = static_cast<Acc>(gp[jj * PTX]);
                     }
                 }
+            }
             }
             unsigned int dmask = 0u;  // doubtful outputs of this task
             if (xg < W) {
