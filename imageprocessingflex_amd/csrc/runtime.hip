@@ -1353,8 +1353,10 @@ void polar_impl(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* ds
     io.finish();
 }
 
-// unsharpMask (filter.hpp:472-483): FP64 Gaussian blur (Reflect101), then the
-// elementwise combine.
+// unsharpMask (filter.hpp:472-483). Integer images outside EXACT: uint8 with
+// windows up to 13 taps in one launch (unsharp.hip), other sizes and types an
+// FP32 blur plus the certified combine; float images and EXACT: the
+// reference's FP64 blur (Reflect101) into a double image, then the combine.
 void unsharp_impl(const ipfg_image* src, const ipfg_image* dst, double sigma, double amount, double threshold,
                   int precision, ipfg_stream stream) {
     const std::vector<double> k = gaussian_kernel(sigma, 0);
