@@ -10,7 +10,7 @@
 //
 // Layout: a row of W * cn interleaved samples is a flat array in which the
 // horizontal neighbours of a sample sit cn samples apart. A thread owns 4
-// consecutive flat samples (one dword of the row) in 4 consecutive rows and
+// consecutive flat samples (one dword of the row) in 8 consecutive rows and
 // loads, per input row, its dword and DH halo dwords on each side, so a
 // wave's loads and stores are contiguous and nothing passes through LDS. The workgroups after the
 // interior ones compute the frame the interior leaves out (rows whose window
@@ -92,7 +92,7 @@ __device__ __forceinline__ void frame_sample(const StencilArgs& a, long long idx
 // Output rows per thread: a thread computes one group of 4 samples in NG
 // consecutive rows, so the NG + KH - 1 input rows it loads (and converts) are
 // shared by all of them.
-constexpr int kStencilRows = 4;
+constexpr int kStencilRows = 8;
 
 // Workgroups [0, nbi) compute interior groups of 4 samples (one dword of the
 // row) from windows of 1 + 2 DH dwords per input row; consecutive lanes take
