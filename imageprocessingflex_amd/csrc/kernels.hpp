@@ -69,8 +69,10 @@ hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s);
 // 3x3 / 5x5 kernels with a centred anchor on uint8 images (uint8 or float32
 // out) whose sums are exact in FP32; k is row-major on the host.
 // hipErrorNotSupported when the call does not fit (types, sizes, alignment).
+// kx / ky (optional): k is their outer product, applied as two 1-D passes.
 hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const float* k, int kw, int kh, int ay,
-                             int border, double delta, double bval, hipStream_t s);
+                             int border, double delta, double bval, hipStream_t s, const float* kx = nullptr,
+                             const float* ky = nullptr);
 // unsharpMask on uint8 images (ks <= 13, 1/3/4 channels, 16-byte aligned
 // rows) in one launch; k_host / k_dev hold the ks FP64 Gaussian weights and
 // bnd bounds |FP32 blur - reference blur|. hipErrorNotSupported otherwise.

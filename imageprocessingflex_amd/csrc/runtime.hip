@@ -678,10 +678,12 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
     }
     hipError_t e = hipErrorNotSupported;
     if (exact_level == 3 && a.mode == MODE_FAST && kw == kh && ax == kw / 2 && ay == kh / 2 && (kw == 3 || kw == 5)) {
-        float k2[25];  // kx[i] * ky[j]: exact in FP32 at this level
+        float k2[25], fx[5], fy[5];  // kx[i] * ky[j] and the factors: exact in FP32 at this level
         for (int j = 0; j < kh; ++j)
             for (int i = 0; i < kw; ++i) k2[j * kw + i] = static_cast<float>(ky[j] * kx[i]);
-        e = launch_stencil_u8(a.src, a.dst, k2, kw, kh, ay, border, delta, bval, s);
+        for (int i = 0; i < kw; ++i) fx[i] = static_cast<float>(kx[i]);
+        for (int j = 0; j < kh; ++j) fy[j] = static_cast<float>(ky[j]);
+        e = launch_stencil_u8(a.src, a.dst, k2, kw, kh, ay, border, delta, bval, s, fx, fy);
     }
     if (e == hipErrorNotSupported) {
         (void)hipGetLastError();

@@ -35,7 +35,8 @@ struct StencilArgs {
     int lside, rside;  // frame: the first lside and the last rside samples of each interior row
     long long nframe;  // frame samples
     float bval, delta;
-    float k[kStencilMax];
+    float k[kStencilMax];   // row-major kh x kw weights
+    float kx[5], ky[5];     // separable launches: k = ky (x) kx
 };
 
 // Saturated uint8 of an exact float (round half away from zero): on (0, 255)
@@ -99,7 +100,7 @@ constexpr int kStencilRows = 8;
 // consecutive groups of a row, so every load and store instruction of a wave
 // covers contiguous dwords. The remaining workgroups compute the frame one
 // sample per thread.
-template <class To, int KW, int KH, int CN>
+template <class To, int KW, int KH, int CN, bool SEP>
 __global__ __launch_bounds__(kBlock) void stencil_u8_kernel(StencilArgs a) {
     constexpr int AX = KW / 2;
     constexpr int DH = (AX * CN + 3) / 4;  // halo dwords on each side
@@ -130,19 +131,40 @@ __global__ __launch_bounds__(kBlock) void stencil_u8_kernel(StencilArgs a) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) w[t][d] = p[d];
     }
+    // separable kernels: the horizontal sums of each input row, shared by the
+    // output rows it reaches
+    float h[SEP ? NR : 1][4];
+    if constexpr (SEP) {
+#pragma unroll
+        for (int t = 0; t < NR; ++t)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                h[t][m] = 0.f;
+#pragma unroll
+                for (int i = 0; i < KW; ++i) h[t][m] = fmaf(a.kx[i], win_byte(w[t], 4 * DH + m + (i - AX) * CN), h[t][m]);
+            }
+    }
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
         float acc[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) acc[m] = a.delta;
+        if constexpr (SEP) {
 #pragma unroll
-        for (int j = 0; j < KH; ++j)
+            for (int j = 0; j < KH; ++j)
 #pragma unroll
-            for (int i = 0; i < KW; ++i) {
-                const float kk = a.k[j * KW + i];
+                for (int m = 0; m < 4; ++m) acc[m] = fmaf(a.ky[j], h[k + j][m], acc[m]);
+        } else {
 #pragma unroll
-                for (int m = 0; m < 4; ++m) acc[m] = fmaf(kk, win_byte(w[k + j], 4 * DH + m + (i - AX) * CN), acc[m]);
-            }
+            for (int j = 0; j < KH; ++j)
+#pragma unroll
+                for (int i = 0; i < KW; ++i) {
+                    const float kk = a.k[j * KW + i];
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+                        acc[m] = fmaf(kk, win_byte(w[k + j], 4 * DH + m + (i - AX) * CN), acc[m]);
+                }
+        }
         if (r0 + k >= a.HI) break;
         char* drow = a.dst + static_cast<size_t>(a.ay + r0 + k) * a.dstride;
         if constexpr (sizeof(To) == 1)
@@ -154,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void stencil_u8_kernel(StencilArgs a) {
     }
 }
 
-template <class To, int KW, int KH>
+template <class To, int KW, int KH, bool SEP>
 hipError_t stencil_by_cn(StencilArgs a, int cn, hipStream_t s) {
     const int ah = (KW / 2) * cn;
     const int dh = (ah + 3) / 4;
@@ -184,9 +206,9 @@ hipError_t stencil_by_cn(StencilArgs a, int cn, hipStream_t s) {
     a.nbi = static_cast<int>((nb + 7) / 8 * 8);
     const dim3 grid(static_cast<unsigned>(a.nbi + nbf));
     switch (cn) {
-        case 1: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 1>), grid, dim3(kBlock), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 3>), grid, dim3(kBlock), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 4>), grid, dim3(kBlock), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 1, SEP>), grid, dim3(kBlock), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 3, SEP>), grid, dim3(kBlock), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 4, SEP>), grid, dim3(kBlock), 0, s, a); break;
         default: return hipErrorNotSupported;
     }
     return hipGetLastError();
@@ -199,7 +221,7 @@ bool aligned16(const void* p, size_t stride) {
 }  // namespace
 
 hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const float* k, int kw, int kh, int ay,
-                             int border, double delta, double bval, hipStream_t s) {
+                             int border, double delta, double bval, hipStream_t s, const float* kx, const float* ky) {
     const int cn = src.cn;
     if (src.dt != IPFG_U8 || (dst.dt != IPFG_U8 && dst.dt != IPFG_F32)) return hipErrorNotSupported;
     if (!((kw == 3 && kh == 3) || (kw == 5 && kh == 5))) return hipErrorNotSupported;
@@ -222,9 +244,18 @@ hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const flo
     a.bval = static_cast<float>(bval);
     a.delta = static_cast<float>(delta);
     for (int i = 0; i < kw * kh; ++i) a.k[i] = k[i];
+    const bool sep = kx != nullptr && ky != nullptr;
+    if (sep) {
+        for (int i = 0; i < kw; ++i) a.kx[i] = kx[i];
+        for (int j = 0; j < kh; ++j) a.ky[j] = ky[j];
+    }
     const bool f32 = dst.dt == IPFG_F32;
-    if (kw == 3) return f32 ? stencil_by_cn<float, 3, 3>(a, cn, s) : stencil_by_cn<uint8_t, 3, 3>(a, cn, s);
-    return f32 ? stencil_by_cn<float, 5, 5>(a, cn, s) : stencil_by_cn<uint8_t, 5, 5>(a, cn, s);
+    if (kw == 3) {
+        if (sep) return f32 ? stencil_by_cn<float, 3, 3, true>(a, cn, s) : stencil_by_cn<uint8_t, 3, 3, true>(a, cn, s);
+        return f32 ? stencil_by_cn<float, 3, 3, false>(a, cn, s) : stencil_by_cn<uint8_t, 3, 3, false>(a, cn, s);
+    }
+    if (sep) return f32 ? stencil_by_cn<float, 5, 5, true>(a, cn, s) : stencil_by_cn<uint8_t, 5, 5, true>(a, cn, s);
+    return f32 ? stencil_by_cn<float, 5, 5, false>(a, cn, s) : stencil_by_cn<uint8_t, 5, 5, false>(a, cn, s);
 }
 
 }  // namespace ipfg

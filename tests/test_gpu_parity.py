@@ -436,6 +436,22 @@ def test_full_c4_filter2d_u8_bit_exact(ipf):
     print("c4 fix-ups", ipf.last_fixups())
 
 
+def test_full_size_derivative_workload_vs_oracle(ipf, oracle):
+    # the derivative / sharpening calls at 3840 x 2160 (RGB and gray uint8),
+    # against the oracle (all host threads)
+    from imageprocessingflex_amd import workloads as wl
+    rgb = wl.uniform_u8((2160, 3840, 3), seed=7, stream=0)
+    gray = wl.uniform_u8((2160, 3840), seed=7, stream=1)
+    assert np.array_equal(ipf.sobel(rgb, 1, 0, 3), oracle.sobel(rgb, 1, 0, 3))
+    assert np.array_equal(ipf.sobel(rgb, 0, 1, 5), oracle.sobel(rgb, 0, 1, 5))
+    assert np.array_equal(ipf.laplacian(rgb, 3), oracle.laplacian(rgb, 3))
+    assert np.array_equal(ipf.unsharp_mask(rgb, 1.0, 1.0, 0.0), oracle.unsharp_mask(rgb, 1.0, 1.0, 0.0))
+    assert np.array_equal(ipf.sharpen(rgb, 1.0), oracle.sharpen(rgb, 1.0))
+    for m in (0, 1):
+        assert np.array_equal(ipf.adaptive_threshold(gray, 255.0, m, 0, 11, 2.0),
+                              oracle.adaptive_threshold(gray, 255.0, m, 0, 11, 2.0)), m
+
+
 def test_full_c2_exact_mode_bit_exact_and_fast_within_bound(ipf):
     from imageprocessingflex_amd import workloads as wl
     img = wl.sinusoid_image()
