@@ -198,23 +198,23 @@ def _deriv():
     u8 = np.uint8
     calls = [
         Call("sobel_dx3", "sobel_dx3", "rgb", f, lambda lib, s, d, st: lib.ipfg_sobel(s, d, 1, 0, 3, 1.0, 3, 0, st),
-             lambda o, im: o.sobel(im, 1, 0, 3), "stencil_u8_kernel<float, 3, 3, 3>"),
+             lambda o, im: o.sobel(im, 1, 0, 3), "stencil_u8_kernel<float, 3, 3, 3, true>"),
         Call("sobel_dy5", "sobel_dy5", "rgb", f, lambda lib, s, d, st: lib.ipfg_sobel(s, d, 0, 1, 5, 1.0, 3, 0, st),
-             lambda o, im: o.sobel(im, 0, 1, 5), "stencil_u8_kernel<float, 5, 5, 3>"),
+             lambda o, im: o.sobel(im, 0, 1, 5), "stencil_u8_kernel<float, 5, 5, 3, true>"),
         Call("scharr_dx", "scharr_dx", "rgb", f, lambda lib, s, d, st: lib.ipfg_scharr(s, d, 1, 0, 1.0, 3, 0, st),
-             lambda o, im: o.scharr(im, 1, 0), "stencil_u8_kernel<float, 3, 3, 3>"),
+             lambda o, im: o.scharr(im, 1, 0), "stencil_u8_kernel<float, 3, 3, 3, true>"),
         Call("laplacian1", "laplacian1", "rgb", f, lambda lib, s, d, st: lib.ipfg_laplacian(s, d, 1, 1.0, 3, 0, st),
-             lambda o, im: o.laplacian(im, 1), "stencil_u8_kernel<float, 3, 3, 3>"),
+             lambda o, im: o.laplacian(im, 1), "stencil_u8_kernel<float, 3, 3, 3, false>"),
         Call("laplacian3", "laplacian3", "rgb", f, lambda lib, s, d, st: lib.ipfg_laplacian(s, d, 3, 1.0, 3, 0, st),
-             lambda o, im: o.laplacian(im, 3), "stencil_u8_kernel<float, 3, 3, 3> (combined kernel)"),
+             lambda o, im: o.laplacian(im, 3), "stencil_u8_kernel<float, 3, 3, 3, false> (combined kernel)"),
         Call("unsharp_s1", "unsharp_s1", "rgb", u8,
              lambda lib, s, d, st: lib.ipfg_unsharp_mask(s, d, 1.0, 1.0, 0.0, 0, st),
              lambda o, im: o.unsharp_mask(im, 1.0, 1.0, 0.0),
              "unsharp_u8_kernel<7, 3>"),
         Call("sharpen", "sharpen", "rgb", u8, lambda lib, s, d, st: lib.ipfg_sharpen(s, d, 1.0, 0, st),
-             lambda o, im: o.sharpen(im, 1.0), "stencil_u8_kernel<unsigned char, 3, 3, 3>"),
+             lambda o, im: o.sharpen(im, 1.0), "stencil_u8_kernel<unsigned char, 3, 3, 3, false>"),
         Call("emboss", "emboss", "rgb", u8, lambda lib, s, d, st: lib.ipfg_emboss(s, d, 0, st),
-             lambda o, im: o.emboss(im), "stencil_u8_kernel<unsigned char, 3, 3, 3>"),
+             lambda o, im: o.emboss(im), "stencil_u8_kernel<unsigned char, 3, 3, 3, false>"),
         Call("adaptive_mean11", "adaptive_mean11", "gray", u8,
              lambda lib, s, d, st: lib.ipfg_adaptive_threshold(s, d, 255.0, 0, 0, 11, 2.0, 0, st),
              lambda o, im: o.adaptive_threshold(im, 255.0, 0, 0, 11, 2.0),
