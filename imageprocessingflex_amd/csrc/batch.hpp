@@ -111,7 +111,7 @@ hipError_t launch_median_batch(const BatchClassHeader* hdr, const BatchJob* jobs
 hipError_t launch_bilateral_batch(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
                                   int grid, size_t lds, hipStream_t s);
 // Occupancy (workgroups per CU at the given LDS) and LDS of the class kernels.
-int sep_batch_occupancy(int cn, size_t lds);
+int sep_batch_occupancy(int cls, size_t lds);
 size_t sep_batch_lds(int cn, int kmax);
 int median_batch_occupancy(size_t lds);
 size_t median_batch_lds();

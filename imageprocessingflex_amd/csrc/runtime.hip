@@ -1517,7 +1517,7 @@ int batch_grid(int cls, size_t lds) {
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     int per_cu = 1;
-    if (cls < BCLS_MEDIAN) per_cu = sep_batch_occupancy(batch_sep_cn(cls), lds);
+    if (cls < BCLS_MEDIAN) per_cu = sep_batch_occupancy(cls, lds);
     else if (cls == BCLS_MEDIAN) per_cu = median_batch_occupancy(lds);
     else per_cu = bilateral_batch_occupancy(lds);
     return std::max(1, cus) * per_cu;

@@ -7,7 +7,9 @@ namespace ipfg {
 namespace sepk {
 namespace {
 
-template <int CN>
+// SJ: stager samples per lane for the class's widest rows (PTX + kmax - 1)
+// x CN, so the small-kernel classes keep their registers for occupancy.
+template <int CN, int SJ>
 __global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeader* __restrict__ hdr,
                                                            const BatchJob* __restrict__ jobs, BatchQueue* q,
                                                            FixList fix) {
@@ -33,7 +35,7 @@ __global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeade
         a.SH = J.rows;
         a.oscale = 1.0;
         a.fix = fix;
-        sep_plane_strip<uint8_t, uint8_t, double, float, CN, 8, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem);
+        sep_plane_strip<uint8_t, uint8_t, double, float, CN, SJ, true, 0>(a, J.kx, J.ky, tx, ty, sep_smem);
     });
     fix_finish(fix);
 }
@@ -41,11 +43,27 @@ __global__ __launch_bounds__(kBlock) void sep_batch_kernel(const BatchClassHeade
 }  // namespace
 }  // namespace sepk
 
-template <int CN>
-static hipError_t launch_cn(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix, int grid,
+template <int CN, int KMAX>
+const void* class_kernel() {
+    return reinterpret_cast<const void*>(sepk::sep_batch_kernel<CN, sepk::plane_sj(CN, KMAX)>);
+}
+
+const void* sep_class_kernel(int cls) {
+    switch (cls) {
+        case 0: return class_kernel<1, kBatchSepSmallK>();
+        case 1: return class_kernel<1, kBatchSepLargeK>();
+        case 2: return class_kernel<3, kBatchSepSmallK>();
+        case 3: return class_kernel<3, kBatchSepLargeK>();
+        case 4: return class_kernel<4, kBatchSepSmallK>();
+        default: return class_kernel<4, kBatchSepLargeK>();
+    }
+}
+
+template <int CN, int KMAX>
+static hipError_t launch_ck(const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix, int grid,
                             size_t lds, hipStream_t s) {
     static size_t raised = 0;
-    auto kern = sepk::sep_batch_kernel<CN>;
+    auto kern = sepk::sep_batch_kernel<CN, sepk::plane_sj(CN, KMAX)>;
     if (lds > 64 * 1024 && lds > raised) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
@@ -58,17 +76,18 @@ static hipError_t launch_cn(const BatchClassHeader* hdr, const BatchJob* jobs, B
 
 hipError_t launch_sep_batch(int cls, const BatchClassHeader* hdr, const BatchJob* jobs, BatchQueue* q, FixList fix,
                             int grid, size_t lds, hipStream_t s) {
-    switch (batch_sep_cn(cls)) {
-        case 1: return launch_cn<1>(hdr, jobs, q, fix, grid, lds, s);
-        case 3: return launch_cn<3>(hdr, jobs, q, fix, grid, lds, s);
-        default: return launch_cn<4>(hdr, jobs, q, fix, grid, lds, s);
+    switch (cls) {
+        case 0: return launch_ck<1, kBatchSepSmallK>(hdr, jobs, q, fix, grid, lds, s);
+        case 1: return launch_ck<1, kBatchSepLargeK>(hdr, jobs, q, fix, grid, lds, s);
+        case 2: return launch_ck<3, kBatchSepSmallK>(hdr, jobs, q, fix, grid, lds, s);
+        case 3: return launch_ck<3, kBatchSepLargeK>(hdr, jobs, q, fix, grid, lds, s);
+        case 4: return launch_ck<4, kBatchSepSmallK>(hdr, jobs, q, fix, grid, lds, s);
+        default: return launch_ck<4, kBatchSepLargeK>(hdr, jobs, q, fix, grid, lds, s);
     }
 }
 
-int sep_batch_occupancy(int cn, size_t lds) {
-    const void* k = cn == 1 ? reinterpret_cast<const void*>(sepk::sep_batch_kernel<1>)
-                  : cn == 3 ? reinterpret_cast<const void*>(sepk::sep_batch_kernel<3>)
-                            : reinterpret_cast<const void*>(sepk::sep_batch_kernel<4>);
+int sep_batch_occupancy(int cls, size_t lds) {
+    const void* k = sep_class_kernel(cls);
     if (lds > 64 * 1024)  // the attribute must allow the LDS before occupancy is asked
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     int n = 0;
