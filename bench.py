@@ -1,6 +1,6 @@
 """Benchmark of the MI355X filtering path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|deriv]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|deriv|pyr]
                     [--no-cpu] [--cpu-budget S]
 
 Workloads (one step = every call listed once; inputs resident in HBM; enough
@@ -18,6 +18,9 @@ Cache):
                 laplacian (ksize 1 and 3) with float32 output, unsharpMask,
                 sharpen, emboss; 3840x2160x1 uint8: adaptiveThreshold (mean and
                 Gaussian, block 11).
+  pyr           3840x2160x3 uint8 polygons + noise (the c3 image): pyrDown and
+                pyrUp (exact FP64 in the reference's operation order); pixels
+                are counted at the input.
 All calls use the default precision: FP64 accumulation, certified bit-identical
 to the reference (median is compare-only).
 
@@ -86,9 +89,26 @@ class Call:
     """One filter call of a step: `launch(lib, src, dst, stream)` enqueues it
     on device descriptors; `cpu(oracle, img)` runs the reference's CPU path."""
 
-    def __init__(self, name, group, image, out_dtype, launch, cpu, kernel, flops=0.0):
+    def __init__(self, name, group, image, out_dtype, launch, cpu, kernel, flops=0.0, out_shape=None):
         self.name, self.group, self.image, self.out_dtype = name, group, image, out_dtype
         self.launch, self.cpu, self.kernel, self.flops = launch, cpu, kernel, flops
+        # out_shape(h, w, c) -> (h', w') when the output's size differs from the input's
+        self.out_shape = out_shape
+
+    def out_key(self):
+        return (self.image, np.dtype(self.out_dtype).str, self.name if self.out_shape else "")
+
+    def out_image(self, ipf, src):
+        if self.out_shape is None:
+            return ipf.DeviceImage.like(src, self.out_dtype)
+        h, w = self.out_shape(src.height, src.width, src.channels)
+        return ipf.DeviceImage(h, w, src.channels, self.out_dtype)
+
+    def out_size(self, im):
+        if self.out_shape is None:
+            return im.size
+        h, w = self.out_shape(*im.shape[:2], im.shape[2] if im.ndim == 3 else 1)
+        return h * w * (im.shape[2] if im.ndim == 3 else 1)
 
 
 def _c2():
@@ -229,7 +249,22 @@ def _deriv():
                         "adaptiveThreshold mean/gaussian 11 on 3840x2160x1 uint8"}
 
 
-WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None, "deriv": _deriv}
+def _pyr():
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.polygon_image()
+    u8 = np.uint8
+    calls = [
+        Call("pyr_down", "pyr_down", "img", u8, lambda lib, s, d, st: lib.ipfg_pyr_down(s, d, st),
+             lambda o, im: o.pyr_down(im), "pyr_down_kernel<unsigned char>",
+             out_shape=lambda h, w, c: ((h + 1) // 2, (w + 1) // 2)),
+        Call("pyr_up", "pyr_up", "img", u8, lambda lib, s, d, st: lib.ipfg_pyr_up(s, d, st),
+             lambda o, im: o.pyr_up(im), "pyr_up_kernel<unsigned char>", out_shape=lambda h, w, c: (2 * h, 2 * w)),
+    ]
+    return {"images": {"img": img}, "calls": calls, "dtype": "f64",
+            "workload": "pyrDown + pyrUp, 3840x2160x3 uint8 polygons + noise (pixels counted at the input)"}
+
+
+WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None, "deriv": _deriv, "pyr": _pyr}
 
 C5_BUDGET = 256 << 20  # device-memory budget of the batched server (>= the largest request, 2 x 67 MB)
 C5_MAX_BATCH = 256
@@ -498,8 +533,7 @@ def main():
     set_bytes = sum(im.nbytes * 2 for im in wd["images"].values())
     nsets = max(3, math.ceil(300e6 / set_bytes))
     sets = [{k: ipf.DeviceImage.from_array(im, stream) for k, im in wd["images"].items()} for _ in range(nsets)]
-    outs = [{(c.image, np.dtype(c.out_dtype).str): ipf.DeviceImage.like(s[c.image], c.out_dtype) for c in wd["calls"]}
-            for s in sets]
+    outs = [{c.out_key(): c.out_image(ipf, s[c.image]) for c in wd["calls"]} for s in sets]
     calls = wd["calls"]
     ev = HipEvents()
     evs = ev.make(2 * len(calls))
@@ -509,7 +543,7 @@ def main():
     def run(c):
         nonlocal k
         s = sets[k % nsets][c.image].descriptor()
-        d = outs[k % nsets][(c.image, np.dtype(c.out_dtype).str)].descriptor()
+        d = outs[k % nsets][c.out_key()].descriptor()
         ipf._lib.check(c.launch(lib, C.byref(s), C.byref(d), stream.handle))
         k += 1
 
@@ -556,7 +590,7 @@ def main():
     per_filter = {}
     for c in calls:
         im = wd["images"][c.image]
-        g = per_filter.setdefault(c.group, {"ms": [], "bytes": im.nbytes + im.size * np.dtype(c.out_dtype).itemsize,
+        g = per_filter.setdefault(c.group, {"ms": [], "bytes": im.nbytes + c.out_size(im) * np.dtype(c.out_dtype).itemsize,
                                             "px": im.shape[0] * im.shape[1], "flops": c.flops, "kernel": c.kernel})
         g["ms"] += per_call[c.name]
     summary = {}

@@ -22,6 +22,7 @@ __all__ = [
     "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
     "sobel", "scharr", "prewitt", "laplacian", "magnitude", "phase", "gradient_magnitude", "unsharp_mask",
     "sharpen", "emboss", "adaptive_threshold", "motion_blur_kernel", "AdaptiveMethod", "ThresholdType",
+    "pyr_down", "pyr_up", "build_pyramid",
 ]
 
 
@@ -389,6 +390,46 @@ def emboss(src, precision=Precision.DEFAULT, stream: Stream | None = None):
     s, d, holder = _io(src, None)
     _lib.check(_lib.load().ipfg_emboss(C.byref(s), C.byref(d), int(precision), _sh(stream)))
     return _ret(holder)
+
+
+def _pyr(src, up: bool, stream):
+    """One pyramid step into a new image of the step's size (same type and channels)."""
+    if isinstance(src, DeviceImage):
+        h, w = (2 * src.height, 2 * src.width) if up else ((src.height + 1) // 2, (src.width + 1) // 2)
+        dst = DeviceImage(h, w, src.channels, src.dtype)
+        s, d, holder = src.descriptor(), dst.descriptor(), dst
+    else:
+        a = np.asarray(src)
+        if not _rows_contiguous(a):
+            a = np.ascontiguousarray(a)
+        h, w = (2 * a.shape[0], 2 * a.shape[1]) if up else ((a.shape[0] + 1) // 2, (a.shape[1] + 1) // 2)
+        out = np.empty((h, w) + a.shape[2:], a.dtype)
+        s, d, holder = _desc(a), _desc(out), (out, a)
+    fn = _lib.load().ipfg_pyr_up if up else _lib.load().ipfg_pyr_down
+    _lib.check(fn(C.byref(s), C.byref(d), _sh(stream)))
+    return _ret(holder)
+
+
+def pyr_down(src, stream: Stream | None = None):
+    """pyrDown (geometry.hpp:501-512): binomial 5-tap blur, every other row and column kept."""
+    return _pyr(src, False, stream)
+
+
+def pyr_up(src, stream: Stream | None = None):
+    """pyrUp (geometry.hpp:514-524): zero insertion to 2w x 2h, binomial kernel x 4."""
+    return _pyr(src, True, stream)
+
+
+def build_pyramid(src, levels: int, stream: Stream | None = None):
+    """buildPyramid (geometry.hpp:526-535): level 0 is src, then pyrDown while both sides exceed 1."""
+    if levels < 1:
+        raise IpfError("ipf: buildPyramid: levels must be >= 1")
+    pyr = [src]
+    def size(im):
+        return (im.width, im.height) if isinstance(im, DeviceImage) else (np.asarray(im).shape[1], np.asarray(im).shape[0])
+    while len(pyr) < levels and size(pyr[-1])[0] > 1 and size(pyr[-1])[1] > 1:
+        pyr.append(pyr_down(pyr[-1], stream))
+    return pyr
 
 
 def adaptive_threshold(src, maxval, method, ttype, block_size, c, precision=Precision.DEFAULT,

@@ -47,6 +47,11 @@ struct BoxLaunch {
 };
 hipError_t launch_box(const BoxLaunch& a, hipStream_t s);
 
+// ---------------------------------------------------------------- pyramids
+// pyrDown (dst (w + 1) / 2 x (h + 1) / 2) or pyrUp (dst 2w x 2h) of src, same
+// pixel type and channels; every output bit-identical to the reference.
+hipError_t launch_pyr(const DevImage& src, const DevImage& dst, bool up, hipStream_t s);
+
 // ---------------------------------------------------------------- filter2D
 struct F2dLaunch {
     DevImage src, dst;

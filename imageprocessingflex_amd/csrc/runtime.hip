@@ -436,8 +436,10 @@ void check_image(const ipfg_image* im, const char* what) {
 // A call's source and destination as device images plus the copy-back step.
 class Staged {
 public:
-    Staged(const ipfg_image* src, const ipfg_image* dst, CallScratch& sc) : src_(*src), dst_(*dst), sc_(sc) {
-        abi_check(src->width == dst->width && src->height == dst->height && src->channels == dst->channels,
+    // any_shape: dst's size is checked by the caller (pyramid steps).
+    Staged(const ipfg_image* src, const ipfg_image* dst, CallScratch& sc, bool any_shape = false)
+        : src_(*src), dst_(*dst), sc_(sc) {
+        abi_check(any_shape || (src->width == dst->width && src->height == dst->height && src->channels == dst->channels),
                   "dst must have the shape of src");
         sync_ = !src->on_device || !dst->on_device;
         ds_ = to_device(src_, true);
@@ -1355,6 +1357,27 @@ void polar_impl(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* ds
     io.finish();
 }
 
+// pyrDown / pyrUp (geometry.hpp:501-524): one launch, bit-identical for
+// every pixel type (pyramid.hip).
+void pyr_impl(const ipfg_image* src, const ipfg_image* dst, bool up, ipfg_stream stream) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    const char* name = up ? "pyrUp" : "pyrDown";
+    abi_check(src->dtype == dst->dtype, (std::string(name) + ": dst must have the pixel type of src").c_str());
+    abi_check(!up || (src->width <= 0x3fffffff && src->height <= 0x3fffffff), "pyrUp: image too large");
+    const int w = up ? 2 * src->width : (src->width + 1) / 2;
+    const int h = up ? 2 * src->height : (src->height + 1) / 2;
+    abi_check(dst->width == w && dst->height == h && dst->channels == src->channels,
+              (std::string(name) + (up ? ": dst must be 2w x 2h" : ": dst must be (w + 1) / 2 x (h + 1) / 2")).c_str());
+    ensure_init();
+    if (static_cast<long long>(w) * h == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc, true);
+    HIPCALL(launch_pyr(io.src(), io.dst(), up, s));
+    io.finish();
+}
+
 // unsharpMask (filter.hpp:472-483). Integer images outside EXACT: uint8 with
 // windows up to 13 taps in one launch (unsharp.hip), other sizes and types an
 // FP32 blur plus the certified combine; float images and EXACT: the
@@ -2246,6 +2269,14 @@ int ipfg_magnitude(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image*
 
 int ipfg_phase(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, int degrees, ipfg_stream stream) {
     return guarded([&] { polar_impl(gx, gy, dst, degrees ? 1 : 2, stream); });
+}
+
+int ipfg_pyr_down(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream) {
+    return guarded([&] { pyr_impl(src, dst, false, stream); });
+}
+
+int ipfg_pyr_up(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream) {
+    return guarded([&] { pyr_impl(src, dst, true, stream); });
 }
 
 int ipfg_unsharp_mask(const ipfg_image* src, const ipfg_image* dst, double sigma, double amount, double threshold,

@@ -209,6 +209,17 @@ int ipfg_emboss(const ipfg_image* src, const ipfg_image* dst, int precision, ipf
 int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type,
                             int block_size, double c, int precision, ipfg_stream stream);
 
+/* ---------------------------------------------------------------- pyramids
+ * pyrDown(src), geometry.hpp:501-512: the 5-tap binomial filter (Reflect101)
+ * sampled at even (x, y); dst is (w + 1) / 2 x (h + 1) / 2.
+ * pyrUp(src), geometry.hpp:514-524: the zero-inserted 2w x 2h image filtered
+ * with the binomial kernel x 4; dst is 2w x 2h.
+ * Same pixel type and channels as src, any pixel type, bit-identical to the
+ * reference. buildPyramid (geometry.hpp:526-535) is a loop of pyrDown calls in
+ * the front ends. */
+int ipfg_pyr_down(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream);
+int ipfg_pyr_up(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream);
+
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference (its filters run one image per call).
  * A batch runs many uint8 images (1, 3 or 4 channels, any sizes) through

@@ -26,6 +26,7 @@
 //   unsharp/sharpen/emboss .. include/ipf/filter.hpp:472-496
 //   motion-blur kernel ...... include/ipf/filter.hpp:498-516
 //   adaptive threshold ...... include/ipf/filter.hpp:545-565
+//   pyrDown / pyrUp ......... include/ipf/geometry.hpp:501-524
 //
 // Pinning: every function here is pinned by tests/test_oracle.py against the
 // reference-built fixtures in tests/golden/ and against closed forms, brute force
@@ -752,6 +753,54 @@ int ipfo_unsharp(const void* src, int w, int h, int cn, int dtype, void* dst, do
                 const double v = s[i], diff = v - blurred[i];
                 d[i] = sat<T>(std::fabs(diff) > threshold ? v + amount * diff : v);
             }
+            return 0;
+        });
+    });
+}
+
+// pyrDown (geometry.hpp:501-512): the separable binomial filter {1, 4, 6, 4, 1}
+// / 16 into a double image (Reflect101, delta 0), then saturate_cast of the
+// samples at even (x, y) into a (w + 1) / 2 x (h + 1) / 2 image.
+int ipfo_pyr_down(const void* src, int w, int h, int cn, int dtype, void* dst) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const double k[5] = {1 / 16.0, 4 / 16.0, 6 / 16.0, 4 / 16.0, 1 / 16.0};
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            std::vector<double> blurred(static_cast<size_t>(w) * h * cn);
+            sep_filter<T, double>(static_cast<const T*>(src), w, h, cn, blurred.data(), k, 5, k, 5, 2, 2,
+                                  B_REFLECT101, 0.0, 0.0);
+            const int W2 = (w + 1) / 2, H2 = (h + 1) / 2;
+            T* out = static_cast<T*>(dst);
+            for (int y = 0; y < H2; ++y)
+                for (int x = 0; x < W2; ++x)
+                    for (int c = 0; c < cn; ++c)
+                        out[(static_cast<size_t>(y) * W2 + x) * cn + c] =
+                            sat<T>(blurred[(static_cast<size_t>(2 * y) * w + 2 * x) * cn + c]);
+            return 0;
+        });
+    });
+}
+
+// pyrUp (geometry.hpp:514-524): a 2w x 2h double image of zeros with sample
+// (x, y) at (2x, 2y), filtered with {1, 4, 6, 4, 1} / 8 (Reflect101, delta 0)
+// and rounded once into the source type.
+int ipfo_pyr_up(const void* src, int w, int h, int cn, int dtype, void* dst) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        const double k[5] = {1 / 8.0, 4 / 8.0, 6 / 8.0, 4 / 8.0, 1 / 8.0};
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const int W2 = 2 * w, H2 = 2 * h;
+            std::vector<double> up(static_cast<size_t>(W2) * H2 * cn, 0.0);
+            const T* in = static_cast<const T*>(src);
+            for (int y = 0; y < h; ++y)
+                for (int x = 0; x < w; ++x)
+                    for (int c = 0; c < cn; ++c)
+                        up[(static_cast<size_t>(2 * y) * W2 + 2 * x) * cn + c] =
+                            static_cast<double>(in[(static_cast<size_t>(y) * w + x) * cn + c]);
+            sep_filter<double, T>(up.data(), W2, H2, cn, static_cast<T*>(dst), k, 5, k, 5, 2, 2, B_REFLECT101, 0.0,
+                                  0.0);
             return 0;
         });
     });

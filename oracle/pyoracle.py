@@ -83,6 +83,8 @@ class Oracle:
             "unsharp": (_I, [_V, _I, _I, _I, _I, _V, _D, _D, _D]),
             "sharpen": (_I, [_V, _I, _I, _I, _I, _V, _D]),
             "emboss": (_I, [_V, _I, _I, _I, _I, _V]),
+            "pyr_down": (_I, [_V, _I, _I, _I, _I, _V]),
+            "pyr_up": (_I, [_V, _I, _I, _I, _I, _V]),
             "adaptive_threshold": (_I, [_V, _I, _I, _I, _V, _D, _I, _I, _I, _D]),
             "motion_kernel": (_I, [_I, _D, _PF, _I]),
         }
@@ -222,6 +224,23 @@ class Oracle:
 
     def emboss(self, src):
         return self._same_op("emboss", src)
+
+    def _pyr(self, name, src, up):
+        src = np.ascontiguousarray(src)
+        h, w = src.shape[:2]
+        cn = src.shape[2] if src.ndim == 3 else 1
+        oh, ow = (2 * h, 2 * w) if up else ((h + 1) // 2, (w + 1) // 2)
+        out = np.empty((oh, ow) + src.shape[2:], src.dtype)
+        self._check(self._f(name)(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data))
+        return out
+
+    def pyr_down(self, src):
+        """pyrDown (geometry.hpp:501-512)."""
+        return self._pyr("pyr_down", src, False)
+
+    def pyr_up(self, src):
+        """pyrUp (geometry.hpp:514-524)."""
+        return self._pyr("pyr_up", src, True)
 
     def adaptive_threshold(self, src, maxval, method, ttype, block, c):
         src = np.ascontiguousarray(src)

@@ -1,12 +1,13 @@
 // TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
 //
 // C-ABI wrapper around the reference ipf headers, compiled by oracle/Makefile
-// from the reference source tree (include/ipf/filter.hpp, core.hpp,
-// parallel.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
+// from the reference source tree (include/ipf/filter.hpp, geometry.hpp,
+// core.hpp, parallel.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
 // restatement bit-for-bit against the real reference, (2) write the golden
 // fixtures under tests/golden/, and (3) serve as the "reference" CPU baseline.
 // Signatures match oracle/ipf_oracle.cpp's ipfo_* functions one for one.
 #include <ipf/filter.hpp>
+#include <ipf/geometry.hpp>
 
 #include <chrono>
 #include <cstring>
@@ -246,6 +247,28 @@ int ipfr_sharpen(const void* src, int w, int h, int cn, int dtype, void* dst, do
             using T = std::remove_pointer_t<decltype(tag)>;
             const auto img = wrap<T>(src, w, h, cn);
             unwrap(timed([&] { return ipf::sharpen(img, strength); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_pyr_down(const void* src, int w, int h, int cn, int dtype, void* dst) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::pyrDown(img); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_pyr_up(const void* src, int w, int h, int cn, int dtype, void* dst) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::pyrUp(img); }), dst);
             return 0;
         });
     });

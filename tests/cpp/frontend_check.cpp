@@ -70,6 +70,7 @@ int errors() {
                         [&] { adaptiveThreshold(img, 255, AdaptiveMethod::Gaussian, ThresholdType::Trunc, 5, 0); },
                         "ipf: adaptiveThreshold: type must be Binary or BinaryInv");
     bad += expect_error("motion", [&] { motionBlurKernel(0, 0.0); }, "ipf: motionBlurKernel: length must be >= 1");
+    bad += expect_error("buildPyramid", [&] { buildPyramid(img, 0); }, "ipf: buildPyramid: levels must be >= 1");
     // empty images come back empty without device work; ksize 1 is a copy
     if (!gaussianBlur(ImageU8(0, 5, 3), 2.0).empty()) ++bad;
     if (!(medianBlur(img, 1) == img)) ++bad;
@@ -132,6 +133,17 @@ int run(const std::string& dir) {
     save(dir + "/o19.bin", adaptiveThreshold(gray, 255, AdaptiveMethod::Gaussian, ThresholdType::BinaryInv, 7, 3.0));
     save(dir + "/o20.bin", prewitt(img, 1, 0, BorderMode::Wrap));
     save(dir + "/o21.bin", gradientMagnitude(img));
+    // pyramids (geometry.hpp:501-535), with the known answers of the
+    // reference's test_geometry.cpp:117-129
+    save(dir + "/o22.bin", pyrDown(img));
+    save(dir + "/o23.bin", pyrUp(imgf));
+    const std::vector<ImageU8> pyr = buildPyramid(img, 9);
+    for (size_t i = 0; i < pyr.size(); ++i) save(dir + "/p" + std::to_string(i) + ".bin", pyr[i]);
+    std::printf("LEVELS %zu\n", pyr.size());
+    const ImageU8 c(13, 8, 3, 90);
+    const ImageU8 dn = pyrDown(c), upc = pyrUp(c);
+    if (!(dn.width() == 7 && dn.height() == 4 && dn == ImageU8(7, 4, 3, 90))) return 1;
+    if (!(upc.width() == 26 && upc == ImageU8(26, 16, 3, 90))) return 1;
     std::printf("RUN OK\n");
     return 0;
 }

@@ -33,6 +33,8 @@ def run_oracle(o, case):
     """Runs a fixture case through the oracle (oracle.pyoracle.Oracle)."""
     c, inp, _, ex = case
     p = c["params"]
+    if c["op"] in ("pyr_down", "pyr_up"):
+        return getattr(o, c["op"])(inp)
     out_dt = np.dtype(p["out"]) if "out" in p else None
     b = p["border"]
     if c["op"] == "gaussian":
@@ -84,6 +86,8 @@ def run_gpu(ipf, case, precision=0):
     """Runs a fixture case through libipfg (the imageprocessingflex_amd API)."""
     c, inp, _, ex = case
     p = c["params"]
+    if c["op"] in ("pyr_down", "pyr_up"):
+        return getattr(ipf, c["op"])(inp)
     out_dt = np.dtype(p["out"]) if "out" in p else None
     b = p["border"]
     if c["op"] == "gaussian":

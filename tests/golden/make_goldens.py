@@ -146,6 +146,20 @@ def main():
     for deg in (1, 0):
         add("phase", {"degrees": deg, "border": 3}, gx, ref.phase(gx, gy, bool(deg)), gy=gy)
 
+    # Gaussian pyramid steps (geometry.hpp:501-524), every pixel type, odd and
+    # one-pixel sizes
+    u16_3 = (wl.uniform_f32((11, 14, 3), seed=seed, stream=8) * 65535).astype(np.uint16)
+    i32_1 = (wl.uniform_f32((9, 12), seed=seed, stream=9) * 4e9 - 2e9).astype(np.int32)
+    f64_2 = wl.uniform_f32((7, 10, 2), seed=seed, stream=10).astype(np.float64) * 1e3 - 500.0
+    line = wl.uniform_u8((1, 9), seed=seed, stream=11)
+    column = wl.uniform_u8((6, 1, 3), seed=seed, stream=12)
+    pixel = wl.uniform_u8((1, 1), seed=seed, stream=13)
+    for img, name in ((u8_3, "u8x3"), (u8_1, "u8x1"), (u8_2, "u8x2"), (f32_1, "f32x1"), (f32_3, "f32x3"),
+                      (i16_1, "i16x1"), (u16_3, "u16x3"), (i32_1, "i32x1"), (f64_2, "f64x2"), (tiny, "tiny"),
+                      (line, "line"), (column, "column"), (pixel, "pixel")):
+        add("pyr_down", {"src": name}, img, ref.pyr_down(img))
+        add("pyr_up", {"src": name}, img, ref.pyr_up(img))
+
     np.savez_compressed(OUT / "fixtures.npz", **arrays)
     (OUT / "manifest.json").write_text(json.dumps(cases, indent=0))
 
@@ -187,6 +201,8 @@ def main():
             full[f"c2_gauss{s:g}_{NAMES[b]}_f32"] = sha(ref.gaussian_blur(c2, s, border=b))
     for b in BORDERS:
         full[f"c2_box15_{NAMES[b]}_f32"] = sha(ref.box_filter(c2, 15, 15, True, b))
+    full["c3_pyrdown_u8"] = sha(ref.pyr_down(c3))
+    full["c3_pyrup_u8"] = sha(ref.pyr_up(c3))
     (OUT / "fullsize.json").write_text(json.dumps(full, indent=1))
     print(f"{len(cases)} cases, {sum(a.nbytes for a in arrays.values()) / 1e6:.2f} MB raw")
 

@@ -59,6 +59,16 @@ def test_motion_blur_kernel_known_answers():
         ipf.motion_blur_kernel(0, 10.0)
 
 
+def _raw_pyr(name, src, dst):
+    """A pyramid step through the C ABI with a caller-chosen destination."""
+    import ctypes as C
+
+    import imageprocessingflex_amd as ipf
+    from imageprocessingflex_amd import _lib
+    s, d = ipf._desc(src), ipf._desc(dst)
+    _lib.check(getattr(_lib.load(), name)(C.byref(s), C.byref(d), None))
+
+
 def test_argument_errors_precede_device_work():
     import imageprocessingflex_amd as ipf
     img = np.zeros((4, 5), np.uint8)
@@ -87,6 +97,12 @@ def test_argument_errors_precede_device_work():
          "ipf: adaptiveThreshold: expected a single-channel image"),
         (lambda: ipf.adaptive_threshold(img, 255, 0, 0, 1, 0.0), "ipf: adaptiveThreshold: blockSize must be odd and >= 3"),
         (lambda: ipf.adaptive_threshold(img, 255, 1, 2, 5, 0.0), "ipf: adaptiveThreshold: type must be Binary or BinaryInv"),
+        (lambda: ipf.build_pyramid(img, 0), "ipf: buildPyramid: levels must be >= 1"),
+        (lambda: _raw_pyr("ipfg_pyr_down", img, np.zeros((3, 3), np.uint8)),
+         "ipfg: pyrDown: dst must be (w + 1) / 2 x (h + 1) / 2"),
+        (lambda: _raw_pyr("ipfg_pyr_up", img, np.zeros((8, 10), np.uint16)),
+         "ipfg: pyrUp: dst must have the pixel type of src"),
+        (lambda: _raw_pyr("ipfg_pyr_up", img, np.zeros((8, 10, 2), np.uint8)), "ipfg: pyrUp: dst must be 2w x 2h"),
     ]
     for fn, msg in cases:
         with pytest.raises(ipf.IpfError) as e:

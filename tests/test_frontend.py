@@ -69,6 +69,14 @@ def test_frontend_results_match_oracle(exe, oracle, tmp_path):
     assert np.array_equal(out(20, np.float32, u8.shape), oracle.prewitt(u8, 1, 0, 4))
     assert np.array_equal(out(21, np.float32, u8.shape),
                           oracle.magnitude(oracle.sobel(u8, 1, 0), oracle.sobel(u8, 0, 1)))
+    # pyramids: 97 x 61 -> 49 x 31 -> 25 x 16 -> 13 x 8 -> 7 x 4 -> 4 x 2 -> 2 x 1 (stops: height 1)
+    assert np.array_equal(out(22, np.uint8, (31, 49, 3)), oracle.pyr_down(u8))
+    assert np.array_equal(out(23, np.float32, (118, 166)), oracle.pyr_up(f))
+    assert "LEVELS 7" in r.stdout
+    level = u8
+    for i in range(7):
+        assert np.array_equal(np.fromfile(tmp_path / f"p{i}.bin", np.uint8).reshape(level.shape), level), i
+        level = oracle.pyr_down(level)
 
 
 def test_frontend_motion_blur_kernel_known_answers(exe):

@@ -303,3 +303,87 @@ def test_oracle_equals_reference_live(oracle, ref, seed):
                 for k in (3, 5, 9):
                     assert np.array_equal(oracle.median(u8, k, b), ref.median(u8, k, b))
                     assert np.array_equal(oracle.median(f, k, b), ref.median(f, k, b))
+
+
+# ------------------------------------------------------------------- pyramids
+def _r101(p, n):
+    """borderIndex(p, n, Reflect101) (core.hpp:187-214) by repeated reflection."""
+    if n == 1:
+        return 0
+    while p < 0 or p >= n:
+        p = -p if p < 0 else 2 * n - 2 - p
+    return p
+
+
+def _sat_ratio(num, den, dt):
+    """saturate_cast<T>(num / den) for integers: half away from zero, clamped (exact)."""
+    q = (2 * abs(num) + den) // (2 * den)
+    v = q if num >= 0 else -q
+    info = np.iinfo(dt)
+    return min(max(v, int(info.min)), int(info.max))
+
+
+def _pyr_brute(a, up, phase=0):
+    """pyrDown / pyrUp of an integer image as exact integer sums of the binomial
+    weights {1, 4, 6, 4, 1} (products over rows and columns), then one rounding
+    of sum / 256 (down) or sum / 64 (up). phase shifts the sampled positions
+    (a planted mistake for the sensitivity check)."""
+    a3 = a.reshape(a.shape[0], a.shape[1], -1)
+    h, w, cn = a3.shape
+    b = (1, 4, 6, 4, 1)
+    if up:
+        H, W = 2 * h, 2 * w
+        U = np.zeros((H, W, cn), np.int64)
+        U[0::2, 0::2] = a3
+        src, den = U, 64
+    else:
+        H, W = (h + 1) // 2, (w + 1) // 2
+        src, den = a3.astype(np.int64), 256
+    sh, sw = src.shape[:2]
+    out = np.zeros((H, W, cn), a.dtype)
+    for y in range(H):
+        for x in range(W):
+            for c in range(cn):
+                s = 0
+                for j in range(5):
+                    for i in range(5):
+                        yy = (y if up else 2 * y) + j - 2 + phase
+                        xx = (x if up else 2 * x) + i - 2 + phase
+                        s += b[j] * b[i] * int(src[_r101(yy, sh), _r101(xx, sw), c])
+                out[y, x, c] = _sat_ratio(s, den, a.dtype)
+    return out.reshape((H, W) + a.shape[2:])
+
+
+def test_pyramids_equal_exact_integer_sums(oracle):
+    # The binomial weights are dyadic, so for integer images the reference's FP64
+    # sums are exact and its result is the integer sum / 256 (or / 64) rounded once.
+    rng = np.random.default_rng(77)
+    for shape in ((1, 1), (1, 3), (2, 1, 2), (5, 6, 3), (7, 9), (6, 5, 2)):
+        for dt in (np.uint8, np.uint16, np.int16, np.int32):
+            info = np.iinfo(dt)
+            a = rng.integers(int(info.min), int(info.max), shape, endpoint=True).astype(dt)
+            for up in (False, True):
+                got = oracle.pyr_up(a) if up else oracle.pyr_down(a)
+                assert np.array_equal(got, _pyr_brute(a, up)), (shape, dt, up)
+
+
+def test_pyramid_checks_catch_planted_mistakes(oracle):
+    # A shifted sampling phase changes the result of the brute force definition.
+    a = np.random.default_rng(3).integers(0, 256, (7, 9), dtype=np.uint8)
+    assert not np.array_equal(oracle.pyr_down(a), _pyr_brute(a, False, phase=1))
+    assert not np.array_equal(oracle.pyr_up(a), _pyr_brute(a, True, phase=1))
+
+
+def test_pyramid_known_answers(oracle):
+    # test_geometry.cpp:117-129: a constant image stays constant at both sizes,
+    # and a 64 x 48 pyramid of 4 levels ends at width 8.
+    c = np.full((8, 13, 3), 90, np.uint8)
+    d, u = oracle.pyr_down(c), oracle.pyr_up(c)
+    assert d.shape == (4, 7, 3) and np.all(d == 90)
+    assert u.shape == (16, 26, 3) and np.all(u == 90)
+    level = np.random.default_rng(0).integers(0, 256, (48, 64), dtype=np.uint8)
+    widths = [level.shape[1]]
+    for _ in range(3):
+        level = oracle.pyr_down(level)
+        widths.append(level.shape[1])
+    assert widths == [64, 32, 16, 8]
