@@ -22,7 +22,7 @@ __all__ = [
     "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
     "sobel", "scharr", "prewitt", "laplacian", "magnitude", "phase", "gradient_magnitude", "unsharp_mask",
     "sharpen", "emboss", "adaptive_threshold", "motion_blur_kernel", "AdaptiveMethod", "ThresholdType",
-    "pyr_down", "pyr_up", "build_pyramid",
+    "pyr_down", "pyr_up", "build_pyramid", "ssim",
 ]
 
 
@@ -418,6 +418,32 @@ def pyr_down(src, stream: Stream | None = None):
 def pyr_up(src, stream: Stream | None = None):
     """pyrUp (geometry.hpp:514-524): zero insertion to 2w x 2h, binomial kernel x 4."""
     return _pyr(src, True, stream)
+
+
+def ssim(a, b, return_map: bool = False, stream: Stream | None = None):
+    """ssim (metrics.hpp:46-78): mean SSIM over the valid region; with return_map also
+    the full-size float32 map (a DeviceImage for device inputs, else an ndarray)."""
+    dev = isinstance(a, DeviceImage)
+    if dev != isinstance(b, DeviceImage):
+        raise IpfError("ipfg: ssim: both images must be on the device or both on the host")
+    if dev:
+        da, db = a.descriptor(), b.descriptor()
+        m = DeviceImage(a.height, a.width, a.channels, np.float32) if return_map else None
+        hold = (a, b)
+    else:
+        aa = np.asarray(a)
+        bb = np.asarray(b)
+        aa = aa if _rows_contiguous(aa) else np.ascontiguousarray(aa)
+        bb = bb if _rows_contiguous(bb) else np.ascontiguousarray(bb)
+        da, db = _desc(aa), _desc(bb)
+        m = np.empty(aa.shape, np.float32) if return_map else None
+        hold = (aa, bb)
+    dm = (m.descriptor() if dev else _desc(m)) if return_map else None
+    out = C.c_double(0.0)
+    _lib.check(_lib.load().ipfg_ssim(C.byref(da), C.byref(db), C.byref(dm) if dm is not None else None,
+                                     C.byref(out), _sh(stream)))
+    del hold
+    return (out.value, m) if return_map else out.value
 
 
 def build_pyramid(src, levels: int, stream: Stream | None = None):

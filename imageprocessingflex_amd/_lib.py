@@ -97,6 +97,7 @@ SIGNATURES = {
     "ipfg_adaptive_threshold": (_I, [_IMG, _IMG, _D, _I, _I, _I, _D, _I, _V]),
     "ipfg_pyr_down": (_I, [_IMG, _IMG, _V]),
     "ipfg_pyr_up": (_I, [_IMG, _IMG, _V]),
+    "ipfg_ssim": (_I, [_IMG, _IMG, _IMG, _P(_D), _V]),
     "ipfg_batch_create": (_I, [_P(_V), _I, _I]),
     "ipfg_batch_run": (_I, [_V, _JOB, _I, _V]),
     "ipfg_batch_destroy": (_I, [_V]),

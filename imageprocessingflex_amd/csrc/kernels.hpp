@@ -52,6 +52,24 @@ hipError_t launch_box(const BoxLaunch& a, hipStream_t s);
 // pixel type and channels; every output bit-identical to the reference.
 hipError_t launch_pyr(const DevImage& src, const DevImage& dst, bool up, hipStream_t s);
 
+// ---------------------------------------------------------------- ssim
+// Weights of the 11-tap Gaussian window and the stabilising constants.
+struct SsimTaps {
+    double w[11];
+    double c1, c2;
+};
+// Region the mean is taken over: columns [x0, x1), rows [y0, y1).
+struct SsimRoi {
+    int x0, x1, y0, y1;
+};
+// Number of per-workgroup partial sums launch_ssim writes for a (doubles).
+long long ssim_partials(const DevImage& a);
+// SSIM map of a and b (same shape and type) into map (float samples, row
+// stride map_stride samples; may be null) and the sum of the map over roi
+// into sum[0]; partial holds ssim_partials(a) doubles.
+hipError_t launch_ssim(const DevImage& a, const DevImage& b, const SsimTaps& k, float* map, size_t map_stride,
+                       const SsimRoi& roi, double* partial, double* sum, hipStream_t s);
+
 // ---------------------------------------------------------------- filter2D
 struct F2dLaunch {
     DevImage src, dst;

@@ -1357,6 +1357,55 @@ void polar_impl(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* ds
     io.finish();
 }
 
+// ssim (metrics.hpp:46-78): the map in one launch (bit-identical to the
+// reference's), its sum over the valid region in a second; the call blocks
+// until the mean is on the host, as the reference returns it.
+void ssim_impl(const ipfg_image* a, const ipfg_image* b, const ipfg_image* map, double* result, ipfg_stream stream) {
+    check_image(a, "null image a");
+    check_image(b, "null image b");
+    abi_check(result != nullptr, "ssim: null result");
+    abi_check(a->dtype == b->dtype, "ssim: images must have the same pixel type");
+    const bool same = a->width == b->width && a->height == b->height && a->channels == b->channels;
+    require(same && static_cast<long long>(a->width) * a->height > 0,
+            "ssim: images must be non-empty and have the same shape");
+    if (map) {
+        check_image(map, "null ssim map image");
+        abi_check(map->dtype == IPFG_F32 && map->width == a->width && map->height == a->height &&
+                      map->channels == a->channels,
+                  "ssim: the map must be a float32 image of the inputs' shape");
+    }
+    ensure_init();
+    // PixelTraits<T>::max() (core.hpp:57-64): 1 for float images, 255 for int32.
+    const double L = a->dtype == IPFG_U8 ? 255.0 : a->dtype == IPFG_U16 ? 65535.0 : a->dtype == IPFG_I16 ? 32767.0
+                     : a->dtype == IPFG_I32 ? 255.0 : 1.0;
+    SsimTaps k;
+    const std::vector<double> g = gaussian_kernel(1.5, 11);
+    for (int i = 0; i < 11; ++i) k.w[i] = g[static_cast<size_t>(i)];
+    k.c1 = (0.01 * L) * (0.01 * L);
+    k.c2 = (0.03 * L) * (0.03 * L);
+    const int r = 5;
+    const bool valid = a->width > 2 * r && a->height > 2 * r;
+    const SsimRoi roi = valid ? SsimRoi{r, a->width - r, r, a->height - r} : SsimRoi{0, a->width, 0, a->height};
+    const double area = static_cast<double>(roi.x1 - roi.x0) * (roi.y1 - roi.y0) * a->channels;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    ipfg_image none{};
+    none.on_device = 1;
+    Staged ia(a, map ? map : &none, sc, true), ib(b, &none, sc, true);
+    const DevImage& da = ia.src();
+    const long long np = ssim_partials(da);
+    double* partial = static_cast<double*>(sc.alloc(static_cast<size_t>(np + 1) * sizeof(double)));
+    float* dmap = map ? static_cast<float*>(ia.dst().data) : nullptr;
+    const size_t mstride = map ? ia.dst().stride / sizeof(float) : 0;
+    HIPCALL(launch_ssim(da, ib.src(), k, dmap, mstride, roi, partial, partial + np, s));
+    ia.finish();
+    ib.finish();
+    double sum = 0.0;
+    HIPCALL(hipMemcpyAsync(&sum, partial + np, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCALL(hipStreamSynchronize(s));
+    *result = sum / area;
+}
+
 // pyrDown / pyrUp (geometry.hpp:501-524): one launch, bit-identical for
 // every pixel type (pyramid.hip).
 void pyr_impl(const ipfg_image* src, const ipfg_image* dst, bool up, ipfg_stream stream) {
@@ -2269,6 +2318,11 @@ int ipfg_magnitude(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image*
 
 int ipfg_phase(const ipfg_image* gx, const ipfg_image* gy, const ipfg_image* dst, int degrees, ipfg_stream stream) {
     return guarded([&] { polar_impl(gx, gy, dst, degrees ? 1 : 2, stream); });
+}
+
+int ipfg_ssim(const ipfg_image* a, const ipfg_image* b, const ipfg_image* ssim_map, double* result,
+              ipfg_stream stream) {
+    return guarded([&] { ssim_impl(a, b, ssim_map, result, stream); });
 }
 
 int ipfg_pyr_down(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream) {

@@ -220,6 +220,18 @@ int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double
 int ipfg_pyr_down(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream);
 int ipfg_pyr_up(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream);
 
+/* ---------------------------------------------------------------- ssim
+ * ssim(a, b, ssimMap), metrics.hpp:46-78: 11 x 11 Gaussian window (sigma 1.5,
+ * Reflect101), K1 = 0.01, K2 = 0.03, dynamic range PixelTraits<T>::max()
+ * (core.hpp:57-64). *result receives the mean of the map over the valid
+ * region (windows fully inside the image; the whole image when a side is at
+ * most 10). ssim_map may be NULL or a float32 image of a's shape; it receives
+ * the full-size map, bit-identical to the reference's. The mean differs from
+ * the reference's only by summation order (a fixed-order tree sum on the
+ * device). Blocks until *result is written. */
+int ipfg_ssim(const ipfg_image* a, const ipfg_image* b, const ipfg_image* ssim_map, double* result,
+              ipfg_stream stream);
+
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference (its filters run one image per call).
  * A batch runs many uint8 images (1, 3 or 4 channels, any sizes) through

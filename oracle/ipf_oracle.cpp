@@ -758,6 +758,62 @@ int ipfo_unsharp(const void* src, int w, int h, int cn, int dtype, void* dst, do
     });
 }
 
+// ssim (metrics.hpp:46-78): x, y, x*x, y*y, x*y as double images, each
+// blurred by the 11-tap sigma 1.5 Gaussian (Reflect101) through sep_filter,
+// the map as float of the reference's expression, and the mean of the map over
+// the valid region (r = 5) summed in row-major order. L = PixelTraits<T>::max()
+// (core.hpp:57-64).
+int ipfo_ssim(const void* pa, const void* pb, int w, int h, int cn, int dtype, float* map, double* result) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        check(static_cast<long long>(w) * h > 0, "ssim: images must be non-empty and have the same shape");
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const T* a = static_cast<const T*>(pa);
+            const T* b = static_cast<const T*>(pb);
+            double L = 1.0;
+            if constexpr (std::is_integral<T>::value)
+                L = (std::is_signed<T>::value && sizeof(T) >= 4) ? 255.0 : static_cast<double>(std::numeric_limits<T>::max());
+            const double C1 = (0.01 * L) * (0.01 * L), C2 = (0.03 * L) * (0.03 * L);
+            const size_t n = static_cast<size_t>(w) * h * cn;
+            std::vector<double> x(n), y(n), xx(n), yy(n), xy(n);
+            for (size_t i = 0; i < n; ++i) {
+                x[i] = static_cast<double>(a[i]);
+                y[i] = static_cast<double>(b[i]);
+                xx[i] = x[i] * x[i];
+                yy[i] = y[i] * y[i];
+                xy[i] = x[i] * y[i];
+            }
+            double k[11];
+            ipfo_gaussian_kernel_1d(1.5, 11, k, 11);
+            std::vector<double> m[5];
+            const std::vector<double>* in[5] = {&x, &y, &xx, &yy, &xy};
+            for (int t = 0; t < 5; ++t) {
+                m[t].resize(n);
+                sep_filter<double, double>(in[t]->data(), w, h, cn, m[t].data(), k, 11, k, 11, 5, 5, B_REFLECT101,
+                                           0.0, 0.0);
+            }
+            std::vector<float> mp(n);
+            for (size_t i = 0; i < n; ++i) {
+                const double ux = m[0][i], uy = m[1][i];
+                const double vx = m[2][i] - ux * ux, vy = m[3][i] - uy * uy, cxy = m[4][i] - ux * uy;
+                mp[i] = static_cast<float>(((2 * ux * uy + C1) * (2 * cxy + C2)) /
+                                           ((ux * ux + uy * uy + C1) * (vx + vy + C2)));
+            }
+            const int r = 5;
+            const bool valid = w > 2 * r && h > 2 * r;
+            const int x0 = valid ? r : 0, x1 = valid ? w - r : w, y0 = valid ? r : 0, y1 = valid ? h - r : h;
+            double s = 0;
+            for (int yy0 = y0; yy0 < y1; ++yy0)
+                for (int xx0 = x0; xx0 < x1; ++xx0)
+                    for (int c = 0; c < cn; ++c) s += mp[(static_cast<size_t>(yy0) * w + xx0) * cn + c];
+            if (map) std::copy(mp.begin(), mp.end(), map);
+            *result = s / (static_cast<double>(x1 - x0) * (y1 - y0) * cn);
+            return 0;
+        });
+    });
+}
+
 // pyrDown (geometry.hpp:501-512): the separable binomial filter {1, 4, 6, 4, 1}
 // / 16 into a double image (Reflect101, delta 0), then saturate_cast of the
 // samples at even (x, y) into a (w + 1) / 2 x (h + 1) / 2 image.

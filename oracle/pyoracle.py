@@ -85,6 +85,7 @@ class Oracle:
             "emboss": (_I, [_V, _I, _I, _I, _I, _V]),
             "pyr_down": (_I, [_V, _I, _I, _I, _I, _V]),
             "pyr_up": (_I, [_V, _I, _I, _I, _I, _V]),
+            "ssim": (_I, [_V, _V, _I, _I, _I, _I, _V, C.POINTER(_D)]),
             "adaptive_threshold": (_I, [_V, _I, _I, _I, _V, _D, _I, _I, _I, _D]),
             "motion_kernel": (_I, [_I, _D, _PF, _I]),
         }
@@ -241,6 +242,19 @@ class Oracle:
     def pyr_up(self, src):
         """pyrUp (geometry.hpp:514-524)."""
         return self._pyr("pyr_up", src, True)
+
+    def ssim(self, a, b):
+        """ssim (metrics.hpp:46-78): (mean over the valid region, float32 map)."""
+        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+        if a.shape != b.shape or a.dtype != b.dtype:
+            raise OracleError("ssim: images must have the same shape and type")
+        h, w = a.shape[:2]
+        cn = a.shape[2] if a.ndim == 3 else 1
+        m = np.empty(a.shape, np.float32)
+        r = C.c_double(0.0)
+        self._check(self._f("ssim")(a.ctypes.data, b.ctypes.data, w, h, cn, NP2DT[a.dtype], m.ctypes.data,
+                                    C.byref(r)))
+        return r.value, m
 
     def adaptive_threshold(self, src, maxval, method, ttype, block, c):
         src = np.ascontiguousarray(src)

@@ -8,6 +8,7 @@
 // Signatures match oracle/ipf_oracle.cpp's ipfo_* functions one for one.
 #include <ipf/filter.hpp>
 #include <ipf/geometry.hpp>
+#include <ipf/metrics.hpp>
 
 #include <chrono>
 #include <cstring>
@@ -247,6 +248,19 @@ int ipfr_sharpen(const void* src, int w, int h, int cn, int dtype, void* dst, do
             using T = std::remove_pointer_t<decltype(tag)>;
             const auto img = wrap<T>(src, w, h, cn);
             unwrap(timed([&] { return ipf::sharpen(img, strength); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_ssim(const void* pa, const void* pb, int w, int h, int cn, int dtype, float* map, double* result) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto a = wrap<T>(pa, w, h, cn), b = wrap<T>(pb, w, h, cn);
+            ipf::ImageF m;
+            *result = timed([&] { return ipf::ssim(a, b, map ? &m : nullptr); });
+            if (map) unwrap(m, map);
             return 0;
         });
     });

@@ -35,6 +35,8 @@ def run_oracle(o, case):
     p = c["params"]
     if c["op"] in ("pyr_down", "pyr_up"):
         return getattr(o, c["op"])(inp)
+    if c["op"] == "ssim":
+        return o.ssim(inp, ex["b"])[1]
     out_dt = np.dtype(p["out"]) if "out" in p else None
     b = p["border"]
     if c["op"] == "gaussian":
@@ -88,6 +90,8 @@ def run_gpu(ipf, case, precision=0):
     p = c["params"]
     if c["op"] in ("pyr_down", "pyr_up"):
         return getattr(ipf, c["op"])(inp)
+    if c["op"] == "ssim":
+        return ipf.ssim(inp, ex["b"], return_map=True)[1]
     out_dt = np.dtype(p["out"]) if "out" in p else None
     b = p["border"]
     if c["op"] == "gaussian":

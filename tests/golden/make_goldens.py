@@ -160,6 +160,22 @@ def main():
         add("pyr_down", {"src": name}, img, ref.pyr_down(img))
         add("pyr_up", {"src": name}, img, ref.pyr_up(img))
 
+    # ssim (metrics.hpp:46-78): every pixel type, sizes on both sides of the
+    # valid-region rule (a side of at most 10 averages the whole map)
+    def near(img, stream, scale):
+        noise = wl.uniform_f32(img.shape, seed=seed, stream=stream) * 2.0 - 1.0
+        if img.dtype.kind == "f":
+            return (img + noise * scale).astype(img.dtype)
+        info = np.iinfo(img.dtype)
+        return np.clip(img.astype(np.float64) + np.rint(noise * scale), info.min, info.max).astype(img.dtype)
+    for img, name, st, scale in ((u8_3, "u8x3", 20, 40.0), (u8_1, "u8x1", 21, 10.0), (f32_1, "f32x1", 22, 0.2),
+                                 (i16_1, "i16x1", 23, 500.0), (u16_3, "u16x3", 24, 3000.0),
+                                 (i32_1, "i32x1", 25, 1e8), (f64_2, "f64x2", 26, 50.0), (tiny, "tiny", 27, 30.0),
+                                 (line, "line", 28, 30.0), (column, "column", 29, 30.0)):
+        other = near(img, st, scale)
+        value, smap = ref.ssim(img, other)
+        add("ssim", {"src": name, "value": value}, img, smap, b=other)
+
     np.savez_compressed(OUT / "fixtures.npz", **arrays)
     (OUT / "manifest.json").write_text(json.dumps(cases, indent=0))
 

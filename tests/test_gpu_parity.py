@@ -726,3 +726,51 @@ def test_server_rejects_request_above_budget(ipf):
     with pytest.raises(ipf.IpfError, match="larger than the device budget"):
         srv.submit(img, np.empty_like(img), "gaussian", sigma=1.0)
     srv.close()
+
+
+# ------------------------------------------------------------------------ ssim
+def _ssim_tol(n, smap):
+    """Summation-order bound on the mean: the reference's running sum and the device
+    tree sum each stay within (n + 64) * 2^-53 * sum|map| of the exact sum."""
+    return 2.0 * (n + 64) * 2.0 ** -53 * float(np.mean(np.abs(smap.astype(np.float64)))) + 1e-300
+
+
+SSIM_SHAPES = [(1, 1, 1), (3, 40, 1), (11, 11, 3), (10, 30, 2), (37, 53, 4), (130, 257, 1), (271, 97, 3)]
+
+
+@pytest.mark.parametrize("shape", SSIM_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_ssim_map_bit_exact_and_mean_vs_oracle(ipf, oracle, shape):
+    rng = np.random.default_rng(sum(shape) + 5)
+    for a in _pyr_images(rng, shape):
+        if a.dtype.kind == "f":
+            a = np.nan_to_num(a, nan=0.0, posinf=1.0, neginf=-1.0)
+            b = (a + rng.normal(size=shape) * 0.1 * (np.abs(a).max() + 1)).astype(a.dtype)
+        else:
+            info = np.iinfo(a.dtype)
+            b = np.clip(a.astype(np.float64) + rng.normal(size=shape) * 50, info.min, info.max).astype(a.dtype)
+        want_v, want_m = oracle.ssim(a, b)
+        got_v, got_m = ipf.ssim(a, b, return_map=True)
+        assert _same_bits(got_m, want_m), (shape, a.dtype)
+        assert abs(got_v - want_v) <= _ssim_tol(a.size, want_m), (shape, a.dtype, got_v, want_v)
+        assert ipf.ssim(a, b) == got_v
+
+
+def test_ssim_full_size_device_resident(ipf, oracle):
+    # 1920x1080x3 uint8 (the gaussianBlur workload's image) against a noisy copy: map bit-exact, mean within the summation-order bound.
+    from imageprocessingflex_amd import workloads as wl
+    a = wl.uniform_u8((1080, 1920, 3), seed=1, stream=0)
+    b = np.clip(a.astype(np.int16) + (wl.uniform_u8(a.shape, seed=1, stream=1) >> 3) - 16, 0, 255).astype(np.uint8)
+    want_v, want_m = oracle.ssim(a, b)
+    da, db = ipf.DeviceImage.from_array(a), ipf.DeviceImage.from_array(b)
+    got_v, dm = ipf.ssim(da, db, return_map=True)
+    assert _same_bits(dm.download(), want_m)
+    assert abs(got_v - want_v) <= _ssim_tol(a.size, want_m), (got_v, want_v)
+    assert ipf.ssim(da, da) == 1.0
+
+
+def test_ssim_rejects_mismatched_images(ipf):
+    a = np.zeros((12, 12), np.uint8)
+    with pytest.raises(ipf.IpfError, match="ssim: images must be non-empty and have the same shape"):
+        ipf.ssim(a, np.zeros((12, 13), np.uint8))
+    with pytest.raises(ipf.IpfError, match="ssim: images must be non-empty and have the same shape"):
+        ipf.ssim(np.zeros((0, 4), np.uint8), np.zeros((0, 4), np.uint8))

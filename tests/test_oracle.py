@@ -387,3 +387,58 @@ def test_pyramid_known_answers(oracle):
         level = oracle.pyr_down(level)
         widths.append(level.shape[1])
     assert widths == [64, 32, 16, 8]
+
+
+# ------------------------------------------------------------------------ ssim
+def test_ssim_golden_values(oracle):
+    # The golden value is the reference's own mean; the oracle sums in the same order.
+    for c, inp, out, ex in fixtures.load_cases():
+        if c["op"] == "ssim":
+            value, smap = oracle.ssim(inp, ex["b"])
+            assert value == c["params"]["value"], c["id"]
+            assert np.array_equal(smap.view(np.uint32), out.view(np.uint32)), c["id"]
+
+
+def test_ssim_identical_images_is_one(oracle):
+    # With a == b: 2*ux*uy == ux*ux + uy*uy and 2*cxy == vx + vy exactly (doubling is
+    # exact), so every map sample is 1 and so is the mean.
+    rng = np.random.default_rng(3)
+    for shape, dt in (((13, 17, 3), np.uint8), ((12, 12), np.float32), ((5, 30), np.int32), ((14, 11, 2), np.float64)):
+        a = (rng.random(shape) * 200).astype(dt)
+        value, smap = oracle.ssim(a, a)
+        assert value == 1.0 and np.all(smap == 1.0), (shape, dt)
+
+
+def _ssim_direct(a, b, L):
+    """SSIM by direct 2D weighted sums (outer product of the 11 Gaussian weights,
+    Reflect101 by index reflection), FP64 numpy: independent of the separable order."""
+    a3 = a.reshape(a.shape[0], a.shape[1], -1).astype(np.float64)
+    b3 = b.reshape(a3.shape).astype(np.float64)
+    h, w, _ = a3.shape
+    g = np.exp(-0.5 * (np.arange(11) - 5.0) ** 2 / 1.5 ** 2)
+    g /= g.sum()
+    W = np.outer(g, g)
+    ry = np.array([[_r101(y + j - 5, h) for j in range(11)] for y in range(h)])
+    rx = np.array([[_r101(x + i - 5, w) for i in range(11)] for x in range(w)])
+
+    def blur(im):
+        win = im[ry[:, None, :, None], rx[None, :, None, :]]  # h, w, 11, 11, cn
+        return np.einsum("yxjic,ji->yxc", win, W)
+    ux, uy = blur(a3), blur(b3)
+    vx, vy, cxy = blur(a3 * a3) - ux * ux, blur(b3 * b3) - uy * uy, blur(a3 * b3) - ux * uy
+    c1, c2 = (0.01 * L) ** 2, (0.03 * L) ** 2
+    m = ((2 * ux * uy + c1) * (2 * cxy + c2)) / ((ux * ux + uy * uy + c1) * (vx + vy + c2))
+    roi = m[5:-5, 5:-5] if h > 10 and w > 10 else m
+    return roi.mean(), m.reshape(a.shape)
+
+
+def test_ssim_matches_direct_2d_sums(oracle):
+    # Pins the window, constants, border and valid region independently of the
+    # reference; a transposed window or a wrong radius moves the values by far more.
+    rng = np.random.default_rng(9)
+    for shape, dt, L in (((16, 21, 3), np.uint8, 255.0), ((9, 14), np.float32, 1.0), ((23, 12), np.int32, 255.0)):
+        a = (rng.random(shape) * (L if dt != np.float32 else 1.0)).astype(dt)
+        b = np.clip(a.astype(np.float64) + rng.normal(size=shape) * 0.1 * L, 0, L).astype(dt)
+        value, smap = oracle.ssim(a, b)
+        want_v, want_m = _ssim_direct(a, b, L)
+        assert abs(value - want_v) < 1e-6 and np.max(np.abs(smap - want_m)) < 1e-5, (shape, dt)
