@@ -602,6 +602,13 @@ class Server:
         _lib.check(_lib.load().ipfg_server_stats(self.handle, C.byref(b), C.byref(m)))
         return b.value, m.value
 
+    def memory(self):
+        """(bytes held for request buffers, peak bytes in use by admitted requests)."""
+        held = C.c_size_t()
+        peak = C.c_size_t()
+        _lib.check(_lib.load().ipfg_server_memory(self.handle, C.byref(held), C.byref(peak)))
+        return held.value, peak.value
+
     def close(self) -> None:
         if self.handle:
             _lib.check(_lib.load().ipfg_server_destroy(self.handle))

@@ -107,6 +107,7 @@ SIGNATURES = {
     "ipfg_server_poll": (_I, [_V, _P(C.c_longlong), _P(_D), _I, _P(_I)]),
     "ipfg_server_drain": (_I, [_V]),
     "ipfg_server_stats": (_I, [_V, _P(C.c_longlong), _P(_I)]),
+    "ipfg_server_memory": (_I, [_V, _P(C.c_size_t), _P(C.c_size_t)]),
     "ipfg_server_destroy": (_I, [_V]),
 }
 

@@ -18,6 +18,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "arena.hpp"
 #include "batch.hpp"
 #include "kernels.hpp"
 #include "sep_kernels.hpp"
@@ -433,6 +434,17 @@ void check_image(const ipfg_image* im, const char* what) {
     abi_check(im->row_stride_bytes == 0 || im->row_stride_bytes >= packed_stride(*im), "row stride too small");
 }
 
+// Whether two strided views share a byte: each spans from its first byte to
+// the end of its last row ((h - 1) * stride + row bytes).
+bool view_end_overlap(const void* a, size_t as, int ah, size_t arow, const void* b, size_t bs, int bh, size_t brow) {
+    if (ah <= 0 || bh <= 0 || arow == 0 || brow == 0) return false;
+    const char* a0 = static_cast<const char*>(a);
+    const char* a1 = a0 + static_cast<size_t>(ah - 1) * as + arow;
+    const char* b0 = static_cast<const char*>(b);
+    const char* b1 = b0 + static_cast<size_t>(bh - 1) * bs + brow;
+    return a0 < b1 && b0 < a1;
+}
+
 // A call's source and destination as device images plus the copy-back step.
 class Staged {
 public:
@@ -444,13 +456,17 @@ public:
         sync_ = !src->on_device || !dst->on_device;
         ds_ = to_device(src_, true);
         dd_ = to_device(dst_, false);
-        // In-place on the device: read from a copy.
+        // In-place on the device: read from a copy of the view's rows (only
+        // the bytes each row holds: a view's last row may end before its
+        // stride does, at the end of the parent allocation).
         if (src->on_device && dst->on_device && overlaps()) {
-            const size_t rows = static_cast<size_t>(ds_.h);
-            const size_t bytes = rows * ds_.stride;
-            void* copy = sc_.alloc(bytes);
-            HIPCALL(hipMemcpyAsync(copy, ds_.data, bytes, hipMemcpyDeviceToDevice, sc_.stream()));
+            const size_t row = packed_stride(src_);
+            const size_t pitch = (row + 255) & ~static_cast<size_t>(255);
+            void* copy = sc_.alloc(pitch * static_cast<size_t>(ds_.h));
+            HIPCALL(hipMemcpy2DAsync(copy, pitch, ds_.data, ds_.stride, row, static_cast<size_t>(ds_.h),
+                                     hipMemcpyDeviceToDevice, sc_.stream()));
             ds_.data = copy;
+            ds_.stride = pitch;
         }
     }
     const DevImage& src() const { return ds_; }
@@ -491,11 +507,8 @@ private:
         return d;
     }
     bool overlaps() const {
-        const char* a0 = static_cast<const char*>(ds_.data);
-        const char* a1 = a0 + static_cast<size_t>(ds_.h) * ds_.stride;
-        const char* b0 = static_cast<const char*>(dd_.data);
-        const char* b1 = b0 + static_cast<size_t>(dd_.h) * dd_.stride;
-        return a0 < b1 && b0 < a1;
+        return view_end_overlap(ds_.data, ds_.stride, ds_.h, packed_stride(src_), dd_.data, dd_.stride, dd_.h,
+                                packed_stride(dst_));
     }
     ipfg_image src_, dst_;
     CallScratch& sc_;
@@ -1036,11 +1049,9 @@ DevImage to_dev(const ipfg_image& im) {
 }
 
 bool images_overlap(const ipfg_image& a, const ipfg_image& b) {
-    const char* a0 = static_cast<const char*>(a.data);
-    const char* a1 = a0 + (a.row_stride_bytes ? a.row_stride_bytes : packed_stride(a)) * static_cast<size_t>(a.height);
-    const char* b0 = static_cast<const char*>(b.data);
-    const char* b1 = b0 + (b.row_stride_bytes ? b.row_stride_bytes : packed_stride(b)) * static_cast<size_t>(b.height);
-    return a0 < b1 && b0 < a1;
+    const size_t ar = packed_stride(a), br = packed_stride(b);
+    return view_end_overlap(a.data, a.row_stride_bytes ? a.row_stride_bytes : ar, a.height, ar, b.data,
+                            b.row_stride_bytes ? b.row_stride_bytes : br, b.height, br);
 }
 
 // ------------------------------------------------------------------ derivatives and sharpening
@@ -1730,6 +1741,7 @@ public:
             HIPCALL(hipHostMalloc(&host_arena_[b], arena_bytes_, hipHostMallocDefault));
             HIPCALL(hipEventCreateWithFlags(&free_[b], hipEventDisableTiming));
         }
+        HIPCALL(hipEventCreateWithFlags(&last_done_, hipEventDisableTiming));
         queues_ = static_cast<BatchQueue*>(dev_state_);
         fix_.count = reinterpret_cast<unsigned long long*>(static_cast<char*>(dev_state_) + 64);
         fix_.done = reinterpret_cast<unsigned int*>(static_cast<char*>(dev_state_) + 72);
@@ -1749,16 +1761,28 @@ public:
             (void)hipEventDestroy(free_[b]);
             (void)hipHostFree(host_arena_[b]);
         }
+        if (last_armed_) (void)hipEventSynchronize(last_done_);
+        (void)hipEventDestroy(last_done_);
         (void)hipFree(dev_arena_);
         (void)hipFree(dev_state_);
     }
 
     void run(const ipfg_job* jobs, int n, hipStream_t s) {
         abi_check(n >= 0 && n <= max_jobs_, "too many jobs for this batch executor");
+        // Every job is checked and built before anything is acquired, so a
+        // bad job leaves the executor as it was.
+        std::vector<BatchJob> built(static_cast<size_t>(n));
+        std::vector<JobParams> params(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            check_job(jobs[i], true);
+            build_job(jobs[i], built[i], params[i]);
+        }
         const int b = next_;
-        next_ ^= 1;
         if (armed_[b]) HIPCALL(hipEventSynchronize(free_[b]));  // this buffer's previous batch has been copied
         release_params(b, s);
+        // The device arena and tile queues are shared by all runs: a run on
+        // another stream starts after the previous run's kernels.
+        if (last_armed_ && s != last_stream_) HIPCALL(hipStreamWaitEvent(s, last_done_, 0));
         char* h = static_cast<char*>(host_arena_[b]);
         BatchClassHeader* hdr[BCLS_COUNT];
         BatchJob* tab[BCLS_COUNT];
@@ -1767,14 +1791,21 @@ public:
             tab[c] = reinterpret_cast<BatchJob*>(h + per_class_ * c + sizeof(BatchClassHeader));
             *hdr[c] = BatchClassHeader{};
         }
+        struct Held {  // parameter entries of this run; released if the run fails before its launch
+            std::vector<ParamCache::Entry*> v;
+            hipStream_t s;
+            bool armed = true;
+            ~Held() {
+                if (armed)
+                    for (auto* e : v) g_params.release(e, s);
+            }
+        } held{{}, s};
         for (int i = 0; i < n; ++i) {
-            check_job(jobs[i], true);
-            BatchJob J{};
-            JobParams p;
-            build_job(jobs[i], J, p);
+            BatchJob& J = built[i];
+            const JobParams& p = params[i];
             if (!p.blob.empty()) {
                 ParamCache::Entry* e = g_params.acquire(std::string(p.blob.data(), p.blob.size()), s);
-                held_[b].push_back(e);
+                held.v.push_back(e);
                 const char* dev = static_cast<const char*>(e->dev);
                 if (p.cls == BCLS_BILATERAL) {
                     J.spaced = reinterpret_cast<const double*>(dev);
@@ -1796,9 +1827,15 @@ public:
         } else {
             enqueue(b, s);
         }
+        next_ ^= 1;
         HIPCALL(hipEventRecord(free_[b], s));
         armed_[b] = true;
-        for (auto* e : held_[b]) g_params.release(e, s);
+        HIPCALL(hipEventRecord(last_done_, s));
+        last_stream_ = s;
+        last_armed_ = true;
+        held.armed = false;
+        for (auto* e : held.v) g_params.release(e, s);
+        held_[b] = std::move(held.v);
     }
 
 private:
@@ -1848,6 +1885,9 @@ private:
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
     std::vector<ParamCache::Entry*> held_[2];
     int next_ = 0;
+    hipEvent_t last_done_ = nullptr;  // recorded after each run's launches
+    hipStream_t last_stream_ = nullptr;
+    bool last_armed_ = false;
     BatchQueue* queues_ = nullptr;
     FixList fix_{};
     size_t lds_[BCLS_COUNT] = {};
@@ -1858,10 +1898,13 @@ private:
 class Server {
 public:
     Server(size_t budget, int max_batch, bool use_graph) : budget_(budget), max_batch_(max_batch) {
-        abi_check(budget > 0, "device budget must be positive");
+        abi_check(budget >= RangeArena::kAlign, "device budget must be positive");
         abi_check(max_batch > 0 && max_batch <= 65536, "max_batch out of range");
         ensure_init();
         device_ = g_device.load();
+        // Request buffers are ranges of one block of exactly the budget.
+        arena_.reset(budget);
+        HIPCALL(hipMalloc(&arena_base_, arena_.capacity()));
         HIPCALL(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
         for (int i = 0; i < kLanes; ++i) {
             HIPCALL(hipStreamCreateWithFlags(&lane_[i].stream, hipStreamNonBlocking));
@@ -1882,12 +1925,13 @@ public:
             (void)hipStreamDestroy(lane_[i].stream);
         }
         (void)hipStreamDestroy(up_);
+        (void)hipFree(arena_base_);
     }
 
     long long submit(const ipfg_job& r) {
         check_job(r, false);
         const size_t need = device_need(r);
-        abi_check(need <= budget_, "request larger than the device budget");
+        abi_check(need <= arena_.capacity(), "request larger than the device budget");
         std::lock_guard<std::mutex> lock(mu_);
         abi_check(error_.empty(), error_.c_str());
         Req q;
@@ -1926,6 +1970,12 @@ public:
         if (max_jobs) *max_jobs = max_jobs_seen_;
     }
 
+    void memory(size_t* held, size_t* peak) {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (held) *held = arena_.capacity();
+        if (peak) *peak = peak_in_use_;
+    }
+
 private:
     static constexpr int kLanes = 2;  // batches in flight
     struct Req {
@@ -1933,9 +1983,9 @@ private:
         long long id = 0;
         std::chrono::steady_clock::time_point t_submit;
         size_t need = 0;
-        void* dsrc = nullptr;
+        void* dsrc = nullptr;  // server-owned device copies (ranges of the arena)
         void* ddst = nullptr;
-        ScratchPool::Block bsrc{}, bdst{};
+        size_t osrc = RangeArena::npos, odst = RangeArena::npos;
     };
     struct Lane {
         hipStream_t stream = nullptr;
@@ -1945,9 +1995,34 @@ private:
         bool busy = false;
     };
 
+    // Bytes of server-owned device copies: the source of a host request and
+    // the output of a host or server-owned destination.
+    static size_t src_bytes(const ipfg_job& r) { return r.src.on_device ? 0 : image_bytes(r.src); }
+    static size_t dst_bytes(const ipfg_job& r) { return r.dst.on_device && !sink_output(r) ? 0 : image_bytes(r.dst); }
     static size_t device_need(const ipfg_job& r) {
-        return (r.src.on_device ? 0 : image_bytes(r.src)) +
-               (r.dst.on_device && !sink_output(r) ? 0 : image_bytes(r.dst));
+        return RangeArena::rounded(src_bytes(r)) + RangeArena::rounded(dst_bytes(r));
+    }
+
+    // Places the request's buffers in the arena; false (nothing placed) when
+    // they do not fit now. Worker thread only.
+    bool place(Req& q) {
+        const size_t sb = src_bytes(q.job), db = dst_bytes(q.job);
+        const size_t os = arena_.take(sb);
+        if (os == RangeArena::npos) return false;
+        const size_t od = arena_.take(db);
+        if (od == RangeArena::npos) {
+            arena_.give(os, sb);
+            return false;
+        }
+        q.osrc = sb ? os : RangeArena::npos;
+        q.odst = db ? od : RangeArena::npos;
+        return true;
+    }
+
+    void unplace(Req& q) {
+        if (q.osrc != RangeArena::npos) arena_.give(q.osrc, src_bytes(q.job));
+        if (q.odst != RangeArena::npos) arena_.give(q.odst, dst_bytes(q.job));
+        q.osrc = q.odst = RangeArena::npos;
     }
 
     void loop() {
@@ -1962,13 +2037,13 @@ private:
                 {
                     std::unique_lock<std::mutex> lock(mu_);
                     if (stop_ && pending_.empty() && !any_busy()) break;
-                    // admit in arrival order while the budget allows
+                    // admit in arrival order while the buffers fit in the budget
                     while (!pending_.empty() && static_cast<int>(admitted.size()) < max_batch_ &&
-                           in_use_ + pending_.front().need <= budget_) {
-                        in_use_ += pending_.front().need;
+                           place(pending_.front())) {
                         admitted.push_back(pending_.front());
                         pending_.pop_front();
                     }
+                    peak_in_use_ = std::max(peak_in_use_, arena_.in_use());
                     if (admitted.empty() && !progressed) {
                         if (any_busy()) {
                             lock.unlock();
@@ -2021,8 +2096,7 @@ private:
         for (Req& q : reqs) {
             ipfg_job j = q.job;
             if (!j.src.on_device) {
-                q.bsrc = g_scratch.take(image_bytes(j.src), L.stream);
-                q.dsrc = q.bsrc.ptr;
+                q.dsrc = static_cast<char*>(arena_base_) + q.osrc;
                 HIPCALL(hipMemcpy2DAsync(q.dsrc, packed_stride(j.src), j.src.data, host_stride(j.src), packed_stride(j.src),
                                          j.src.height, hipMemcpyHostToDevice, L.stream));
                 j.src.data = q.dsrc;
@@ -2030,8 +2104,7 @@ private:
                 j.src.on_device = 1;
             }
             if (!j.dst.on_device || sink_output(j)) {
-                q.bdst = g_scratch.take(image_bytes(j.dst), L.stream);
-                q.ddst = q.bdst.ptr;
+                q.ddst = static_cast<char*>(arena_base_) + q.odst;
                 j.dst.data = q.ddst;
                 j.dst.row_stride_bytes = packed_stride(q.job.dst);
                 j.dst.on_device = 1;
@@ -2043,8 +2116,6 @@ private:
             if (q.ddst && !q.job.dst.on_device)
                 HIPCALL(hipMemcpy2DAsync(q.job.dst.data, host_stride(q.job.dst), q.ddst, packed_stride(q.job.dst),
                                          packed_stride(q.job.dst), q.job.dst.height, hipMemcpyDeviceToHost, L.stream));
-            if (q.dsrc) g_scratch.give(q.bsrc, L.stream);
-            if (q.ddst) g_scratch.give(q.bdst, L.stream);
         }
         HIPCALL(hipEventRecord(L.done, L.stream));
         L.reqs = std::move(reqs);
@@ -2065,8 +2136,8 @@ private:
             HIPCALL(e);
             const auto now = std::chrono::steady_clock::now();
             std::lock_guard<std::mutex> lock(mu_);
-            for (const Req& q : L.reqs) {
-                in_use_ -= q.need;
+            for (Req& q : L.reqs) {
+                unplace(q);  // the lane's work, copies included, has completed
                 done_.emplace_back(q.id, std::chrono::duration<double, std::milli>(now - q.t_submit).count());
                 --outstanding_;
             }
@@ -2099,7 +2170,9 @@ private:
     std::condition_variable cv_, cv_done_;
     std::deque<Req> pending_;
     std::deque<std::pair<long long, double>> done_;
-    size_t in_use_ = 0;
+    RangeArena arena_;          // placement of request buffers in arena_base_ (worker thread)
+    void* arena_base_ = nullptr;
+    size_t peak_in_use_ = 0;
     long long next_id_ = 0;
     long long outstanding_ = 0;
     long long batches_ = 0;
@@ -2421,6 +2494,13 @@ int ipfg_server_stats(ipfg_server server, long long* batches, int* max_jobs_seen
     return guarded([&] {
         abi_check(server != nullptr, "null server");
         static_cast<Server*>(server)->stats(batches, max_jobs_seen);
+    });
+}
+
+int ipfg_server_memory(ipfg_server server, size_t* held_bytes, size_t* peak_in_use_bytes) {
+    return guarded([&] {
+        abi_check(server != nullptr, "null server");
+        static_cast<Server*>(server)->memory(held_bytes, peak_in_use_bytes);
     });
 }
 

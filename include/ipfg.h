@@ -262,11 +262,14 @@ int ipfg_batch_destroy(ipfg_batch batch);
 
 /* ---------------------------------------------------------------- serving
  * No counterpart in the reference. A request loop with continuous batching:
+ * the server allocates one device block of device_budget_bytes at creation;
  * a worker thread admits submitted requests in arrival order while their
- * device buffers fit in device_budget_bytes, uploads host images, groups the
+ * device buffers can be placed in that block, uploads host images, groups the
  * admitted requests into batches of at most max_batch jobs and keeps two
  * batches in flight on two streams (requests that arrive while a batch runs
- * join the next one), downloads host results and frees the buffers. */
+ * join the next one), downloads host results and returns the buffers' space
+ * to the block when their batch completes. The server never holds more
+ * request-buffer memory than the budget. */
 typedef void* ipfg_server;
 int ipfg_server_create(ipfg_server* server, size_t device_budget_bytes, int max_batch, int use_graph);
 /* Queues a request (the buffers must stay valid until it completes); *id
@@ -285,6 +288,11 @@ int ipfg_server_poll(ipfg_server server, long long* ids, double* latency_ms, int
 int ipfg_server_drain(ipfg_server server);
 /* Batches launched so far and the largest number of jobs in one batch. */
 int ipfg_server_stats(ipfg_server server, long long* batches, int* max_jobs_seen);
+/* Device memory of the server's request buffers: *held_bytes is what the
+ * server holds for them (one block of the budget, rounded down to 256 bytes,
+ * allocated at creation), *peak_in_use_bytes the most any set of admitted
+ * requests occupied at once (never above *held_bytes). Either may be NULL. */
+int ipfg_server_memory(ipfg_server server, size_t* held_bytes, size_t* peak_in_use_bytes);
 int ipfg_server_destroy(ipfg_server server);
 
 #ifdef __cplusplus

@@ -208,9 +208,10 @@ def main():
         for b in (1, 2, 3, 4):
             full[f"c3_median{k}_{NAMES[b]}"] = sha(ref.median(c3, k, b))
     full["c3_bilateral9_75_75_reflect101"] = sha(ref.bilateral(c3, 9, 75.0, 75.0, 3))
-    u8, _ = wl.c4_images()
+    u8, f32 = wl.c4_images()
     k31, _ = wl.c4_kernel()
     full["c4_filter2d31_reflect101_u8"] = sha(ref.filter2d(u8, k31, 3))
+    full["c4_filter2d31_reflect101_f32"] = sha(ref.filter2d(f32, k31, 3))
     c2 = wl.sinusoid_image()
     for s in (1.0, 3.0, 8.0):
         for b in BORDERS:
