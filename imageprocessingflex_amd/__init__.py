@@ -240,6 +240,21 @@ def sep_filter2d(src, kx, ky, border=BorderMode.REFLECT101, delta=0.0, border_va
     return _ret(holder)
 
 
+def sep_filter2d_noalloc(src: "DeviceImage", dst: "DeviceImage", kx, ky, border=BorderMode.REFLECT101, delta=0.0,
+                         border_value=0.0, precision=Precision.DEFAULT, anchor=None, stream: Stream | None = None):
+    """sepFilter2D into an existing device image through ipfg_sep_filter_noalloc
+    (no allocation; at most 128 taps per axis; stream None = the null stream)."""
+    kx = np.ascontiguousarray(kx, np.float64)
+    ky = np.ascontiguousarray(ky, np.float64)
+    ax, ay = anchor if anchor is not None else (len(kx) // 2, len(ky) // 2)
+    s = src.descriptor() if isinstance(src, DeviceImage) else src
+    d = dst.descriptor() if isinstance(dst, DeviceImage) else dst
+    _lib.check(_lib.load().ipfg_sep_filter_noalloc(C.byref(s), C.byref(d), _dptr(kx), len(kx), _dptr(ky), len(ky),
+                                                   ax, ay, int(border), float(delta), float(border_value),
+                                                   int(precision), _sh(stream)))
+    return dst
+
+
 def gaussian_blur(src, sigma_x, sigma_y=0.0, ksize=0, border=BorderMode.REFLECT101, out_dtype=None,
                   precision=Precision.DEFAULT, stream: Stream | None = None):
     """gaussianBlur (filter.hpp:261-267)."""

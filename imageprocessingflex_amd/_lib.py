@@ -80,6 +80,7 @@ SIGNATURES = {
     "ipfg_device_synchronize": (_I, []),
     "ipfg_gaussian_kernel_1d": (_I, [_D, _I, _P(_D), _I]),
     "ipfg_sep_filter": (_I, [_IMG, _IMG, _P(_D), _I, _P(_D), _I, _I, _I, _I, _D, _D, _I, _V]),
+    "ipfg_sep_filter_noalloc": (_I, [_IMG, _IMG, _P(_D), _I, _P(_D), _I, _I, _I, _I, _D, _D, _I, _V]),
     "ipfg_gaussian_blur": (_I, [_IMG, _IMG, _D, _D, _I, _I, _I, _V]),
     "ipfg_box_filter": (_I, [_IMG, _IMG, _I, _I, _I, _I, _D, _I, _V]),
     "ipfg_filter2d": (_I, [_IMG, _IMG, _P(C.c_float), _I, _I, _I, _I, _I, _D, _D, _I, _V]),

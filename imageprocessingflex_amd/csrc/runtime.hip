@@ -25,6 +25,21 @@
 #include "tuner.hpp"
 
 namespace ipfg {
+
+// ------------------------------------------------------------------ no-alloc scratch
+// Device memory of ipfg_sep_filter_noalloc, which allocates nothing: the
+// certification counter, workgroup ticket and fix-up report (bytes 0-23) and
+// the parameter block (from kNoallocParamOff: kx and ky as double, then as
+// float, each followed by 16 zeros). Each GPU holds its own copy until the
+// process ends; every call on a GPU reuses it, so calls that overlap on one
+// GPU race.
+constexpr int kNoallocMaxTaps = 128;
+constexpr size_t kNoallocParamOff = 256;
+constexpr size_t kNoallocBytes = 4096;
+static_assert(kNoallocParamOff + (sizeof(double) + sizeof(float)) * 2 * (kNoallocMaxTaps + 16) <= kNoallocBytes,
+              "parameter block must fit the no-alloc scratch");
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
 namespace {
 
 // ------------------------------------------------------------------ errors
@@ -79,14 +94,23 @@ int guarded(F&& f) {
 std::mutex g_init_mu;
 std::atomic<int> g_device{-1};
 std::atomic<int> g_threads{0};
+constexpr int kMaxDevices = 64;
+void* g_noalloc[kMaxDevices] = {};  // device address of `scratch` per device (under g_init_mu)
 
 void init_device(int device) {
     std::lock_guard<std::mutex> lock(g_init_mu);
     if (g_device.load() == device) return;
     int n = 0;
     HIPCALL(hipGetDeviceCount(&n));
-    abi_check(device >= 0 && device < n, "device index out of range");
+    abi_check(device >= 0 && device < n && device < kMaxDevices, "device index out of range");
     HIPCALL(hipSetDevice(device));
+    // The no-alloc entry point's scratch: its address on this device (host
+    // code uses this address, never the array's name).
+    if (!g_noalloc[device]) {
+        void* p = nullptr;
+        HIPCALL(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc[device] = p;
+    }
     // Keep freed stream-ordered allocations cached: per-call staging and
     // workspace come from this pool.
     hipMemPool_t pool;
@@ -606,21 +630,22 @@ std::vector<double> gaussian_kernel(double sigma, int ksize) {
     return k;
 }
 
-void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky, int kh,
-              int ax, int ay, int border, double delta, double bval, int precision, ipfg_stream stream) {
-    require(kx != nullptr && ky != nullptr && kw > 0 && kh > 0, "separable filter: empty kernel");
-    check_image(src, "null src image");
-    check_image(dst, "null dst image");
-    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
-    abi_check(ax >= 0 && ax < kw && ay >= 0 && ay < kh, "anchor outside kernel");
-    ensure_init();
-    if (static_cast<long long>(src->width) * src->height == 0) return;
-    const hipStream_t s = pick_stream(stream);
-    CallScratch sc(s);
-    Staged io(src, dst, sc);
+// Launch description of a separable call, built the same way by every entry
+// point: the kernel family, the exact-sum level and the certification bound.
+struct SepPlan {
     SepLaunch a{};
-    a.src = io.src();
-    a.dst = io.dst();
+    int exact_level = 0;
+    std::vector<char> blob;  // kx and ky as double, then as float; each + kWeightPad zeros
+    int nx = 0, ny = 0;
+};
+constexpr int kWeightPad = 16;  // the kernels prefetch weights ahead
+
+SepPlan sep_plan(const DevImage& s, const DevImage& d, int src_dt, int dst_dt, int cn, const double* kx, int kw,
+                 const double* ky, int kh, int ax, int ay, int border, double delta, double bval, int precision) {
+    SepPlan p;
+    SepLaunch& a = p.a;
+    a.src = s;
+    a.dst = d;
     a.kw = kw;
     a.kh = kh;
     a.ax = ax;
@@ -628,22 +653,19 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
     a.border = border;
     a.delta = delta;
     a.bval = bval;
-    a.mode = resolve_mode(precision, src->dtype, dst->dtype);
+    a.mode = resolve_mode(precision, src_dt, dst_dt);
     // Exact sums need no certification; exact FP32 sums take the FP32 kernels.
-    const int exact_level = a.mode == MODE_CERTIFY ? exact_sum_level(src->dtype, border, bval, delta,
-                                                                     dyadic_weights(kx, kw), dyadic_weights(ky, kh))
-                                                   : 0;
-    if (exact_level == 3 && sep_fast_fits(src->channels, kw, kh, src->dtype, dst->dtype, false)) a.mode = MODE_FAST;
-    if (a.mode != MODE_EXACT &&
-        !sep_fast_fits(src->channels, kw, kh, src->dtype, dst->dtype, a.mode == MODE_CERTIFY))
+    p.exact_level = a.mode == MODE_CERTIFY ? exact_sum_level(src_dt, border, bval, delta, dyadic_weights(kx, kw),
+                                                             dyadic_weights(ky, kh))
+                                           : 0;
+    if (p.exact_level == 3 && sep_fast_fits(cn, kw, kh, src_dt, dst_dt, false)) a.mode = MODE_FAST;
+    if (a.mode != MODE_EXACT && !sep_fast_fits(cn, kw, kh, src_dt, dst_dt, a.mode == MODE_CERTIFY))
         a.mode = MODE_EXACT;
-    // Parameter block: kx and ky as double, then as float; every array is
-    // followed by kWeightPad zeros (the kernels prefetch weights ahead).
-    constexpr int kWeightPad = 16;
-    const int nx = kw + kWeightPad, ny = kh + kWeightPad;
-    std::vector<char> blob((sizeof(double) + sizeof(float)) * (nx + ny), 0);
-    double* bd = reinterpret_cast<double*>(blob.data());
-    float* bf = reinterpret_cast<float*>(bd + nx + ny);
+    p.nx = kw + kWeightPad;
+    p.ny = kh + kWeightPad;
+    p.blob.assign((sizeof(double) + sizeof(float)) * (p.nx + p.ny), 0);
+    double* bd = reinterpret_cast<double*>(p.blob.data());
+    float* bf = reinterpret_cast<float*>(bd + p.nx + p.ny);
     double sx = 0, sy = 0;
     for (int i = 0; i < kw; ++i) {
         bd[i] = kx[i];
@@ -651,15 +673,10 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         sx += std::fabs(kx[i]);
     }
     for (int j = 0; j < kh; ++j) {
-        bd[nx + j] = ky[j];
-        bf[nx + j] = static_cast<float>(ky[j]);
+        bd[p.nx + j] = ky[j];
+        bf[p.nx + j] = static_cast<float>(ky[j]);
         sy += std::fabs(ky[j]);
     }
-    char* dev = static_cast<char*>(sc.params(blob));
-    a.kxd = reinterpret_cast<const double*>(dev);
-    a.kyd = a.kxd + nx;
-    a.kxf = reinterpret_cast<const float*>(a.kxd + nx + ny);
-    a.kyf = a.kxf + nx;
     if (a.mode == MODE_CERTIFY) {
         // Both the FP64 FMA chains and the reference's multiply/add chains are
         // within (kw + kh + 1) u * sum|ky| sum|kx| X (+ (kh + 1) u |delta|) of
@@ -670,15 +687,69 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         a.bdelta = 2.02 * (kh + 2) * kU64 * std::fabs(delta) + 1e-300;
         if (border == IPFG_BORDER_CONSTANT && static_cast<double>(static_cast<float>(bval)) != bval)
             a.bdelta += 1.01 * std::fabs(static_cast<double>(static_cast<float>(bval)) - bval) * sx * sy;
-        a.xbound = input_bound(src->dtype, bval, border);
-        if (exact_level >= 2) {  // exact sums: only results exactly on a rounding tie are recomputed
+        a.xbound = input_bound(src_dt, bval, border);
+        if (p.exact_level >= 2) {  // exact sums: only results exactly on a rounding tie are recomputed
             a.bcoef = 0.0;
             a.kabs = 0.0;
             a.bdelta = 1e-300;
             a.xbound = 1.0;
         }
-        a.fix = sc.fixlist();
     }
+    return p;
+}
+
+// Points the launch at a device copy of the plan's parameter block.
+void sep_bind(SepPlan& p, const void* dev) {
+    p.a.kxd = static_cast<const double*>(dev);
+    p.a.kyd = p.a.kxd + p.nx;
+    p.a.kxf = reinterpret_cast<const float*>(p.a.kxd + p.nx + p.ny);
+    p.a.kyf = p.a.kxf + p.nx;
+}
+
+// Enqueues the plan: the register stencil for 3 x 3 / 5 x 5 exact FP32 sums
+// with a centred anchor, the separable kernels otherwise.
+hipError_t sep_dispatch(const SepPlan& p, const double* kx, const double* ky, hipStream_t s) {
+    const SepLaunch& a = p.a;
+    const int kw = a.kw, kh = a.kh;
+    hipError_t e = hipErrorNotSupported;
+    if (p.exact_level == 3 && a.mode == MODE_FAST && kw == kh && a.ax == kw / 2 && a.ay == kh / 2 &&
+        (kw == 3 || kw == 5)) {
+        float k2[25], fx[5], fy[5];  // kx[i] * ky[j] and the factors: exact in FP32 at this level
+        for (int j = 0; j < kh; ++j)
+            for (int i = 0; i < kw; ++i) k2[j * kw + i] = static_cast<float>(ky[j] * kx[i]);
+        for (int i = 0; i < kw; ++i) fx[i] = static_cast<float>(kx[i]);
+        for (int j = 0; j < kh; ++j) fy[j] = static_cast<float>(ky[j]);
+        e = launch_stencil_u8(a.src, a.dst, k2, kw, kh, a.ay, a.border, a.delta, a.bval, s, fx, fy);
+    }
+    if (e == hipErrorNotSupported) {
+        (void)hipGetLastError();
+        e = launch_sep(a, s);
+    }
+    return e;
+}
+
+void sep_check_args(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky, int kh,
+                    int ax, int ay, int border) {
+    require(kx != nullptr && ky != nullptr && kw > 0 && kh > 0, "separable filter: empty kernel");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    abi_check(ax >= 0 && ax < kw && ay >= 0 && ay < kh, "anchor outside kernel");
+}
+
+void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky, int kh,
+              int ax, int ay, int border, double delta, double bval, int precision, ipfg_stream stream) {
+    sep_check_args(src, dst, kx, kw, ky, kh, ax, ay, border);
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    SepPlan p = sep_plan(io.src(), io.dst(), src->dtype, dst->dtype, src->channels, kx, kw, ky, kh, ax, ay, border,
+                         delta, bval, precision);
+    SepLaunch& a = p.a;
+    sep_bind(p, sc.params(p.blob));
+    if (a.mode == MODE_CERTIFY) a.fix = sc.fixlist();
     if (a.mode == MODE_EXACT) {
         a.tmp = static_cast<double*>(sc.alloc(sep_exact_tmp_bytes(a.src.w, a.src.h, a.src.cn, kh)));
     } else {
@@ -691,25 +762,11 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
                                 return launch_sep(t, s);
                             });
     }
-    hipError_t e = hipErrorNotSupported;
-    if (exact_level == 3 && a.mode == MODE_FAST && kw == kh && ax == kw / 2 && ay == kh / 2 && (kw == 3 || kw == 5)) {
-        float k2[25], fx[5], fy[5];  // kx[i] * ky[j] and the factors: exact in FP32 at this level
-        for (int j = 0; j < kh; ++j)
-            for (int i = 0; i < kw; ++i) k2[j * kw + i] = static_cast<float>(ky[j] * kx[i]);
-        for (int i = 0; i < kw; ++i) fx[i] = static_cast<float>(kx[i]);
-        for (int j = 0; j < kh; ++j) fy[j] = static_cast<float>(ky[j]);
-        e = launch_stencil_u8(a.src, a.dst, k2, kw, kh, ay, border, delta, bval, s, fx, fy);
-    }
-    if (e == hipErrorNotSupported) {
-        (void)hipGetLastError();
-        e = launch_sep(a, s);
-    }
-    HIPCALL(e);
+    HIPCALL(sep_dispatch(p, kx, ky, s));
     if (a.mode == MODE_CERTIFY) record_fixups(a.fix, s);
     else t_ctx.fix_valid = false;
     io.finish();
 }
-
 
 void box_impl(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int normalize, int border, double bval,
               int precision, ipfg_stream stream) {
@@ -1052,6 +1109,78 @@ bool images_overlap(const ipfg_image& a, const ipfg_image& b) {
     const size_t ar = packed_stride(a), br = packed_stride(b);
     return view_end_overlap(a.data, a.row_stride_bytes ? a.row_stride_bytes : ar, a.height, ar, b.data,
                             b.row_stride_bytes ? b.row_stride_bytes : br, b.height, br);
+}
+
+// ------------------------------------------------------------------ no-alloc separable filter
+// Weights of one axis passed by value and written into the scratch's
+// parameter block (with the zero padding the kernels prefetch).
+struct NoallocTaps {
+    double w[kNoallocMaxTaps];
+    int n;
+};
+
+__global__ void noalloc_taps_kernel(NoallocTaps t, double* __restrict__ d, float* __restrict__ f) {
+    for (int i = threadIdx.x; i < t.n + kWeightPad; i += blockDim.x) {
+        const double v = i < t.n ? t.w[i] : 0.0;
+        d[i] = v;
+        f[i] = static_cast<float>(v);
+    }
+}
+
+void* noalloc_base() {
+    std::lock_guard<std::mutex> lock(g_init_mu);
+    void* p = g_noalloc[g_device.load()];
+    abi_check(p != nullptr, "no-alloc scratch not set up on this device");
+    return p;
+}
+
+// ipfg_sep_filter on device-resident images with no allocation: parameters,
+// counters and the fix-up report live in `scratch`, the strip height is the
+// built-in rule (no measurement launches) and a NULL stream is the device's
+// null stream.
+void sep_noalloc_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky,
+                      int kh, int ax, int ay, int border, double delta, double bval, int precision,
+                      ipfg_stream stream) {
+    sep_check_args(src, dst, kx, kw, ky, kh, ax, ay, border);
+    abi_check(src->on_device && dst->on_device, "the no-alloc separable filter takes device-resident images");
+    abi_check(src->width == dst->width && src->height == dst->height && src->channels == dst->channels,
+              "dst must have the shape of src");
+    if (kw > kNoallocMaxTaps || kh > kNoallocMaxTaps)
+        throw Unsupported{"ipfg: the no-alloc separable filter takes at most 128 taps per axis"};
+    if (precision == IPFG_PREC_EXACT)
+        throw Unsupported{"ipfg: the no-alloc separable filter has no EXACT mode (it needs an FP64 intermediate)"};
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    abi_check(!images_overlap(*src, *dst), "the no-alloc separable filter cannot run in place (dst overlaps src)");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    SepPlan p = sep_plan(to_dev(*src), to_dev(*dst), src->dtype, dst->dtype, src->channels, kx, kw, ky, kh, ax, ay,
+                         border, delta, bval, precision);
+    if (p.a.mode == MODE_EXACT)
+        throw Unsupported{"ipfg: this call needs the exact two-pass path and its FP64 intermediate (types or sizes "
+                          "outside the fast kernels); use ipfg_sep_filter"};
+    char* base = static_cast<char*>(noalloc_base());
+    sep_bind(p, base + kNoallocParamOff);
+    NoallocTaps t{};
+    t.n = kw;
+    std::memcpy(t.w, kx, sizeof(double) * kw);
+    hipLaunchKernelGGL(noalloc_taps_kernel, dim3(1), dim3(64), 0, s, t, const_cast<double*>(p.a.kxd),
+                       const_cast<float*>(p.a.kxf));
+    HIPCALL(hipGetLastError());
+    t = NoallocTaps{};
+    t.n = kh;
+    std::memcpy(t.w, ky, sizeof(double) * kh);
+    hipLaunchKernelGGL(noalloc_taps_kernel, dim3(1), dim3(64), 0, s, t, const_cast<double*>(p.a.kyd),
+                       const_cast<float*>(p.a.kyf));
+    HIPCALL(hipGetLastError());
+    if (p.a.mode == MODE_CERTIFY) {
+        p.a.fix.count = reinterpret_cast<unsigned long long*>(base);
+        p.a.fix.done = reinterpret_cast<unsigned int*>(base + 8);
+        p.a.fix.report = reinterpret_cast<unsigned long long*>(base + 16);
+        HIPCALL(launch_zero_counter(p.a.fix.count, p.a.fix.done, s));  // a kernel, not a memset
+    }
+    p.a.sh = 0;
+    HIPCALL(sep_dispatch(p, kx, ky, s));
+    t_ctx.fix_valid = false;
 }
 
 // ------------------------------------------------------------------ derivatives and sharpening
@@ -2325,6 +2454,13 @@ int ipfg_sep_filter(const ipfg_image* src, const ipfg_image* dst, const double* 
                     int ax, int ay, int border, double delta, double border_value, int precision,
                     ipfg_stream stream) {
     return guarded([&] { sep_impl(src, dst, kx, kw, ky, kh, ax, ay, border, delta, border_value, precision, stream); });
+}
+
+int ipfg_sep_filter_noalloc(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky,
+                            int kh, int ax, int ay, int border, double delta, double border_value, int precision,
+                            ipfg_stream stream) {
+    return guarded(
+        [&] { sep_noalloc_impl(src, dst, kx, kw, ky, kh, ax, ay, border, delta, border_value, precision, stream); });
 }
 
 int ipfg_gaussian_blur(const ipfg_image* src, const ipfg_image* dst, double sigma_x, double sigma_y, int ksize,

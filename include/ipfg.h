@@ -151,6 +151,21 @@ int ipfg_sep_filter(const ipfg_image* src, const ipfg_image* dst, const double* 
                     int kh, int ax, int ay, int border, double delta, double border_value, int precision,
                     ipfg_stream stream);
 
+/* No counterpart in the reference. ipfg_sep_filter for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same arguments and results, but it calls no
+ * hipMalloc, hipFree or other allocating routine. Weights, the certification
+ * counters and the fix-up report live in a fixed 4 KB device array per GPU
+ * (held until the process ends). Limits: device-resident src and dst that do
+ * not overlap, at most 128 taps per axis, precision DEFAULT or FP32 and types
+ * the fast kernels take (IPFG_ERR_UNSUPPORTED otherwise); the strip height is
+ * the built-in rule (no autotuning); a NULL stream is the device's null
+ * stream; ipfg_last_fixups does not report these calls. All calls on one GPU
+ * share the array, so calls that overlap in time on one GPU race. */
+int ipfg_sep_filter_noalloc(const ipfg_image* src, const ipfg_image* dst, const double* kx, int kw, const double* ky,
+                            int kh, int ax, int ay, int border, double delta, double border_value, int precision,
+                            ipfg_stream stream);
+
 /* gaussianBlur (filter.hpp:261-267): sigmaY <= 0 uses sigmaX; ksize <= 0
  * derives the window from sigma. */
 int ipfg_gaussian_blur(const ipfg_image* src, const ipfg_image* dst, double sigma_x, double sigma_y, int ksize,
