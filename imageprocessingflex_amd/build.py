@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -49,9 +50,26 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def _stale(src: Path, newest_header: float) -> bool:
+_INCLUDE = re.compile(r'^\s*#\s*include\s*"([^"]+)"', re.M)
+
+
+def _deps(src: Path, seen: set[Path] | None = None) -> set[Path]:
+    """The source and every header it includes with quotes, recursively."""
+    seen = set() if seen is None else seen
+    if src in seen or not src.exists():
+        return seen
+    seen.add(src)
+    for name in _INCLUDE.findall(src.read_text(errors="replace")):
+        _deps((src.parent / name).resolve(), seen)
+    return seen
+
+
+def _stale(src: Path) -> bool:
     obj = BUILD / (src.stem + ".o")
-    return not obj.exists() or obj.stat().st_mtime < max(src.stat().st_mtime, newest_header)
+    if not obj.exists():
+        return True
+    newest = max(p.stat().st_mtime for p in _deps(src.resolve()))
+    return obj.stat().st_mtime < newest
 
 
 def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> Path:
@@ -62,9 +80,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     BUILD.mkdir(exist_ok=True)
     srcs = sources()
-    headers = list(CSRC.glob("*.hpp")) + [PKG.parent / "include" / "ipfg.h"]
-    newest = max(h.stat().st_mtime for h in headers if h.exists())
-    todo = [s for s in srcs if not incremental or _stale(s, newest)]
+    todo = [s for s in srcs if not incremental or _stale(s)]
     # largest translation units first: they bound the wall time
     todo.sort(key=lambda p: -p.stat().st_size if "_k" not in p.stem else -10**9)
     jobs = jobs or min(max(1, len(todo)), max(1, (os.cpu_count() or 2)))

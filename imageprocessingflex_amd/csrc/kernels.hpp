@@ -30,8 +30,8 @@ struct SepLaunch {
     int sh;       // strip height of the fast kernels (0: built-in choice)
 };
 hipError_t launch_sep(const SepLaunch& a, hipStream_t s);
-// Strip heights worth measuring for an image of height h.
-std::vector<int> sep_sh_candidates(int h);
+// Strip heights (launch shapes) worth measuring for this call.
+std::vector<int> sep_sh_candidates(const SepLaunch& l);
 // The fast separable kernel supports this call (channels, kernel size, types).
 bool sep_fast_fits(int cn, int kw, int kh, int src_dt, int dst_dt, bool cert);
 size_t sep_exact_tmp_bytes(int w, int h, int cn, int kh);

@@ -756,7 +756,7 @@ void sep_impl(const ipfg_image* src, const ipfg_image* dst, const double* kx, in
         const TuneKey key{"sep", src->dtype, dst->dtype, src->channels, kw, kh, a.mode,
                           ceil_log2(src->width), ceil_log2(src->height)};
         a.sh = g_tuner.pick(key, static_cast<long long>(src->width) * src->height * src->channels,
-                            sep_sh_candidates(src->height), s, [&](int v) {
+                            sep_sh_candidates(a), s, [&](int v) {
                                 SepLaunch t = a;
                                 t.sh = v;
                                 return launch_sep(t, s);
@@ -839,7 +839,7 @@ void box_impl(const ipfg_image* src, const ipfg_image* dst, int kw, int kh, int 
         const TuneKey key{"sep", src->dtype, dst->dtype, src->channels, kw, kh, l.mode, ceil_log2(src->width),
                           ceil_log2(src->height)};
         l.sh = g_tuner.pick(key, static_cast<long long>(src->width) * src->height * src->channels,
-                            sep_sh_candidates(src->height), s, [&](int v) {
+                            sep_sh_candidates(l), s, [&](int v) {
                                 SepLaunch t = l;
                                 t.sh = v;
                                 return launch_sep(t, s);

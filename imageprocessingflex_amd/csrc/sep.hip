@@ -25,7 +25,7 @@
 //
 // Exact path: the reference's two passes in FP64 with separate multiply and
 // add (the library is built with -ffp-contract=off), one thread per sample.
-#include "sep_kernels.hpp"
+#include "sep1.hpp"
 
 namespace ipfg {
 namespace {
@@ -89,6 +89,19 @@ int sep_grid(size_t n) {
 
 using namespace sepk;
 
+// The one-channel float32 certified kernel (sep1.hpp): kw = kh odd in
+// [3, 49], centred anchors, float-aligned rows.
+static bool sep1_fits(const DevImage& src, const DevImage& dst, int kw, int kh, int ax, int ay) {
+    return src.cn == 1 && src.dt == IPFG_F32 && dst.dt == IPFG_F32 && kw == kh && kw % 2 == 1 && kw >= 3 &&
+           kw <= 49 && ax == kw / 2 && ay == kh / 2 && src.stride % 4 == 0 && dst.stride % 4 == 0 && src.w > 0 &&
+           src.h > 0;
+}
+
+static hipError_t sep1_launch(const SepArgs& a, hipStream_t s) {
+    if (!sep1_fits(a.src, a.dst, a.kw, a.kh, a.ax, a.ay)) return hipErrorNotSupported;
+    return a.kw <= 25 ? sep1::launch_small(a, s) : sep1::launch_large(a, s);
+}
+
 static bool sep_planar(int cn, int src_dt, int dst_dt) {
     return (cn == 1 || cn == 3 || cn == 4) &&
            ((src_dt == IPFG_U8 && (dst_dt == IPFG_U8 || dst_dt == IPFG_F32)) || (src_dt == IPFG_F32 && dst_dt == IPFG_F32));
@@ -108,8 +121,17 @@ hipError_t launch_zero_counter(unsigned long long* count, unsigned int* done, hi
     return hipGetLastError();
 }
 
-std::vector<int> sep_sh_candidates(int h) {
+std::vector<int> sep_sh_candidates(const SepLaunch& l) {
+    const int h = l.src.h;
     std::vector<int> c;
+    if (l.mode == MODE_CERTIFY && sep1_fits(l.src, l.dst, l.kw, l.kh, l.ax, l.ay)) {
+        // the one-channel float32 kernel: 16-row steps, then 8-row steps
+        for (int v : {128, 256, 512})
+            if (v < 2 * h) c.push_back(v);
+        for (int v : {64, 128, 256})
+            if (v < 2 * h) c.push_back(sep1::kHalfStep + v);
+        return c;
+    }
     for (int v : {16, 32, 48, 64, 96, 128, 192, 256, 384, 512})
         if (v < 2 * h) c.push_back(v);
     return c;
@@ -153,6 +175,7 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     a.SH = l.sh;
     a.oscale = l.oscale != 0.0 ? l.oscale : 1.0;
     a.unit = l.unit;
+    a.kabs = l.kabs;
     const int cn = l.src.cn;
     const bool planar = sep_planar(cn, l.src.dt, l.dst.dt);
     if (planar && l.src.dt == IPFG_U8) {
@@ -169,7 +192,10 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     }
     if (planar) {  // float32 -> float32
         if (cert && cn == 1) {
-            const hipError_t e = plane_f32_k1(a, s);
+            hipError_t e = sep1_launch(a, s);
+            if (e != hipErrorNotSupported) return e;
+            (void)hipGetLastError();
+            e = plane_f32_k1(a, s);
             if (e != hipErrorNotSupported) return e;
             (void)hipGetLastError();
         }

@@ -121,6 +121,7 @@ struct SepArgs {
     int unit;            // every weight is 1 (box): the strip kernel slides its sums
     int SH;
     FixList fix;
+    double kabs;         // sum|kx| * sum|ky| (bounds |result - delta| / X)
 };
 
 template <class Ti, class Acc, int CN, int SJ, bool CERT>

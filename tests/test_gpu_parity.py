@@ -550,7 +550,9 @@ def test_every_strip_height_gives_the_reference_result(ipf, oracle):
             dt = 0 if img.dtype == np.uint8 else 4
             k = 2 * max(1, int(3 * sigma + 0.5)) + 1
             wl, hl = int(np.ceil(np.log2(w))), int(np.ceil(np.log2(h)))
-            for sh in (16, 32, 48, 64, 96, 128, 192, 256, 384, 512):
+            # 16-row steps at any height; for one-channel float32, 8-row steps (1 << 20 + height)
+            extra = (1048576 + 16, 1048576 + 64, 1048576 + 256) if img.ndim == 2 else ()
+            for sh in (16, 32, 48, 64, 96, 128, 192, 256, 384, 512) + extra:
                 ipf.tune_clear()
                 ipf.tune_import(f"sep {dt} {dt} {cn} {k} {k} 1 {wl} {hl} {sh}\n")
                 got = ipf.gaussian_blur(img, sigma)
@@ -774,3 +776,41 @@ def test_ssim_rejects_mismatched_images(ipf):
         ipf.ssim(a, np.zeros((12, 13), np.uint8))
     with pytest.raises(ipf.IpfError, match="ssim: images must be non-empty and have the same shape"):
         ipf.ssim(np.zeros((0, 4), np.uint8), np.zeros((0, 4), np.uint8))
+
+
+# ------------------------------------------------------------- one-channel float32 strip kernel
+@pytest.mark.parametrize("ks", [3, 5, 7, 13, 15, 19, 25, 27, 33, 49])
+def test_float_gray_strip_kernel_edges_vs_oracle(ipf, oracle, ks):
+    # widths on both sides of the 128-column strip, heights below and above the
+    # window, every border (Constant with a non-zero value), a delta, box sums
+    rng = np.random.default_rng(300 + ks)
+    k = oracle.gaussian_kernel_1d(0.0, ks)
+    shapes = [(1, 1), (3, 127), (37, 128), (64, 129), (ks + 5, 255), (131, 257), (260, 384)]
+    for i, shape in enumerate(shapes):
+        img = (rng.random(shape) * 4.0 - 1.0).astype(np.float32)
+        b = i % 5
+        got = ipf.sep_filter2d(img, k, k, b, delta=0.25, border_value=0.75)
+        want = oracle.sep_filter(img, k, k, border=b, delta=0.25, bval=0.75)
+        assert np.array_equal(got, want), (shape, b)
+        got = ipf.box_filter(img, ks, ks, True, b, 0.75)
+        want = oracle.box_filter(img, ks, ks, True, b, 0.75)
+        err = np.abs(got.astype(np.float64) - want)
+        assert np.all(err <= 1e-12 * ks * ks + 2.0 ** -24 * np.abs(want)), (shape, b, err.max())
+
+
+def test_float_gray_strip_kernel_ties_and_specials(ipf, oracle):
+    # dyadic inputs with weights 1/4, 1/2, 1/4 put many results exactly on float
+    # rounding ties (recomputed in the reference's order); inf and NaN inputs
+    rng = np.random.default_rng(310)
+    img = (rng.integers(0, 1 << 20, (300, 333)) * 2.0 ** -20).astype(np.float32)
+    k = np.array([0.25, 0.5, 0.25])
+    for b in range(5):
+        assert np.array_equal(ipf.sep_filter2d(img, k, k, b), oracle.sep_filter(img, k, k, border=b)), b
+    spec = rng.random((140, 150)).astype(np.float32)
+    spec[10, 20] = np.inf
+    spec[70, 140] = -np.inf
+    spec[100, 3] = np.nan
+    kg = oracle.gaussian_kernel_1d(1.5)
+    got = ipf.sep_filter2d(spec, kg, kg, 4)
+    want = oracle.sep_filter(spec, kg, kg, border=4)
+    assert np.array_equal(got, want, equal_nan=True)
