@@ -94,7 +94,8 @@ using namespace sepk;
 static bool sep1_fits(const DevImage& src, const DevImage& dst, int kw, int kh, int ax, int ay) {
     return src.cn == 1 && src.dt == IPFG_F32 && dst.dt == IPFG_F32 && kw == kh && kw % 2 == 1 && kw >= 3 &&
            kw <= 49 && ax == kw / 2 && ay == kh / 2 && src.stride % 4 == 0 && dst.stride % 4 == 0 && src.w > 0 &&
-           src.h > 0;
+           src.h > 0 && static_cast<size_t>(src.h) * src.stride < (size_t(1) << 31) &&
+           static_cast<size_t>(dst.h) * dst.stride < (size_t(1) << 31);
 }
 
 static hipError_t sep1_launch(const SepArgs& a, hipStream_t s) {

@@ -39,6 +39,9 @@ constexpr int kThreads = 256;
 // value minus kHalfStep is the strip height).
 constexpr int kHalfStep = 1 << 20;
 
+constexpr int gcd(int a, int b) { return b == 0 ? a : gcd(b, a % b); }
+constexpr int lcm(int a, int b) { return a / gcd(a, b) * b; }
+
 template <int K, int CH>
 struct Geo {
     static constexpr int P = CH / 2;                       // horizontal outputs per thread
@@ -61,7 +64,18 @@ struct Geo {
     static constexpr size_t ring_bytes = sizeof(double) * R * RP;
     static constexpr size_t lds_bytes = ring_bytes + sizeof(float) * CH * SP;
     static_assert(LR <= 64, "a staged row is one 16-byte load per lane");
+    static constexpr int D = 2;                            // chunks whose loads are in flight
+    static constexpr int U = lcm(D, NR);                   // steps of one unrolled loop turn
 };
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, U - 1>).
+template <int U, int I = 0, class F>
+__device__ __forceinline__ void unroll_steps(F&& f) {
+    if constexpr (I < U) {
+        f(std::integral_constant<int, I>{});
+        unroll_steps<U, I + 1>(f);
+    }
+}
 
 __device__ __forceinline__ float max3abs(float a, float b, float m) {
     return __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(a), __builtin_fabsf(b)), m);
@@ -91,8 +105,12 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
     const int gxs = x0 - G::A - G::S0;  // image column of staged sample 0 (a multiple of 4)
     const bool cst = a.border == IPFG_BORDER_CONSTANT;
     const float bvalf = static_cast<float>(a.bval);
-    const char* __restrict__ src = static_cast<const char*>(a.src.data);
-    const size_t sstride = a.src.stride;
+    // Buffer resources over the source and destination (the launcher checks
+    // both fit 2^31 bytes): row offsets go in scalar registers, lane offsets
+    // stay fixed, so no per-row address arithmetic runs on the vector ALU.
+    const int sstride = static_cast<int>(a.src.stride);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        a.src.data, 0, static_cast<int>(static_cast<size_t>(H) * a.src.stride), 0x00020000);
 
     // ---- staging: wave w loads rows w, w + 4, ... of a chunk; lane l loads
     // segment min(l, LR - 1) (lanes past the row re-read its last segment)
@@ -102,18 +120,25 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
 #pragma unroll
         for (int q = 0; q < 4; ++q) sxe[q] = border_index(gxs + 4 * seg + q, W, a.border);
     }
-    auto src_row = [&](int gy) {  // image row of padded row gy (wave-uniform); Constant: row 0
+    auto src_row = [&](int gy) {  // byte offset of padded row gy (wave-uniform); Constant: row 0
         const int sy = (gy >= 0 && gy < H) ? gy : border_index(gy, H, a.border);
-        return reinterpret_cast<const float*>(src + static_cast<size_t>(max(sy, 0)) * sstride);
+        return max(sy, 0) * sstride;
     };
+    const int voff = (gxs + 4 * seg) * 4;  // interior strips: this lane's 16 bytes of a row
     auto load = [&](float4 (&reg)[RPW], int c) {
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
-            const float* rp = src_row(y0 - G::A + CH * c + wave + 4 * i);
+            const int ro = src_row(y0 - G::A + CH * c + wave + 4 * i);
             if constexpr (HIN) {
-                reg[i] = *reinterpret_cast<const float4*>(rp + gxs + 4 * seg);
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, ro, 0);
+                reg[i] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                                     __uint_as_float(v[3]));
             } else {
-                reg[i] = make_float4(rp[max(sxe[0], 0)], rp[max(sxe[1], 0)], rp[max(sxe[2], 0)], rp[max(sxe[3], 0)]);
+                float e[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    e[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, max(sxe[q], 0) * 4, ro, 0));
+                reg[i] = make_float4(e[0], e[1], e[2], e[3]);
             }
         }
     };
@@ -156,13 +181,18 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
     const int xg = x0 + vcol;
     float* __restrict__ dst = static_cast<float*>(a.dst.data);
     const size_t dstride = a.dst.stride / sizeof(float);
+    const __amdgpu_buffer_rsrc_t wsrc = __builtin_amdgcn_make_buffer_rsrc(
+        a.dst.data, 0, static_cast<int>(static_cast<size_t>(H) * a.dst.stride), 0x00020000);
     const double oscale = a.oscale;
     unsigned int xmax_bits = 0u;
 
-    // Two register sets alternate: the loads of chunk k + 2 are issued when
-    // chunk k has been stored, so each chunk's loads have two steps to land.
-    auto step = [&](float4 (&reg)[RPW], const int k) {
-        const int slot = k % G::NR;
+    // D register sets rotate: the loads of chunk k + D are issued when chunk
+    // k has been stored, so each chunk's loads have D steps to land.
+    // PH = k mod U (U a multiple of the ring period NR and of D): the ring
+    // rows of a step are compile-time constants.
+    auto step = [&](auto phc, float4 (&reg)[RPW], const int k) {
+        constexpr int PH = decltype(phc)::value;
+        constexpr int slot = PH % G::NR;
         if (k < nchunks) {
             store(reg, k);
             // wave maximum of what this wave staged so far (only grows); the
@@ -170,7 +200,7 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
             const unsigned int m = wave_max_u32(__float_as_uint(my_max));
             if (lane == 0) wave_max[wave] = __uint_as_float(m);
         }
-        load(reg, k + 2);  // past the last chunk: valid rows, never stored
+        load(reg, k + G::D);  // past the last chunk: valid rows, never stored
         __syncthreads();
         // ---- horizontal: row hrow of chunk k, columns [P hgrp, P hgrp + P)
         if (k < nchunks) {
@@ -222,14 +252,17 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
         const double X = static_cast<double>(__uint_as_float(xmax_bits));
         const double rmax = a.kabs * X + fabs(a.delta);
         const double bb = a.bcoef * X + a.bdelta + 4.5e-16 * rmax;  // non-finite X: every output doubtful
-        const int s0 = ob % G::R;
+        // ring row of the task's first tap: CH ((k - K0) mod NR) + M vhalf,
+        // known at compile time for each (PH, vhalf)
+        constexpr int base = CH * (((PH - G::K0) % G::NR + G::NR) % G::NR);
         double h[M + K - 1];
+        auto read_window = [&](auto vhc) {
+            constexpr int s0 = base + M * decltype(vhc)::value;
 #pragma unroll
-        for (int j = 0; j < M + K - 1; ++j) {
-            int rr = s0 + j;
-            rr = rr >= G::R ? rr - G::R : rr;
-            h[j] = ring[rr * G::RP + vcol];
-        }
+            for (int j = 0; j < M + K - 1; ++j) h[j] = ring[((s0 + j) % G::R) * G::RP + vcol];
+        };
+        if (vhalf == 0) read_window(std::integral_constant<int, 0>{});
+        else read_window(std::integral_constant<int, 1>{});
         double acc[M];
         if constexpr (UNIT) {
             double s = a.delta;
@@ -253,6 +286,7 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
         }
         const int yb = y0 + ob;
         float* op = dst + static_cast<size_t>(yb) * dstride + xg;
+        const int orow = yb * static_cast<int>(a.dst.stride);  // byte offset of row yb (fits 2^31)
         // Results and doubt flags of all M rows first (independent chains),
         // then the stores: whole tiles store unconditionally.
         float out[M];
@@ -270,7 +304,9 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
         const int nrow = min(M, rows_out - ob);
         if (x0 + TX <= W && nrow == M) {
 #pragma unroll
-            for (int m = 0; m < M; ++m) op[static_cast<size_t>(m) * dstride] = out[m];
+            for (int m = 0; m < M; ++m)
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(out[m]), wsrc, xg * 4,
+                                                      orow + m * static_cast<int>(a.dst.stride), 0);
         } else {
             const unsigned long long inside = __ballot(xg < W);
             any = 0ull;
@@ -301,12 +337,13 @@ __device__ __forceinline__ void sep1_strip(const SepArgs& a, const double* __res
             if (lane == 0) atomicAdd(a.fix.count, static_cast<unsigned long long>(total));
         }
     };
-    float4 ra[RPW], rb[RPW];
-    load(ra, 0);
-    load(rb, 1);
-    for (int k = 0; k < nsteps; k += 2) {
-        step(ra, k);
-        if (k + 1 < nsteps) step(rb, k + 1);
+    float4 regs[G::D][RPW];
+    unroll_steps<G::D>([&](auto c) { load(regs[decltype(c)::value], decltype(c)::value); });
+    for (int k = 0; k < nsteps; k += G::U) {
+        unroll_steps<G::U>([&](auto phc) {
+            constexpr int PH = decltype(phc)::value;
+            if (k + PH < nsteps) step(phc, regs[PH % G::D], k + PH);
+        });
     }
 }
 
@@ -372,7 +409,7 @@ hipError_t sep1_launch_kc(const SepArgs& a, int sh, hipStream_t s) {
 // height of the 8-row-step variant; <= 0 picks the variant by window size.
 template <int K, bool UNIT>
 hipError_t sep1_launch_k(const SepArgs& a, hipStream_t s) {
-    const bool half = a.SH >= kHalfStep || (a.SH <= 0 && K <= 15);
+    const bool half = a.SH >= kHalfStep || (a.SH <= 0 && K <= 31);  // 16-row steps pay off for long windows
     const int sh = a.SH >= kHalfStep ? a.SH - kHalfStep : a.SH;
     return half ? sep1_launch_kc<K, 8, UNIT>(a, sh, s) : sep1_launch_kc<K, 16, UNIT>(a, sh, s);
 }
