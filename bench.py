@@ -1,7 +1,12 @@
 """Benchmark of the MI355X filtering path (one GPU).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|deriv|pyr]
-                    [--no-cpu] [--cpu-budget S]
+                    [--no-cpu] [--no-extra] [--cpu-budget S]
+
+The default run (c2) prints the c2 headline line with the other
+configurations measured in the same run under "configs" (c1, c3, c4: the
+same measurement with 5 steps; c5: a 3000-request prefix of the serving
+stream), each with its own roofline and CPU baseline.
 
 Workloads (one step = every call listed once; inputs resident in HBM; enough
 input/output sets rotate that the working set exceeds the 256 MB Infinity
@@ -29,8 +34,9 @@ back to back, wall clock between two device synchronisations), the roofline
 of the dominant kernel (algorithmic bytes per launch / mean launch time from
 HIP events around 4 back-to-back launches on the launch stream), per-filter
 Gpix/s and roofline fraction, the latency of one call on an idle stream, and
-the reference's multithreaded CPU path timed on a bounded sample of the same
-calls on this host.
+the reference's multithreaded CPU path on this host (one call per filter
+group, one warm-up then the median of 5 runs, 3 for filter2D), with the host
+CPU model and its physical and logical core counts.
 """
 from __future__ import annotations
 
@@ -120,12 +126,12 @@ def _c2():
             if sigma is None:
                 launch = (lambda b: lambda lib, s, d, st: lib.ipfg_box_filter(s, d, 15, 15, 1, b, 0.0, 0, st))(b)
                 cpu = (lambda b: lambda o, im: o.box_filter(im, 15, 15, True, b))(b)
-                kern = "sep_plane_kernel<float, float, double, double, 1, 2, true, 15> (unit weights)"
+                kern = "sep1_kernel<15, CH, true> (one-channel float32 strip kernel, unit weights)"
             else:
                 launch = (lambda b, sg: lambda lib, s, d, st: lib.ipfg_gaussian_blur(s, d, sg, 0.0, 0, b, 0, st))(b, sigma)
                 cpu = (lambda b, sg: lambda o, im: o.gaussian_blur(im, sg, border=b))(b, sigma)
                 ks = 2 * max(1, int(3 * sigma + 0.5)) + 1
-                kern = f"sep_plane_kernel<float, float, double, double, 1, 2, true, {ks}>"
+                kern = f"sep1_kernel<{ks}, CH, false> (one-channel float32 strip kernel)"
             # one FMA per tap and output sample in each pass (halo rows not counted)
             taps = 30 if sigma is None else 2 * (2 * max(1, int(3 * sigma + 0.5)) + 1)
             calls.append(Call(f"{name}/{BORDERS[b]}", name, "img", np.float32, launch, cpu, kern,
@@ -434,7 +440,7 @@ def run_c5(args, ipf):
         "latency": loads,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def c5_cpu_baseline(reqs, banks, budget_s):
@@ -465,13 +471,42 @@ def c5_cpu_baseline(reqs, banks, budget_s):
         if t_total > budget_s:
             break
     return {"value": round(px / t_total / 1e9, 5), "unit": "Gpix/s", "cores": o.get_num_threads(),
-            "kind": "reference" if o.which == "ref" else "port",
-            "sample": f"first {done} requests of the stream in arrival order, {t_total:.1f} s"}
+            "kind": "reference" if o.which == "ref" else "port", "host": host_cpu(),
+            "sample": f"first {done} requests of the stream in arrival order, {t_total:.1f} s (prefix "
+                      f"bounded by time; the rate extrapolates to the whole stream)"}
 
 
-def cpu_baseline(wd, budget_s: float):
+def host_cpu():
+    """Model name and physical / logical core counts of this host (from /proc/cpuinfo)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or None, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_baseline(wd, runs: int = 5):
     """The reference's CPU path (oracle/_ref when built, else the oracle
-    restatement) with all host threads, on a bounded sample of the step."""
+    restatement) with all host threads: for one call of each filter group of
+    the step (the Reflect101 / first call), one warm-up run then the median of
+    `runs` timed runs; value = the step's pixels over the step's time with each
+    call costing its group's median."""
     try:
         from oracle.pyoracle import Oracle, ref_available
         o = Oracle("ref") if ref_available() else Oracle("oracle")
@@ -479,21 +514,34 @@ def cpu_baseline(wd, budget_s: float):
         return {"value": None, "unit": "Gpix/s", "cores": os.cpu_count(), "kind": None, "sample": f"unavailable: {e}"}
     o.set_num_threads(0)
     threads = o.get_num_threads()
-    done, t_total, px = [], 0.0, 0
-    while t_total < budget_s:  # whole passes over the step's calls until the budget is spent
-        for c in wd["calls"]:
-            im = wd["images"][c.image]
+    def cpu_group(c):  # the GPU precision variants of a call run the same reference code
+        return c.group.replace("_fp16x2", "").replace("_bf16x3", "")
+
+    reps = {}
+    for c in wd["calls"]:
+        g = cpu_group(c)
+        if g not in reps or "reflect101" in c.name:
+            reps[g] = c
+    med, t_all = {}, 0.0
+    for g, c in reps.items():
+        im = wd["images"][c.image]
+        c.cpu(o, im)  # warm-up
+        ts = []
+        for _ in range(runs):
             t0 = time.perf_counter()
             c.cpu(o, im)
-            t_total += time.perf_counter() - t0
-            done.append(c.name)
-            px += im.shape[0] * im.shape[1]
-            if t_total > budget_s:
-                break
-    return {"value": round(px / t_total / 1e9, 5), "unit": "Gpix/s", "cores": threads,
+            ts.append(time.perf_counter() - t0)
+        med[g] = statistics.median(ts)
+        t_all += sum(ts)
+    step_s = sum(med[cpu_group(c)] for c in wd["calls"])
+    step_px = sum(wd["images"][c.image].shape[0] * wd["images"][c.image].shape[1] for c in wd["calls"])
+    return {"value": round(step_px / step_s / 1e9, 5), "unit": "Gpix/s", "cores": threads,
             "kind": "reference" if o.which == "ref" else "port",
-            "sample": f"{len(done)} calls ({len(wd['calls'])} per step, taken in step order), {threads} threads, "
-                      f"{t_total:.1f} s: " + ", ".join(sorted(set(done)))}
+            "host": host_cpu(),
+            "sample": f"{len(reps)} calls ({', '.join(c.name for c in reps.values())}), each 1 warm-up + median "
+                      f"of {runs} runs, {threads} threads, {t_all:.1f} s timed; the step's other border modes "
+                      f"are costed at their filter's median",
+            "median_ms": {g: round(v * 1e3, 2) for g, v in med.items()}}
 
 
 def traffic_from_profiles(group: str):
@@ -504,30 +552,10 @@ def traffic_from_profiles(group: str):
     return d.get(group, {}).get("bytes_per_launch")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--requests", type=int, default=20000, help="c5: number of requests")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}).",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import imageprocessingflex_amd as ipf
-
-    ipf.init(0)
-    if args.workload == "c5":
-        run_c5(args, ipf)
-        return
-    t0 = time.time()
-    wd = WORKLOADS[args.workload]()
-    print(f"# {args.workload} inputs ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+def measure(wd, ipf, steps: int, warmup: int, cpu: bool, cpu_runs: int = 5):
+    """One workload: the step's calls back to back (value), per-call launch
+    times from HIP events (per filter, roofline of the dominant one), call
+    latency on an idle stream, and the CPU baseline. Returns the JSON dict."""
     stream = ipf.Stream()
     # rotating device sets: > 256 MB of inputs + outputs in flight
     set_bytes = sum(im.nbytes * 2 for im in wd["images"].values())
@@ -547,20 +575,20 @@ def main():
         ipf._lib.check(c.launch(lib, C.byref(s), C.byref(d), stream.handle))
         k += 1
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         for c in calls:
             run(c)
     stream.synchronize()
     ipf.synchronize()
     # timed region: the step's calls back to back, nothing recorded between them
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         for c in calls:
             run(c)
     stream.synchronize()
     ipf.synchronize()
     wall = time.perf_counter() - t_start
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step = wall * 1e3 / steps
     step_px = sum(wd["images"][c.image].shape[0] * wd["images"][c.image].shape[1] for c in calls)
     value = step_px / (ms_per_step * 1e-3) / 1e9
 
@@ -568,7 +596,7 @@ def main():
     # pair of HIP events on the launch stream, repeated `steps` times
     REP = 4
     per_call = {c.name: [] for c in calls}
-    for _ in range(args.steps):
+    for _ in range(steps):
         for i, c in enumerate(calls):
             ev.record(evs[2 * i], stream.handle)
             for _r in range(REP):
@@ -578,7 +606,7 @@ def main():
             per_call[c.name].append(ev.ms(evs[2 * i], evs[2 * i + 1]) / REP)
     # call latency: one call between a pair of events, the device idle before it
     lat = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         for i, c in enumerate(calls):
             stream.synchronize()
             ev.record(evs[0], stream.handle)
@@ -635,22 +663,22 @@ def main():
     copy_ms = ev.ms(evs[0], evs[1]) / 5
     copy = {"bytes_per_launch": 2 * big.nbytes, "mean_ms": round(copy_ms, 4),
             "gb_s": round(2 * big.nbytes / (copy_ms * 1e-3) / 1e9, 1)}
-    cpu = None if args.no_cpu else cpu_baseline(wd, args.cpu_budget)
+    cpu_line = cpu_baseline(wd, cpu_runs) if cpu else None
     extra = {}
-    if "post" in wd and not args.no_cpu:
+    if "post" in wd and cpu:
         extra["accuracy"] = wd["post"](ipf)
     mfma = {n: {"algorithmic_tflop_s": v["tflop_s"], "peak_tflop_s": MFMA_PEAK_TFLOPS,
                 "frac": round(v["tflop_s"] / MFMA_PEAK_TFLOPS, 4)}
             for n, v in summary.items() if "tflop_s" in v and ("bf16" in n or "fp16" in n)}
     if mfma:
         extra["mfma_roofline"] = mfma
-    line = {
+    return {
         "metric": METRIC,
         "value": round(value, 2),
         "unit": "Gpix/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -661,14 +689,71 @@ def main():
                    "precision": "default: FP64 accumulation, certified bit-identical to the reference",
                    "rotating_sets": nsets},
         "roofline": roof,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_line,
         "per_filter": summary,
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4),
                        "what": "one filter call on an idle stream, HIP events around the call"},
-        "launch_timing": f"mean of {REP} back-to-back launches between two HIP events, x {args.steps}",
+        "launch_timing": f"mean of {REP} back-to-back launches between two HIP events, x {steps}",
         "device_copy": copy,
         **extra,
     }
+
+
+# The other configurations reported beside the headline (default run): the
+# same measurement with fewer steps; c5 on a prefix of the serving stream.
+EXTRA_CONFIGS = ("c1", "c3", "c4", "c5")
+EXTRA_STEPS = 5
+EXTRA_C5_REQUESTS = 3000
+
+
+def _brief(line: dict) -> dict:
+    """A config's entry in the headline line: its numbers without the metric header."""
+    drop = {"metric", "unit", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"}
+    return {k: v for k, v in line.items() if k not in drop}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="c2 only: skip the other configurations")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="c5: seconds of CPU baseline")
+    ap.add_argument("--requests", type=int, default=20000, help="c5: number of requests")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import imageprocessingflex_amd as ipf
+
+    ipf.init(0)
+    if args.workload == "c5":
+        print(json.dumps(run_c5(args, ipf)), flush=True)
+        return
+    t0 = time.time()
+    wd = WORKLOADS[args.workload]()
+    print(f"# {args.workload} inputs ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    cpu_runs = 3 if args.workload == "c4" else 5
+    line = measure(wd, ipf, args.steps, args.warmup, not args.no_cpu, cpu_runs)
+    if args.workload == "c2" and not args.no_extra:
+        # every other configuration in the same run (its own roofline and CPU baseline)
+        del wd
+        configs = {}
+        for name in EXTRA_CONFIGS:
+            t0 = time.time()
+            if name == "c5":
+                sub = argparse.Namespace(**{**vars(args), "requests": EXTRA_C5_REQUESTS, "cpu_budget": 5.0})
+                configs[name] = _brief(run_c5(sub, ipf))
+            else:
+                w = WORKLOADS[name]()
+                configs[name] = _brief(measure(w, ipf, EXTRA_STEPS, 1, not args.no_cpu, 3 if name == "c4" else 5))
+                del w
+            print(f"# {name} measured in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+        line["configs"] = configs
     print(json.dumps(line), flush=True)
 
 
