@@ -27,6 +27,9 @@
 //   motion-blur kernel ...... include/ipf/filter.hpp:498-516
 //   adaptive threshold ...... include/ipf/filter.hpp:545-565
 //   pyrDown / pyrUp ......... include/ipf/geometry.hpp:501-524
+//   canny / cannyAuto ....... include/ipf/features.hpp:26-101
+//   toGray / rgbToGray ...... include/ipf/color.hpp:51-81, core.hpp:465-473
+//   corner responses ........ include/ipf/features.hpp:107-160
 //
 // Pinning: every function here is pinned by tests/test_oracle.py against the
 // reference-built fixtures in tests/golden/ and against closed forms, brute force
@@ -942,6 +945,218 @@ int ipfo_motion_kernel(int length, double angle, float* out, int cap) {
         for (float& v : k) v /= s;
         for (int i = 0; i < size * size && i < cap; ++i) out[i] = k[i];
         return size;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- features
+namespace {
+
+// canny (features.hpp:26-90): Sobel gradients (Replicate border), per pixel
+// the channel with the largest float magnitude (L1, or L2 computed in float
+// and square-rooted), non-maximum suppression along the quantised gradient
+// direction (neighbours outside the image count as 0), then hysteresis: the
+// weak candidates 8-connected to a strong one (directly or through other
+// candidates) become edges. Output 255 / 0, one channel.
+template <class T>
+void canny(const T* src, int W, int H, int cn, uint8_t* out, double low, double high, int ksize, bool l2) {
+    check(low <= high, "canny: lowThreshold must be <= highThreshold");
+    const size_t n = static_cast<size_t>(W) * H;
+    std::vector<float> gx(n * cn), gy(n * cn);
+    sobel<T>(src, W, H, cn, gx.data(), 1, 0, ksize, 1.0, B_REPLICATE);
+    sobel<T>(src, W, H, cn, gy.data(), 0, 1, ksize, 1.0, B_REPLICATE);
+    std::vector<float> mag(n), sx(n, 0.f), sy(n, 0.f);
+    rows_parallel(H, static_cast<double>(W) * cn * 4, [&](int y0, int y1) {
+        for (size_t p = static_cast<size_t>(y0) * W; p < static_cast<size_t>(y1) * W; ++p) {
+            float best = -1.f;
+            for (int c = 0; c < cn; ++c) {
+                const float a = gx[p * cn + c], b = gy[p * cn + c];
+                const float m = l2 ? std::sqrt(a * a + b * b) : std::fabs(a) + std::fabs(b);
+                if (m > best) {
+                    best = m;
+                    sx[p] = a;
+                    sy[p] = b;
+                }
+            }
+            mag[p] = best;
+        }
+    });
+    std::vector<uint8_t> st(n, 0);  // 0 none, 1 weak candidate, 2 strong
+    const float tan22 = 0.41421356f;
+    auto at = [&](int x, int y) { return (x < 0 || y < 0 || x >= W || y >= H) ? 0.f : mag[static_cast<size_t>(y) * W + x]; };
+    rows_parallel(H, static_cast<double>(W) * 6, [&](int y0, int y1) {
+        for (int y = y0; y < y1; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t p = static_cast<size_t>(y) * W + x;
+                const float m = mag[p];
+                if (m <= low) continue;
+                const float ax = std::fabs(sx[p]), ay = std::fabs(sy[p]);
+                bool peak;
+                if (ay <= ax * tan22) {
+                    peak = m > at(x - 1, y) && m >= at(x + 1, y);
+                } else if (ay * tan22 >= ax) {
+                    peak = m > at(x, y - 1) && m >= at(x, y + 1);
+                } else {
+                    const int s = (sx[p] < 0) != (sy[p] < 0) ? -1 : 1;
+                    peak = m > at(x - s, y - 1) && m > at(x + s, y + 1);
+                }
+                if (peak) st[p] = m > high ? 2 : 1;
+            }
+    });
+    // hysteresis by an explicit stack (the result does not depend on the order)
+    std::vector<size_t> stack;
+    for (size_t p = 0; p < n; ++p)
+        if (st[p] == 2) stack.push_back(p);
+    while (!stack.empty()) {
+        const size_t p = stack.back();
+        stack.pop_back();
+        const int x = static_cast<int>(p % W), y = static_cast<int>(p / W);
+        for (int j = -1; j <= 1; ++j)
+            for (int i = -1; i <= 1; ++i) {
+                const int qx = x + i, qy = y + j;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                const size_t q = static_cast<size_t>(qy) * W + qx;
+                if (st[q] != 1) continue;
+                st[q] = 2;
+                stack.push_back(q);
+            }
+    }
+    for (size_t p = 0; p < n; ++p) out[p] = st[p] == 2 ? 255 : 0;
+}
+
+// toGray (color.hpp:78-81): channel 0 of a 1- or 2-channel image; Luma 601
+// of the first three channels (0.299 r + 0.587 g + 0.114 b in FP64, then
+// saturate_cast) for 3 or 4 channels (color.hpp:51-67).
+template <class T>
+std::vector<T> to_gray(const T* src, int W, int H, int cn) {
+    const size_t n = static_cast<size_t>(W) * H;
+    std::vector<T> g(n);
+    if (cn <= 2) {
+        for (size_t p = 0; p < n; ++p) g[p] = src[p * cn];
+        return g;
+    }
+    check(cn == 3 || cn == 4, "rgbToGray: expected a 3-channel (RGB) or 4-channel (RGBA) image");
+    for (size_t p = 0; p < n; ++p) {
+        const double r = src[p * cn], gr = src[p * cn + 1], b = src[p * cn + 2];
+        g[p] = sat<T>(0.299 * r + 0.587 * gr + 0.114 * b);
+    }
+    return g;
+}
+
+// cannyAuto (features.hpp:92-101): gray, FP64 Gaussian blur to float
+// (Reflect101), Sobel gradient magnitude (ksize 3, Reflect101, hypot), high
+// threshold = the element of rank 7n/10 of the magnitudes (at least 1e-6),
+// low = 0.4 high, then canny with ksize 3 and the L2 magnitude.
+template <class T>
+void canny_auto(const T* src, int W, int H, int cn, uint8_t* out, double sigma) {
+    const std::vector<T> gray = to_gray<T>(src, W, H, cn);
+    const int ks = ipfo_gaussian_kernel_1d(sigma, 0, nullptr, 0);
+    check(ks > 0, "gaussianKernel1D: need sigma > 0 or ksize > 0");
+    std::vector<double> k(static_cast<size_t>(ks));
+    ipfo_gaussian_kernel_1d(sigma, 0, k.data(), ks);
+    const size_t n = static_cast<size_t>(W) * H;
+    std::vector<float> smooth(n), a(n), b(n);
+    sep_filter<T, float>(gray.data(), W, H, 1, smooth.data(), k.data(), ks, k.data(), ks, ks / 2, ks / 2,
+                         B_REFLECT101, 0.0, 0.0);
+    if (n == 0) return;
+    sobel<float>(smooth.data(), W, H, 1, a.data(), 1, 0, 3, 1.0, B_REFLECT101);
+    sobel<float>(smooth.data(), W, H, 1, b.data(), 0, 1, 3, 1.0, B_REFLECT101);
+    std::vector<float> v(n);
+    for (size_t p = 0; p < n; ++p) v[p] = std::hypot(a[p], b[p]);
+    const size_t r = n * 7 / 10;
+    std::nth_element(v.begin(), v.begin() + static_cast<std::ptrdiff_t>(r), v.end());
+    const double high = std::max(1e-6, static_cast<double>(v[r]));
+    canny<float>(smooth.data(), W, H, 1, out, 0.4 * high, high, 3, true);
+}
+
+// structure tensor (features.hpp:107-126): Sobel derivatives scaled by
+// 1 / (2^(ksize - 1) * block * nominal max) (float images: no nominal
+// max), float products, unnormalised box sums (Reflect101).
+template <class T>
+void structure_tensor(const T* src, int W, int H, int cn, int block, int ksize, std::vector<float>& a,
+                      std::vector<float>& b, std::vector<float>& c) {
+    check(cn == 1, "corner detection: expected a single-channel image");
+    double scale = static_cast<double>(1 << ((ksize > 0 ? ksize : 3) - 1)) * block;
+    if (!std::is_floating_point<T>::value) scale *= nominal_max<T>();
+    scale = 1.0 / scale;
+    const size_t n = static_cast<size_t>(W) * H;
+    std::vector<float> dx(n), dy(n), xx(n), xy(n), yy(n);
+    sobel<T>(src, W, H, 1, dx.data(), 1, 0, ksize, scale, B_REFLECT101);
+    sobel<T>(src, W, H, 1, dy.data(), 0, 1, ksize, scale, B_REFLECT101);
+    for (size_t i = 0; i < n; ++i) {
+        xx[i] = dx[i] * dx[i];
+        xy[i] = dx[i] * dy[i];
+        yy[i] = dy[i] * dy[i];
+    }
+    a.resize(n);
+    b.resize(n);
+    c.resize(n);
+    box_filter<float, float>(xx.data(), W, H, 1, a.data(), block, block, false, B_REFLECT101, 0.0);
+    box_filter<float, float>(xy.data(), W, H, 1, b.data(), block, block, false, B_REFLECT101, 0.0);
+    box_filter<float, float>(yy.data(), W, H, 1, c.data(), block, block, false, B_REFLECT101, 0.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipfo_canny(const void* src, int w, int h, int cn, int dtype, uint8_t* dst, double low, double high, int ksize,
+               int l2) {
+    return guarded([&] {
+        check(low <= high, "canny: lowThreshold must be <= highThreshold");
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            canny<T>(static_cast<const T*>(src), w, h, cn, dst, low, high, ksize, l2 != 0);
+            return 0;
+        });
+    });
+}
+
+int ipfo_canny_auto(const void* src, int w, int h, int cn, int dtype, uint8_t* dst, double sigma) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            canny_auto<T>(static_cast<const T*>(src), w, h, cn, dst, sigma);
+            return 0;
+        });
+    });
+}
+
+// cornerHarris (features.hpp:128-139): A C - B^2 - k (A + C)^2 in FP64, to float.
+int ipfo_corner_harris(const void* src, int w, int h, int cn, int dtype, float* dst, int block, int ksize, double k) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            std::vector<float> a, b, c;
+            structure_tensor<T>(static_cast<const T*>(src), w, h, cn, block, ksize, a, b, c);
+            for (size_t i = 0; i < a.size(); ++i) {
+                const double A = a[i], B = b[i], C = c[i];
+                dst[i] = static_cast<float>(A * C - B * B - k * (A + C) * (A + C));
+            }
+            return 0;
+        });
+    });
+}
+
+// cornerMinEigenVal (features.hpp:141-153): (A + C) - sqrt((A - C)^2 + B^2)
+// with A, C halved, in FP64, to float.
+int ipfo_corner_min_eigen(const void* src, int w, int h, int cn, int dtype, float* dst, int block, int ksize) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            std::vector<float> a, b, c;
+            structure_tensor<T>(static_cast<const T*>(src), w, h, cn, block, ksize, a, b, c);
+            for (size_t i = 0; i < a.size(); ++i) {
+                const double A = a[i] * 0.5, B = b[i], C = c[i] * 0.5;
+                dst[i] = static_cast<float>((A + C) - std::sqrt((A - C) * (A - C) + B * B));
+            }
+            return 0;
+        });
     });
 }
 

@@ -88,6 +88,10 @@ class Oracle:
             "ssim": (_I, [_V, _V, _I, _I, _I, _I, _V, C.POINTER(_D)]),
             "adaptive_threshold": (_I, [_V, _I, _I, _I, _V, _D, _I, _I, _I, _D]),
             "motion_kernel": (_I, [_I, _D, _PF, _I]),
+            "canny": (_I, [_V, _I, _I, _I, _I, _V, _D, _D, _I, _I]),
+            "canny_auto": (_I, [_V, _I, _I, _I, _I, _V, _D]),
+            "corner_harris": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _D]),
+            "corner_min_eigen": (_I, [_V, _I, _I, _I, _I, _V, _I, _I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(self.lib, pre + name)
@@ -277,4 +281,37 @@ class Oracle:
         out = np.empty_like(src)
         self._check(self._f("bilateral")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, d,
                                          sigma_color, sigma_space, border))
+        return out
+
+    # ---- feature callers of the filters (features.hpp:26-160)
+    def canny(self, src, low, high, ksize=3, l2=False):
+        """canny (features.hpp:26-90): uint8 edge map (255 / 0), one channel."""
+        src, h, w, cn = self._shape(src)
+        out = np.empty((h, w), np.uint8)
+        self._check(self._f("canny")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data, float(low),
+                                     float(high), int(ksize), int(bool(l2))))
+        return out
+
+    def canny_auto(self, src, sigma=1.4):
+        """cannyAuto (features.hpp:92-101)."""
+        src, h, w, cn = self._shape(src)
+        out = np.empty((h, w), np.uint8)
+        self._check(self._f("canny_auto")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                          float(sigma)))
+        return out
+
+    def corner_harris(self, src, block=3, ksize=3, k=0.04):
+        """cornerHarris (features.hpp:128-139): float32 response."""
+        src, h, w, cn = self._shape(src)
+        out = np.empty((h, w), np.float32)
+        self._check(self._f("corner_harris")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                             int(block), int(ksize), float(k)))
+        return out
+
+    def corner_min_eigen(self, src, block=3, ksize=3):
+        """cornerMinEigenVal (features.hpp:141-153): float32 response."""
+        src, h, w, cn = self._shape(src)
+        out = np.empty((h, w), np.float32)
+        self._check(self._f("corner_min_eigen")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                                int(block), int(ksize)))
         return out

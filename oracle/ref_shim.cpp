@@ -2,10 +2,11 @@
 //
 // C-ABI wrapper around the reference ipf headers, compiled by oracle/Makefile
 // from the reference source tree (include/ipf/filter.hpp, geometry.hpp,
-// core.hpp, parallel.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
+// core.hpp, parallel.hpp, features.hpp, color.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
 // restatement bit-for-bit against the real reference, (2) write the golden
 // fixtures under tests/golden/, and (3) serve as the "reference" CPU baseline.
 // Signatures match oracle/ipf_oracle.cpp's ipfo_* functions one for one.
+#include <ipf/features.hpp>
 #include <ipf/filter.hpp>
 #include <ipf/geometry.hpp>
 #include <ipf/metrics.hpp>
@@ -321,6 +322,52 @@ int ipfr_motion_kernel(int length, double angle, float* out, int cap) {
         const int n = static_cast<int>(k.sampleCount());
         for (int i = 0; i < n && i < cap; ++i) out[i] = k.data()[i];
         return k.width();
+    });
+}
+
+// ---- feature callers of the filters (features.hpp:26-160)
+int ipfr_canny(const void* src, int w, int h, int cn, int dtype, std::uint8_t* dst, double low, double high, int ksize,
+               int l2) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::canny(img, low, high, ksize, l2 != 0); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_canny_auto(const void* src, int w, int h, int cn, int dtype, std::uint8_t* dst, double sigma) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::cannyAuto(img, sigma); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_corner_harris(const void* src, int w, int h, int cn, int dtype, float* dst, int block, int ksize, double k) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::cornerHarris(img, block, ksize, k); }), dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_corner_min_eigen(const void* src, int w, int h, int cn, int dtype, float* dst, int block, int ksize) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            unwrap(timed([&] { return ipf::cornerMinEigenVal(img, block, ksize); }), dst);
+            return 0;
+        });
     });
 }
 

@@ -37,6 +37,8 @@ def run_oracle(o, case):
         return getattr(o, c["op"])(inp)
     if c["op"] == "ssim":
         return o.ssim(inp, ex["b"])[1]
+    if c["op"] in FEATURE_OPS:
+        return _run_feature(o, case)
     out_dt = np.dtype(p["out"]) if "out" in p else None
     b = p["border"]
     if c["op"] == "gaussian":
@@ -53,6 +55,25 @@ def run_oracle(o, case):
     if c["op"] == "bilateral":
         return o.bilateral(inp, p["d"], p["sc"], p["ss"], b)
     return _run_composite(o, case)
+
+
+FEATURE_OPS = ("canny", "canny_auto", "harris", "min_eigen")
+
+
+def _run_feature(m, case, gpu=False):
+    """canny / cannyAuto / cornerHarris / cornerMinEigenVal cases; m is the
+    oracle or the package (whose min-eigen function carries the reference's
+    name)."""
+    c, inp, _, _ = case
+    p = c["params"]
+    if c["op"] == "canny":
+        return m.canny(inp, p["low"], p["high"], p["ksize"], bool(p["l2"]))
+    if c["op"] == "canny_auto":
+        return m.canny_auto(inp, p["sigma"])
+    if c["op"] == "harris":
+        return m.corner_harris(inp, p["block"], p["ksize"], p["k"])
+    fn = m.corner_min_eigen_val if gpu else m.corner_min_eigen
+    return fn(inp, p["block"], p["ksize"])
 
 
 def _run_composite(m, case, **kw):
@@ -92,6 +113,8 @@ def run_gpu(ipf, case, precision=0):
         return getattr(ipf, c["op"])(inp)
     if c["op"] == "ssim":
         return ipf.ssim(inp, ex["b"], return_map=True)[1]
+    if c["op"] in FEATURE_OPS:
+        return _run_feature(ipf, case, gpu=True)
     out_dt = np.dtype(p["out"]) if "out" in p else None
     b = p["border"]
     if c["op"] == "gaussian":

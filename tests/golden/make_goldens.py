@@ -176,6 +176,35 @@ def main():
         value, smap = ref.ssim(img, other)
         add("ssim", {"src": name, "value": value}, img, smap, b=other)
 
+    # canny / cannyAuto / cornerHarris / cornerMinEigenVal (features.hpp:26-160):
+    # polygon scenes with long edges, noise, every channel count toGray accepts
+    poly1 = np.ascontiguousarray(poly[..., 1])
+    u8_4 = wl.uniform_u8((21, 18, 4), seed=seed, stream=14)
+    poly_f = (poly.astype(np.float32) / 255.0).astype(np.float32)
+    for img, name, scale in ((poly, "poly", 1.0), (poly1, "poly1", 1.0), (u8_1, "u8x1", 1.0),
+                             (f32_1, "f32x1", 1.0 / 255.0), (i16_1, "i16x1", 200.0), (poly_f, "polyf", 1.0 / 255.0)):
+        for (lo, hi) in ((30.0, 90.0), (100.0, 100.0), (5.0, 400.0)):
+            for k in (3, 5, 7):
+                for l2 in (0, 1):
+                    if k == 7 and (lo, hi) != (30.0, 90.0):
+                        continue
+                    gain = {3: 1, 5: 12, 7: 160}[k] * scale  # Sobel step response grows with the aperture
+                    t = (lo * gain, hi * gain)
+                    add("canny", {"low": t[0], "high": t[1], "ksize": k, "l2": l2, "src": name}, img,
+                        ref.canny(img, t[0], t[1], k, l2))
+    for img, name in ((poly, "poly"), (poly1, "poly1"), (u8_1, "u8x1"), (u8_2, "u8x2"), (f32_3, "f32x3"),
+                      (u16_3, "u16x3"), (u8_4, "u8x4"), (i16_1, "i16x1"), (tiny, "tiny"), (line, "line")):
+        for sg in (1.4, 0.6, 3.0):
+            add("canny_auto", {"sigma": sg, "src": name}, img, ref.canny_auto(img, sg))
+    for img, name in ((poly1, "poly1"), (u8_1, "u8x1"), (f32_1, "f32x1"), (i16_1, "i16x1"), (line, "line"),
+                      (pixel, "pixel")):
+        for blk in (1, 2, 3, 5):
+            for k in (1, 3, 5, 7):
+                add("harris", {"block": blk, "ksize": k, "k": 0.04, "src": name}, img,
+                    ref.corner_harris(img, blk, k, 0.04))
+                add("min_eigen", {"block": blk, "ksize": k, "src": name}, img, ref.corner_min_eigen(img, blk, k))
+        add("harris", {"block": 3, "ksize": 3, "k": -0.15, "src": name}, img, ref.corner_harris(img, 3, 3, -0.15))
+
     np.savez_compressed(OUT / "fixtures.npz", **arrays)
     (OUT / "manifest.json").write_text(json.dumps(cases, indent=0))
 
@@ -220,6 +249,11 @@ def main():
         full[f"c2_box15_{NAMES[b]}_f32"] = sha(ref.box_filter(c2, 15, 15, True, b))
     full["c3_pyrdown_u8"] = sha(ref.pyr_down(c3))
     full["c3_pyrup_u8"] = sha(ref.pyr_up(c3))
+    full["c3_canny_50_150_u8"] = sha(ref.canny(c3, 50.0, 150.0))
+    full["c3_canny_auto_u8"] = sha(ref.canny_auto(c3))
+    c3g = np.ascontiguousarray(c3[..., 0])
+    full["c3g_harris_u8"] = sha(ref.corner_harris(c3g, 3, 3, 0.04))
+    full["c3g_min_eigen_u8"] = sha(ref.corner_min_eigen(c3g, 3, 3))
     (OUT / "fullsize.json").write_text(json.dumps(full, indent=1))
     print(f"{len(cases)} cases, {sum(a.nbytes for a in arrays.values()) / 1e6:.2f} MB raw")
 
