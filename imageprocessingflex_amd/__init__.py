@@ -461,6 +461,51 @@ def ssim(a, b, return_map: bool = False, stream: Stream | None = None):
     return (out.value, m) if return_map else out.value
 
 
+def _one_channel_io(src, out_dtype):
+    """Descriptors for src and a new one-channel destination of its size."""
+    if isinstance(src, DeviceImage):
+        dst = DeviceImage(src.height, src.width, 1, out_dtype)
+        return src.descriptor(), dst.descriptor(), dst
+    src = np.asarray(src)
+    if not _rows_contiguous(src):
+        src = np.ascontiguousarray(src)
+    h, w, _ = _shape(src)
+    dst = np.empty((h, w), np.dtype(out_dtype))
+    return _desc(src), _desc(dst), (dst, src)
+
+
+def canny(src, low_threshold, high_threshold, ksize=3, l2_gradient=False, stream: Stream | None = None):
+    """canny (features.hpp:26-90): one-channel uint8 edge map (255 / 0)."""
+    s, d, holder = _one_channel_io(src, np.uint8)
+    _lib.check(_lib.load().ipfg_canny(C.byref(s), C.byref(d), float(low_threshold), float(high_threshold), int(ksize),
+                                      int(bool(l2_gradient)), _sh(stream)))
+    return _ret(holder)
+
+
+def canny_auto(src, blur_sigma=1.4, stream: Stream | None = None):
+    """cannyAuto (features.hpp:92-101): canny with thresholds from the gradient magnitude's
+    70th percentile of the blurred gray image."""
+    s, d, holder = _one_channel_io(src, np.uint8)
+    _lib.check(_lib.load().ipfg_canny_auto(C.byref(s), C.byref(d), float(blur_sigma), _sh(stream)))
+    return _ret(holder)
+
+
+def corner_harris(src, block_size=3, ksize=3, k=0.04, stream: Stream | None = None):
+    """cornerHarris (features.hpp:128-139): float32 response det(M) - k trace(M)^2."""
+    s, d, holder = _one_channel_io(src, np.float32)
+    _lib.check(_lib.load().ipfg_corner_harris(C.byref(s), C.byref(d), int(block_size), int(ksize), float(k),
+                                              _sh(stream)))
+    return _ret(holder)
+
+
+def corner_min_eigen_val(src, block_size=3, ksize=3, stream: Stream | None = None):
+    """cornerMinEigenVal (features.hpp:141-153): float32 Shi-Tomasi response."""
+    s, d, holder = _one_channel_io(src, np.float32)
+    _lib.check(_lib.load().ipfg_corner_min_eigen_val(C.byref(s), C.byref(d), int(block_size), int(ksize),
+                                                     _sh(stream)))
+    return _ret(holder)
+
+
 def build_pyramid(src, levels: int, stream: Stream | None = None):
     """buildPyramid (geometry.hpp:526-535): level 0 is src, then pyrDown while both sides exceed 1."""
     if levels < 1:

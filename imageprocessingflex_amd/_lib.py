@@ -99,6 +99,10 @@ SIGNATURES = {
     "ipfg_pyr_down": (_I, [_IMG, _IMG, _V]),
     "ipfg_pyr_up": (_I, [_IMG, _IMG, _V]),
     "ipfg_ssim": (_I, [_IMG, _IMG, _IMG, _P(_D), _V]),
+    "ipfg_canny": (_I, [_IMG, _IMG, _D, _D, _I, _I, _V]),
+    "ipfg_canny_auto": (_I, [_IMG, _IMG, _D, _V]),
+    "ipfg_corner_harris": (_I, [_IMG, _IMG, _I, _I, _D, _V]),
+    "ipfg_corner_min_eigen_val": (_I, [_IMG, _IMG, _I, _I, _V]),
     "ipfg_batch_create": (_I, [_P(_V), _I, _I]),
     "ipfg_batch_run": (_I, [_V, _JOB, _I, _V]),
     "ipfg_batch_destroy": (_I, [_V]),

@@ -132,6 +132,29 @@ struct BilateralLaunch {
 hipError_t launch_bilateral(const BilateralLaunch& a, hipStream_t s);
 bool bilateral_fast_ok(int dt, int cn, int r, int lut_len);
 
+// ---------------------------------------------------------------- features
+// canny after the gradients: gx, gy float images (cn channels) of the same
+// shape; dst a one-channel uint8 image of that shape. Thresholds low / high,
+// or (sel non-null) cannyAuto's thresholds from the selected magnitude bits.
+struct CannyScratch {
+    uint8_t* state;  // W x H
+    int* parent;     // W x H
+    uint8_t* flag;   // W x H
+};
+hipError_t launch_canny(const DevImage& gx, const DevImage& gy, double low, double high, bool l2,
+                        const unsigned int* sel, const CannyScratch& sc, const DevImage& dst, hipStream_t s);
+// toGray (color.hpp:78-81) of src (1-4 channels) into the one-channel dst.
+hipError_t launch_to_gray(const DevImage& src, const DevImage& dst, hipStream_t s);
+// The element of rank k (ascending) of n non-negative floats; its bits end in
+// scratch[0] (scratch: select_scratch_bytes()).
+size_t select_scratch_bytes();
+hipError_t launch_select_rank(const float* v, size_t n, unsigned long long k, unsigned int* scratch, hipStream_t s);
+// cornerHarris (harris) or cornerMinEigenVal response of the structure tensor
+// of derivatives dx, dy (one-channel float) over block x block windows into
+// the float image dst; rows holds 3 W x H doubles.
+hipError_t launch_structure_response(const DevImage& dx, const DevImage& dy, int block, bool harris, double k,
+                                     double* rows, const DevImage& dst, hipStream_t s);
+
 // Zeroes a stream's fix-up counter and ticket with a kernel.
 hipError_t launch_zero_counter(unsigned long long* count, unsigned int* done, hipStream_t s);
 

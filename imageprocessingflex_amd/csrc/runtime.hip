@@ -1716,6 +1716,126 @@ void adaptive_impl(const ipfg_image* src, const ipfg_image* dst, double maxval, 
     io.finish();
 }
 
+// ------------------------------------------------------------------ features
+void check_edge_dst(const ipfg_image* src, const ipfg_image* dst, const char* what) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(dst->dtype == IPFG_U8 && dst->channels == 1 && dst->width == src->width && dst->height == src->height,
+              what);
+}
+
+// canny after the image is on the device: Sobel gradients with Replicate
+// borders (features.hpp:31-32), then suppression and hysteresis.
+void canny_device(const DevImage& src, const DevImage& dst, double low, double high, int ksize, bool l2,
+                  const unsigned int* sel, CallScratch& sc) {
+    const hipStream_t s = sc.stream();
+    const ipfg_stream st = reinterpret_cast<ipfg_stream>(s);
+    const int W = src.w, H = src.h;
+    const DevImage gx = scratch_image(sc, W, H, src.cn, IPFG_F32), gy = scratch_image(sc, W, H, src.cn, IPFG_F32);
+    const ipfg_image ds = dev_view(src), dgx = dev_view(gx), dgy = dev_view(gy);
+    sobel_impl(&ds, &dgx, 1, 0, ksize, 1.0, IPFG_BORDER_REPLICATE, IPFG_PREC_DEFAULT, st);
+    sobel_impl(&ds, &dgy, 0, 1, ksize, 1.0, IPFG_BORDER_REPLICATE, IPFG_PREC_DEFAULT, st);
+    const size_t n = static_cast<size_t>(W) * H;
+    CannyScratch cs;
+    cs.parent = static_cast<int*>(sc.alloc(n * sizeof(int)));
+    cs.state = static_cast<uint8_t*>(sc.alloc(n));
+    cs.flag = static_cast<uint8_t*>(sc.alloc(n));
+    const hipError_t e = launch_canny(gx, gy, low, high, l2, sel, cs, dst, s);
+    if (e == hipErrorNotSupported) throw Unsupported{"canny: images of 2^31 pixels or more are not supported"};
+    HIPCALL(e);
+}
+
+// canny (features.hpp:26-90)
+void canny_impl(const ipfg_image* src, const ipfg_image* dst, double low, double high, int ksize, int l2,
+                ipfg_stream stream) {
+    require(low <= high, "canny: lowThreshold must be <= highThreshold");
+    check_edge_dst(src, dst, "canny: dst must be a one-channel uint8 image of the source's size");
+    deriv_kernel(1, ksize);  // sobel's argument errors, before any device work
+    deriv_kernel(0, ksize);
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc, true);
+    canny_device(io.src(), io.dst(), low, high, ksize, l2 != 0, nullptr, sc);
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
+// cannyAuto (features.hpp:92-101): toGray, gaussianBlur<float> (Reflect101),
+// the L2 Sobel magnitude (Reflect101), its element of rank 7n/10 as the high
+// threshold (selected on the device), canny(smooth, 0.4 high, high, 3, true).
+void canny_auto_impl(const ipfg_image* src, const ipfg_image* dst, double sigma, ipfg_stream stream) {
+    check_edge_dst(src, dst, "cannyAuto: dst must be a one-channel uint8 image of the source's size");
+    if (src->channels > 2)
+        require(src->channels == 3 || src->channels == 4, "rgbToGray: expected a 3-channel (RGB) or 4-channel (RGBA) image");
+    const std::vector<double> k = gaussian_kernel(sigma, 0);
+    const int ks = static_cast<int>(k.size());
+    ensure_init();
+    const int W = src->width, H = src->height;
+    if (static_cast<long long>(W) * H == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    const ipfg_stream st = reinterpret_cast<ipfg_stream>(s);
+    CallScratch sc(s);
+    Staged io(src, dst, sc, true);
+    DevImage gray = io.src();
+    if (src->channels > 1) {
+        gray = scratch_image(sc, W, H, 1, src->dtype);
+        HIPCALL(launch_to_gray(io.src(), gray, s));
+    }
+    const DevImage smooth = scratch_image(sc, W, H, 1, IPFG_F32);
+    const DevImage a = scratch_image(sc, W, H, 1, IPFG_F32), b = scratch_image(sc, W, H, 1, IPFG_F32);
+    const DevImage mag = scratch_image(sc, W, H, 1, IPFG_F32);
+    const ipfg_image dgray = dev_view(gray), dsm = dev_view(smooth), da = dev_view(a), db = dev_view(b);
+    sep_impl(&dgray, &dsm, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0,
+             IPFG_PREC_DEFAULT, st);
+    sobel_impl(&dsm, &da, 1, 0, 3, 1.0, IPFG_BORDER_REFLECT101, IPFG_PREC_DEFAULT, st);
+    sobel_impl(&dsm, &db, 0, 1, 3, 1.0, IPFG_BORDER_REFLECT101, IPFG_PREC_DEFAULT, st);
+    hipLaunchKernelGGL(polar_kernel, dim3(elementwise_grid(mag)), dim3(kBlock), 0, s, a, b, mag, 0);
+    HIPCALL(hipGetLastError());
+    const size_t n = static_cast<size_t>(W) * H;
+    unsigned int* sel = static_cast<unsigned int*>(sc.alloc(select_scratch_bytes()));
+    HIPCALL(launch_select_rank(static_cast<const float*>(mag.data), n, n * 7 / 10, sel, s));
+    canny_device(smooth, io.dst(), 0.0, 0.0, 3, true, sel, sc);
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
+// cornerHarris / cornerMinEigenVal (features.hpp:107-153): Sobel derivatives
+// scaled by 1 / (2^(ksize - 1) block max) (features.hpp:110-112), the
+// structure tensor's box sums and the response.
+void corner_impl(const ipfg_image* src, const ipfg_image* dst, int block, int ksize, bool harris, double k,
+                 ipfg_stream stream) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    require(src->channels == 1, "corner detection: expected a single-channel image");
+    const int kk = ksize > 0 ? ksize : 3;
+    double scale = std::ldexp(1.0, kk - 1) * block;
+    if (dtype_is_int(src->dtype)) scale *= nominal_max(src->dtype);
+    scale = 1.0 / scale;
+    deriv_kernel(1, ksize);  // sobel's argument errors, then boxFilter's
+    deriv_kernel(0, ksize);
+    require(block > 0, "boxFilter: kernel size must be positive");
+    abi_check(dst->dtype == IPFG_F32 && dst->channels == 1 && dst->width == src->width && dst->height == src->height,
+              harris ? "cornerHarris: dst must be a one-channel float32 image of the source's size"
+                     : "cornerMinEigenVal: dst must be a one-channel float32 image of the source's size");
+    ensure_init();
+    const int W = src->width, H = src->height;
+    if (static_cast<long long>(W) * H == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    const ipfg_stream st = reinterpret_cast<ipfg_stream>(s);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    const DevImage dx = scratch_image(sc, W, H, 1, IPFG_F32), dy = scratch_image(sc, W, H, 1, IPFG_F32);
+    const ipfg_image ds = dev_view(io.src()), ddx = dev_view(dx), ddy = dev_view(dy);
+    sobel_impl(&ds, &ddx, 1, 0, ksize, scale, IPFG_BORDER_REFLECT101, IPFG_PREC_DEFAULT, st);
+    sobel_impl(&ds, &ddy, 0, 1, ksize, scale, IPFG_BORDER_REFLECT101, IPFG_PREC_DEFAULT, st);
+    double* rows = static_cast<double*>(sc.alloc(3 * static_cast<size_t>(W) * H * sizeof(double)));
+    HIPCALL(launch_structure_response(dx, dy, block, harris, k, rows, io.dst(), s));
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
 }  // namespace
 
 size_t batch_lds_bytes(int cls) {
@@ -2565,6 +2685,25 @@ int ipfg_emboss(const ipfg_image* src, const ipfg_image* dst, int precision, ipf
 int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type,
                             int block_size, double c, int precision, ipfg_stream stream) {
     return guarded([&] { adaptive_impl(src, dst, maxval, method, type, block_size, c, precision, stream); });
+}
+
+int ipfg_canny(const ipfg_image* src, const ipfg_image* dst, double low_threshold, double high_threshold, int ksize,
+               int l2_gradient, ipfg_stream stream) {
+    return guarded([&] { canny_impl(src, dst, low_threshold, high_threshold, ksize, l2_gradient, stream); });
+}
+
+int ipfg_canny_auto(const ipfg_image* src, const ipfg_image* dst, double blur_sigma, ipfg_stream stream) {
+    return guarded([&] { canny_auto_impl(src, dst, blur_sigma, stream); });
+}
+
+int ipfg_corner_harris(const ipfg_image* src, const ipfg_image* dst, int block_size, int ksize, double k,
+                       ipfg_stream stream) {
+    return guarded([&] { corner_impl(src, dst, block_size, ksize, true, k, stream); });
+}
+
+int ipfg_corner_min_eigen_val(const ipfg_image* src, const ipfg_image* dst, int block_size, int ksize,
+                              ipfg_stream stream) {
+    return guarded([&] { corner_impl(src, dst, block_size, ksize, false, 0.0, stream); });
 }
 
 int ipfg_batch_create(ipfg_batch* batch, int max_jobs, int use_graph) {

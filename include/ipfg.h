@@ -247,6 +247,30 @@ int ipfg_pyr_up(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream
 int ipfg_ssim(const ipfg_image* a, const ipfg_image* b, const ipfg_image* ssim_map, double* result,
               ipfg_stream stream);
 
+/* ---------------------------------------------------------------- features
+ * canny(src, low, high, ksize, l2Gradient), features.hpp:26-90: dst is a
+ * one-channel uint8 image of src's size (255 on edges, 0 elsewhere); any pixel
+ * type and channel count. Sobel gradients (Replicate borders) as the
+ * reference computes them, the strongest channel per pixel, non-maximum
+ * suppression and hysteresis; bit-identical to the reference.
+ * cannyAuto(src, blurSigma), features.hpp:92-101: toGray (1-4 channels),
+ * gaussianBlur<float>, the high threshold at the 70th percentile of the L2
+ * Sobel magnitude (selected on the device; no host round trip), low = 0.4
+ * high. Same output and parity as ipfg_canny. */
+int ipfg_canny(const ipfg_image* src, const ipfg_image* dst, double low_threshold, double high_threshold, int ksize,
+               int l2_gradient, ipfg_stream stream);
+int ipfg_canny_auto(const ipfg_image* src, const ipfg_image* dst, double blur_sigma, ipfg_stream stream);
+
+/* cornerHarris(src, blockSize, ksize, k), features.hpp:128-139, and
+ * cornerMinEigenVal(src, blockSize, ksize), features.hpp:141-153: src one
+ * channel of any pixel type, dst a one-channel float32 image of its size.
+ * The structure tensor's unnormalised box sums follow the reference's running
+ * sums in FP64 (boxFilter, filter.hpp:205-253); bit-identical results. */
+int ipfg_corner_harris(const ipfg_image* src, const ipfg_image* dst, int block_size, int ksize, double k,
+                       ipfg_stream stream);
+int ipfg_corner_min_eigen_val(const ipfg_image* src, const ipfg_image* dst, int block_size, int ksize,
+                              ipfg_stream stream);
+
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference (its filters run one image per call).
  * A batch runs many uint8 images (1, 3 or 4 channels, any sizes) through

@@ -60,6 +60,15 @@ int errors() {
     bad += expect_error("prewitt", [&] { prewitt(img, 0, 0); }, "ipf: prewitt: (dx, dy) must be (1,0) or (0,1)");
     bad += expect_error("magnitude", [&] { magnitude(ImageF(3, 2), ImageF(2, 3)); }, "ipf: magnitude: shape mismatch");
     bad += expect_error("phase", [&] { phase(ImageF(3, 2), ImageF(3, 2, 2)); }, "ipf: phase: shape mismatch");
+    bad += expect_error("canny thresholds", [&] { canny(img, 9.0, 3.0); },
+                        "ipf: canny: lowThreshold must be <= highThreshold");
+    bad += expect_error("canny ksize", [&] { canny(img, 1.0, 3.0, 2); }, "ipf: derivKernel: ksize must be odd in [3, 31]");
+    bad += expect_error("cannyAuto gray", [&] { cannyAuto(ImageU8(4, 4, 5)); },
+                        "ipf: rgbToGray: expected a 3-channel (RGB) or 4-channel (RGBA) image");
+    bad += expect_error("harris channels", [&] { cornerHarris(ImageU8(4, 4, 3)); },
+                        "ipf: corner detection: expected a single-channel image");
+    bad += expect_error("min eigen block", [&] { cornerMinEigenVal(img, 0); },
+                        "ipf: boxFilter: kernel size must be positive");
     bad += expect_error("adaptive channels",
                         [&] { adaptiveThreshold(ImageU8(4, 4, 3), 255, AdaptiveMethod::Mean, ThresholdType::Binary, 3, 0); },
                         "ipf: adaptiveThreshold: expected a single-channel image");
@@ -144,6 +153,12 @@ int run(const std::string& dir) {
     const ImageU8 dn = pyrDown(c), upc = pyrUp(c);
     if (!(dn.width() == 7 && dn.height() == 4 && dn == ImageU8(7, 4, 3, 90))) return 1;
     if (!(upc.width() == 26 && upc == ImageU8(26, 16, 3, 90))) return 1;
+    // feature callers (features.hpp:26-153)
+    save(dir + "/o24.bin", canny(img, 40.0, 120.0));
+    save(dir + "/o25.bin", canny(imgf, 0.5, 1.5, 5, true));
+    save(dir + "/o26.bin", cannyAuto(img));
+    save(dir + "/o27.bin", cornerHarris(gray, 3, 3, 0.04));
+    save(dir + "/o28.bin", cornerMinEigenVal(imgf, 5, 5));
     std::printf("RUN OK\n");
     return 0;
 }

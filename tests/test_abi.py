@@ -103,6 +103,17 @@ def test_argument_errors_precede_device_work():
         (lambda: _raw_pyr("ipfg_pyr_up", img, np.zeros((8, 10), np.uint16)),
          "ipfg: pyrUp: dst must have the pixel type of src"),
         (lambda: _raw_pyr("ipfg_pyr_up", img, np.zeros((8, 10, 2), np.uint8)), "ipfg: pyrUp: dst must be 2w x 2h"),
+        (lambda: ipf.canny(img, 5.0, 3.0), "ipf: canny: lowThreshold must be <= highThreshold"),
+        (lambda: ipf.canny(img, 5.0, 3.0, 4), "ipf: canny: lowThreshold must be <= highThreshold"),
+        (lambda: ipf.canny(img, 1.0, 3.0, 4), "ipf: derivKernel: ksize must be odd in [3, 31]"),
+        (lambda: ipf.canny_auto(img, 0.0), "ipf: gaussianKernel1D: need sigma > 0 or ksize > 0"),
+        (lambda: ipf.canny_auto(np.zeros((4, 5, 5), np.uint8), 0.0),
+         "ipf: rgbToGray: expected a 3-channel (RGB) or 4-channel (RGBA) image"),
+        (lambda: ipf.corner_harris(np.zeros((4, 5, 3), np.uint8), 0, 4),
+         "ipf: corner detection: expected a single-channel image"),
+        (lambda: ipf.corner_harris(img, 0, 4), "ipf: derivKernel: ksize must be odd in [3, 31]"),
+        (lambda: ipf.corner_min_eigen_val(img, 0, 3), "ipf: boxFilter: kernel size must be positive"),
+        (lambda: ipf.corner_min_eigen_val(img, -2, 5), "ipf: boxFilter: kernel size must be positive"),
     ]
     for fn, msg in cases:
         with pytest.raises(ipf.IpfError) as e:

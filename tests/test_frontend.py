@@ -73,6 +73,12 @@ def test_frontend_results_match_oracle(exe, oracle, tmp_path):
     assert np.array_equal(out(22, np.uint8, (31, 49, 3)), oracle.pyr_down(u8))
     assert np.array_equal(out(23, np.float32, (118, 166)), oracle.pyr_up(f))
     assert "LEVELS 7" in r.stdout
+    # feature callers
+    assert np.array_equal(out(24, np.uint8, u8.shape[:2]), oracle.canny(u8, 40.0, 120.0))
+    assert np.array_equal(out(25, np.uint8, f.shape), oracle.canny(f, 0.5, 1.5, 5, True))
+    assert np.array_equal(out(26, np.uint8, u8.shape[:2]), oracle.canny_auto(u8))
+    assert np.array_equal(out(27, np.float32, gray.shape), oracle.corner_harris(gray, 3, 3, 0.04))
+    assert np.array_equal(out(28, np.float32, f.shape), oracle.corner_min_eigen(f, 5, 5))
     level = u8
     for i in range(7):
         assert np.array_equal(np.fromfile(tmp_path / f"p{i}.bin", np.uint8).reshape(level.shape), level), i
