@@ -22,7 +22,8 @@ __all__ = [
     "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
     "sobel", "scharr", "prewitt", "laplacian", "magnitude", "phase", "gradient_magnitude", "unsharp_mask",
     "sharpen", "emboss", "adaptive_threshold", "motion_blur_kernel", "AdaptiveMethod", "ThresholdType",
-    "pyr_down", "pyr_up", "build_pyramid", "ssim",
+    "pyr_down", "pyr_up", "build_pyramid", "ssim", "filter2d_fft", "integral_image", "integral_sum", "canny", "canny_auto", "corner_harris",
+    "corner_min_eigen_val",
 ]
 
 
@@ -290,6 +291,43 @@ def filter2d(src, kernel, border=BorderMode.REFLECT101, delta=0.0, anchor=(-1, -
     _lib.check(_lib.load().ipfg_filter2d(C.byref(s), C.byref(d), k.ctypes.data_as(C.POINTER(C.c_float)), kw, kh,
                                          int(anchor[0]), int(anchor[1]), int(border), float(delta),
                                          float(border_value), int(precision), _sh(stream)))
+    return _ret(holder)
+
+
+def integral_image(src, squared=False, stream: Stream | None = None):
+    """integralImage (filter.hpp:522-538): float64 (H + 1, W + 1[, C]) summed-area table."""
+    if isinstance(src, DeviceImage):
+        dst = DeviceImage(src.height + 1, src.width + 1, src.channels, np.float64)
+        s, d, holder = src.descriptor(), dst.descriptor(), dst
+    else:
+        a = np.asarray(src)
+        if not _rows_contiguous(a):
+            a = np.ascontiguousarray(a)
+        h, w, c = _shape(a)
+        out = np.empty((h + 1, w + 1) if a.ndim == 2 else (h + 1, w + 1, c), np.float64)
+        s, d, holder = _desc(a), _desc(out), (out, a)
+    _lib.check(_lib.load().ipfg_integral_image(C.byref(s), C.byref(d), int(bool(squared)), _sh(stream)))
+    return _ret(holder)
+
+
+def integral_sum(table: np.ndarray, x: int, y: int, w: int, h: int, channel: int = 0) -> float:
+    """integralSum (filter.hpp:540-543): the sum over the rectangle (x, y, w, h) from a table."""
+    t = table if table.ndim == 3 else table[..., None]
+    r, b = x + w, y + h
+    return float(t[b, r, channel] - t[b, x, channel] - t[y, r, channel] + t[y, x, channel])
+
+
+def filter2d_fft(src, kernel, border=BorderMode.REFLECT101, delta=0.0, border_value=0.0, out_dtype=None,
+                 stream: Stream | None = None):
+    """filter2DFFT (fft.hpp:196-216): filter2D (anchor at the centre) through FP64 FFTs,
+    bit-identical to the reference's; kernel is a 2-D float array (kh, kw)."""
+    k = np.ascontiguousarray(kernel, np.float32)
+    if k.ndim != 2 or k.size == 0:
+        raise IpfError("ipf: filter2DFFT: kernel must be single-channel")
+    kh, kw = k.shape
+    s, d, holder = _io(src, out_dtype)
+    _lib.check(_lib.load().ipfg_filter2d_fft(C.byref(s), C.byref(d), k.ctypes.data_as(C.POINTER(C.c_float)), kw, kh,
+                                             int(border), float(delta), float(border_value), _sh(stream)))
     return _ret(holder)
 
 

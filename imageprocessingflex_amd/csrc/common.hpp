@@ -8,7 +8,7 @@
 #include <limits>
 #include <type_traits>
 
-#include "../../include/ipfg.h"
+#include "../../include/ipfg_types.h"
 
 namespace ipfg {
 

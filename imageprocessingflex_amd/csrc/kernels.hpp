@@ -155,6 +155,11 @@ hipError_t launch_select_rank(const float* v, size_t n, unsigned long long k, un
 hipError_t launch_structure_response(const DevImage& dx, const DevImage& dy, int block, bool harris, double k,
                                      double* rows, const DevImage& dst, hipStream_t s);
 
+// integralImage (filter.hpp:522-538) of src into the (W + 1) x (H + 1) x cn
+// double table I (rows istride_samples doubles apart), in the reference's
+// summation order.
+hipError_t launch_integral(const DevImage& src, double* I, size_t istride_samples, bool squared, hipStream_t s);
+
 // Zeroes a stream's fix-up counter and ticket with a kernel.
 hipError_t launch_zero_counter(unsigned long long* count, unsigned int* done, hipStream_t s);
 

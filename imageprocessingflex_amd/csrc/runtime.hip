@@ -18,8 +18,10 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/ipfg.h"
 #include "arena.hpp"
 #include "batch.hpp"
+#include "fft.hpp"
 #include "kernels.hpp"
 #include "sep_kernels.hpp"
 #include "tuner.hpp"
@@ -1836,6 +1838,95 @@ void corner_impl(const ipfg_image* src, const ipfg_image* dst, int block, int ks
     io.finish();
 }
 
+// ------------------------------------------------------------------ integralImage
+// integralImage (filter.hpp:522-538): dst is a float64 (W + 1) x (H + 1) image
+// with src's channels.
+void integral_impl(const ipfg_image* src, const ipfg_image* dst, int squared, ipfg_stream stream) {
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(dst->dtype == IPFG_F64 && dst->width == src->width + 1 && dst->height == src->height + 1 &&
+                  dst->channels == src->channels,
+              "integralImage: dst must be a float64 (W + 1) x (H + 1) image with the source's channels");
+    abi_check(dst->row_stride_bytes % sizeof(double) == 0, "integralImage: dst row stride must be a multiple of 8");
+    ensure_init();
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc, true);
+    const DevImage d = io.dst();
+    HIPCALL(launch_integral(io.src(), static_cast<double*>(d.data), d.stride / sizeof(double), squared != 0, s));
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
+// ------------------------------------------------------------------ filter2DFFT
+// Twiddles exp(i sign 2 pi k / len) of every radix-2 stage of an n-point
+// transform (len = 2 .. n), evaluated as the reference does (fft.hpp:38-42:
+// angle sign * 2 * pi * k / len, then cos and sin of the host libm); the
+// stage of length len starts at entry len / 2 - 1.
+std::vector<double> fft_twiddles(int n, int sign) {
+    constexpr double kPi = 3.14159265358979323846;
+    std::vector<double> t(2 * static_cast<size_t>(std::max(n - 1, 1)), 0.0);
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len / 2;
+        for (int k = 0; k < half; ++k) {
+            const double ang = sign * 2.0 * kPi * static_cast<double>(k) / static_cast<double>(len);
+            t[2 * static_cast<size_t>(half - 1 + k)] = std::cos(ang);
+            t[2 * static_cast<size_t>(half - 1 + k) + 1] = std::sin(ang);
+        }
+    }
+    return t;
+}
+
+// filter2DFFT (fft.hpp:196-216)
+void fft_impl(const ipfg_image* src, const ipfg_image* dst, const float* kernel, int kw, int kh, int border,
+              double delta, double bval, ipfg_stream stream) {
+    require(kernel != nullptr && kw > 0 && kh > 0, "filter2DFFT: kernel must be single-channel");
+    check_image(src, "null src image");
+    check_image(dst, "null dst image");
+    abi_check(border >= IPFG_BORDER_CONSTANT && border <= IPFG_BORDER_WRAP, "invalid border mode");
+    require(static_cast<long long>(src->width) * src->height > 0 || border == IPFG_BORDER_CONSTANT,
+            "copyMakeBorder: empty source image");
+    ensure_init();
+    if (static_cast<long long>(src->width) * src->height == 0) return;
+    const int lw = ceil_log2(static_cast<long long>(src->width) + kw - 1);
+    const int lh = ceil_log2(static_cast<long long>(src->height) + kh - 1);
+    if (!fft_shape_ok(lw, lh))
+        throw Unsupported{"ipfg: filter2DFFT: the padded image needs more than 2^28 transform points"};
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    Staged io(src, dst, sc);
+    // parameters: four twiddle tables, then the kernel (floats)
+    const std::vector<double> t[4] = {fft_twiddles(1 << lw, -1), fft_twiddles(1 << lh, -1), fft_twiddles(1 << lw, 1),
+                                      fft_twiddles(1 << lh, 1)};
+    size_t off[5] = {0};
+    for (int i = 0; i < 4; ++i) off[i + 1] = off[i] + t[i].size() * sizeof(double);
+    std::vector<char> blob(off[4] + static_cast<size_t>(kw) * kh * sizeof(float));
+    for (int i = 0; i < 4; ++i) std::memcpy(blob.data() + off[i], t[i].data(), t[i].size() * sizeof(double));
+    std::memcpy(blob.data() + off[4], kernel, static_cast<size_t>(kw) * kh * sizeof(float));
+    char* dev = static_cast<char*>(sc.params(blob));
+    FftLaunch a{};
+    a.src = io.src();
+    a.dst = io.dst();
+    a.kw = kw;
+    a.kh = kh;
+    a.border = border;
+    a.bval = bval;
+    a.delta = delta;
+    a.kernel = reinterpret_cast<const float*>(dev + off[4]);
+    a.tw_fwd_w = reinterpret_cast<const double2*>(dev + off[0]);
+    a.tw_fwd_h = reinterpret_cast<const double2*>(dev + off[1]);
+    a.tw_inv_w = reinterpret_cast<const double2*>(dev + off[2]);
+    a.tw_inv_h = reinterpret_cast<const double2*>(dev + off[3]);
+    a.lw = lw;
+    a.lh = lh;
+    const size_t bytes = sizeof(double2) << (lw + lh);
+    a.buf0 = static_cast<double2*>(sc.alloc(bytes));
+    a.buf1 = static_cast<double2*>(sc.alloc(bytes));
+    HIPCALL(launch_filter2d_fft(a, s));
+    t_ctx.fix_valid = false;
+    io.finish();
+}
+
 }  // namespace
 
 size_t batch_lds_bytes(int cls) {
@@ -2685,6 +2776,15 @@ int ipfg_emboss(const ipfg_image* src, const ipfg_image* dst, int precision, ipf
 int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type,
                             int block_size, double c, int precision, ipfg_stream stream) {
     return guarded([&] { adaptive_impl(src, dst, maxval, method, type, block_size, c, precision, stream); });
+}
+
+int ipfg_integral_image(const ipfg_image* src, const ipfg_image* dst, int squared, ipfg_stream stream) {
+    return guarded([&] { integral_impl(src, dst, squared, stream); });
+}
+
+int ipfg_filter2d_fft(const ipfg_image* src, const ipfg_image* dst, const float* kernel, int kw, int kh, int border,
+                      double delta, double border_value, ipfg_stream stream) {
+    return guarded([&] { fft_impl(src, dst, kernel, kw, kh, border, delta, border_value, stream); });
 }
 
 int ipfg_canny(const ipfg_image* src, const ipfg_image* dst, double low_threshold, double high_threshold, int ksize,

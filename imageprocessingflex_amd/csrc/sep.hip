@@ -100,7 +100,9 @@ static bool sep1_fits(const DevImage& src, const DevImage& dst, int kw, int kh, 
 
 static hipError_t sep1_launch(const SepArgs& a, hipStream_t s) {
     if (!sep1_fits(a.src, a.dst, a.kw, a.kh, a.ax, a.ay)) return hipErrorNotSupported;
-    return a.kw <= 25 ? sep1::launch_small(a, s) : sep1::launch_large(a, s);
+    if (a.kw <= 13) return sep1::launch_k3_13(a, s);
+    if (a.kw <= 25) return sep1::launch_k15_25(a, s);
+    return a.kw <= 37 ? sep1::launch_k27_37(a, s) : sep1::launch_k39_49(a, s);
 }
 
 static bool sep_planar(int cn, int src_dt, int dst_dt) {

@@ -416,8 +416,11 @@ hipError_t sep1_launch_k(const SepArgs& a, hipStream_t s) {
 
 // Dispatch over the instantiated sizes; hipErrorNotSupported for calls this
 // kernel does not take.
-hipError_t launch_small(const SepArgs& a, hipStream_t s);  // K = 3 .. 25
-hipError_t launch_large(const SepArgs& a, hipStream_t s);  // K = 27 .. 49
+// Four translation units share the sizes so that they compile in parallel.
+hipError_t launch_k3_13(const SepArgs& a, hipStream_t s);   // K = 3 .. 13
+hipError_t launch_k15_25(const SepArgs& a, hipStream_t s);  // K = 15 .. 25
+hipError_t launch_k27_37(const SepArgs& a, hipStream_t s);  // K = 27 .. 37
+hipError_t launch_k39_49(const SepArgs& a, hipStream_t s);  // K = 39 .. 49
 
 template <int KLO, int KHI>
 hipError_t launch_range(const SepArgs& a, hipStream_t s) {

@@ -8,7 +8,7 @@
 //   medianBlur (300-332), bilateralFilter (334-387), and the derivative and
 //   sharpening filters of 388-565: sobel, scharr, prewitt, laplacian,
 //   magnitude, phase, gradientMagnitude, unsharpMask, sharpen, emboss,
-//   motionBlurKernel, adaptiveThreshold.
+//   motionBlurKernel, adaptiveThreshold, integralImage and integralSum.
 // Every call validates its arguments in the reference's order, then runs the
 // gfx950 kernels of libipfg.so through the C ABI (ipfg.h) and returns when
 // the result is in the output image. Link with -lipfg.
@@ -334,6 +334,22 @@ inline Kernel motionBlurKernel(int length, double angleDegrees) {
     for (int y = 0; y < n; ++y)
         for (int x = 0; x < n; ++x) k(x, y) /= total;
     return k;
+}
+
+/// Summed-area table with an extra zero row/column: I(x, y) = sum of src over [0,x) x [0,y).
+/// Size (W+1) x (H+1), same channel count, double precision.
+template <class T>
+ImageD integralImage(const Image<T>& src, bool squared = false) {
+    ImageD I(src.width() + 1, src.height() + 1, src.channels());
+    const ipfg_image s = detail::describe(src), d = detail::describe(I);
+    detail::check(ipfg_integral_image(&s, &d, squared ? 1 : 0, gpu::streamSetting()));
+    return I;
+}
+
+/// Sum over the rectangle r using an integral image from integralImage().
+inline double integralSum(const ImageD& I, Rect r, int channel = 0) {
+    return I(r.right(), r.bottom(), channel) - I(r.x, r.bottom(), channel) - I(r.right(), r.y, channel) +
+           I(r.x, r.y, channel);
 }
 
 /// Threshold types (the reference declares them in pixel_ops.hpp:253-259);

@@ -30,6 +30,8 @@
 //   canny / cannyAuto ....... include/ipf/features.hpp:26-101
 //   toGray / rgbToGray ...... include/ipf/color.hpp:51-81, core.hpp:465-473
 //   corner responses ........ include/ipf/features.hpp:107-160
+//   filter2DFFT ............. include/ipf/fft.hpp:21-98, 163-216; core.hpp:526-544
+//   integralImage ........... include/ipf/filter.hpp:522-538
 //
 // Pinning: every function here is pinned by tests/test_oracle.py against the
 // reference-built fixtures in tests/golden/ and against closed forms, brute force
@@ -37,6 +39,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -1154,6 +1157,421 @@ int ipfo_corner_min_eigen(const void* src, int w, int h, int cn, int dtype, floa
             for (size_t i = 0; i < a.size(); ++i) {
                 const double A = a[i] * 0.5, B = b[i], C = c[i] * 0.5;
                 dst[i] = static_cast<float>((A + C) - std::sqrt((A - C) * (A - C) + B * B));
+            }
+            return 0;
+        });
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- filter2DFFT
+namespace {
+
+using cplx = std::complex<double>;
+constexpr double kPiFft = 3.14159265358979323846;  // core.hpp:45
+
+size_t pow2_ceil(size_t n) {
+    size_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// Radix-2 decimation in time on a power-of-two length (fft.hpp:29-50): the
+// bit-reversal permutation, then stages of length 2, 4, ..., n whose
+// twiddles exp(i sign 2 pi k / len) are evaluated directly per stage (angle
+// formed as sign * 2 * pi * k / len, left to right).
+void fft_radix2(cplx* a, size_t n, int sign) {
+    if (n <= 1) return;
+    size_t j = 0;
+    for (size_t i = 1; i < n; ++i) {
+        size_t bit = n >> 1;
+        while (j & bit) {
+            j ^= bit;
+            bit >>= 1;
+        }
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    std::vector<cplx> tw;
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t half = len / 2;
+        tw.resize(half);
+        for (size_t k = 0; k < half; ++k) {
+            const double ang = sign * 2.0 * kPiFft * static_cast<double>(k) / static_cast<double>(len);
+            tw[k] = cplx(std::cos(ang), std::sin(ang));
+        }
+        for (size_t base = 0; base < n; base += len)
+            for (size_t k = 0; k < half; ++k) {
+                const cplx u = a[base + k];
+                const cplx v = a[base + k + half] * tw[k];
+                a[base + k] = u + v;
+                a[base + k + half] = u - v;
+            }
+    }
+}
+
+// fft2D (fft.hpp:79-92): every row, then every column (power-of-two sides).
+void fft_2d(std::vector<cplx>& d, int W, int H, int sign) {
+    rows_parallel(H, static_cast<double>(W) * 20, [&](int y0, int y1) {
+        for (int y = y0; y < y1; ++y) fft_radix2(&d[static_cast<size_t>(y) * W], static_cast<size_t>(W), sign);
+    });
+    rows_parallel(W, static_cast<double>(H) * 20, [&](int x0, int x1) {
+        std::vector<cplx> col(static_cast<size_t>(H));
+        for (int x = x0; x < x1; ++x) {
+            for (int y = 0; y < H; ++y) col[y] = d[static_cast<size_t>(y) * W + x];
+            fft_radix2(col.data(), static_cast<size_t>(H), sign);
+            for (int y = 0; y < H; ++y) d[static_cast<size_t>(y) * W + x] = col[y];
+        }
+    });
+}
+
+// filter2DFFT (fft.hpp:196-216) with correlateFFT (fft.hpp:167-193): the source
+// as double, padded by copyMakeBorder (core.hpp:526-544: ay rows above, kh-1-ay
+// below, ax columns left, kw-1-ax right, Constant cells = borderValue), placed
+// per channel in the real part of a pow2 x pow2 zero buffer with the kernel in
+// the imaginary part; one forward 2-D FFT; the two real spectra separated as
+// (Z(k) + conj Z(-k)) / 2 and (Z(k) - conj Z(-k)) * (-i / 2); their product
+// with the kernel's spectrum conjugated; the inverse 2-D FFT; the real part
+// times 1 / (fw fh), plus delta, saturated.
+template <class Ti, class To>
+void filter2d_fft(const Ti* src, int W, int H, int cn, To* dst, const float* K, int kw, int kh, int border,
+                  double delta, double bval) {
+    const int ax = kw / 2, ay = kh / 2;
+    const int PW = W + kw - 1, PH = H + kh - 1;
+    std::vector<double> pad(static_cast<size_t>(PW) * PH * cn, bval);
+    for (int y = 0; y < PH; ++y) {
+        const int sy = border_index(y - ay, H, border);
+        if (sy < 0) continue;
+        for (int x = 0; x < PW; ++x) {
+            const int sx = border_index(x - ax, W, border);
+            if (sx < 0) continue;
+            for (int c = 0; c < cn; ++c)
+                pad[(static_cast<size_t>(y) * PW + x) * cn + c] =
+                    static_cast<double>(src[(static_cast<size_t>(sy) * W + sx) * cn + c]);
+        }
+    }
+    const int fw = static_cast<int>(pow2_ceil(static_cast<size_t>(PW)));
+    const int fh = static_cast<int>(pow2_ceil(static_cast<size_t>(PH)));
+    std::vector<cplx> A(static_cast<size_t>(fw) * fh), B(A.size());
+    const double s = 1.0 / (static_cast<double>(fw) * fh);
+    for (int c = 0; c < cn; ++c) {
+        std::fill(A.begin(), A.end(), cplx(0, 0));
+        for (int y = 0; y < PH; ++y)
+            for (int x = 0; x < PW; ++x)
+                A[static_cast<size_t>(y) * fw + x].real(pad[(static_cast<size_t>(y) * PW + x) * cn + c]);
+        for (int y = 0; y < kh; ++y)
+            for (int x = 0; x < kw; ++x)
+                A[static_cast<size_t>(y) * fw + x].imag(static_cast<double>(K[static_cast<size_t>(y) * kw + x]));
+        fft_2d(A, fw, fh, -1);
+        for (int v = 0; v < fh; ++v)
+            for (int u = 0; u < fw; ++u) {
+                const cplx z = A[static_cast<size_t>(v) * fw + u];
+                const cplx zm = std::conj(A[static_cast<size_t>((fh - v) % fh) * fw + (fw - u) % fw]);
+                const cplx fi = (z + zm) * 0.5;
+                const cplx fk = (z - zm) * cplx(0, -0.5);
+                B[static_cast<size_t>(v) * fw + u] = fi * std::conj(fk);
+            }
+        fft_2d(B, fw, fh, +1);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+                dst[(static_cast<size_t>(y) * W + x) * cn + c] =
+                    sat<To>(B[static_cast<size_t>(y) * fw + x].real() * s + delta);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// filter2DFFT (fft.hpp:196-216); the kernel's anchor is its centre.
+int ipfo_filter2d_fft(const void* src, int w, int h, int cn, int tin, void* dst, int tout, const float* K, int kw,
+                      int kh, int border, double delta, double bval) {
+    return guarded([&] {
+        check(K != nullptr && kw > 0 && kh > 0, "filter2DFFT: kernel must be single-channel");
+        check(static_cast<long long>(w) * h > 0 || border == B_CONSTANT, "copyMakeBorder: empty source image");
+        check_image(w, h, cn);
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                filter2d_fft<Ti, To>(static_cast<const Ti*>(src), w, h, cn, static_cast<To*>(dst), K, kw, kh, border,
+                                     delta, bval);
+                return 0;
+            });
+        });
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- integralImage
+namespace {
+
+// integralImage (filter.hpp:522-538): row 0 and column 0 zero; per row a
+// running sum (of v or v * v, in FP64) left to right, each entry the one above
+// plus that running sum.
+template <class T>
+void integral_image(const T* src, int W, int H, int cn, double* I, bool squared) {
+    const size_t row = static_cast<size_t>(W + 1) * cn;
+    std::fill(I, I + row * static_cast<size_t>(H + 1), 0.0);
+    std::vector<double> run(static_cast<size_t>(cn));
+    for (int y = 0; y < H; ++y) {
+        std::fill(run.begin(), run.end(), 0.0);
+        const double* above = I + static_cast<size_t>(y) * row;
+        double* here = I + static_cast<size_t>(y + 1) * row;
+        for (int x = 0; x < W; ++x)
+            for (int c = 0; c < cn; ++c) {
+                const double v = static_cast<double>(src[(static_cast<size_t>(y) * W + x) * cn + c]);
+                run[c] += squared ? v * v : v;
+                here[static_cast<size_t>(x + 1) * cn + c] = above[static_cast<size_t>(x + 1) * cn + c] + run[c];
+            }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipfo_integral_image(const void* src, int w, int h, int cn, int dtype, double* dst, int squared) {
+    return guarded([&] {
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            integral_image<T>(static_cast<const T*>(src), w, h, cn, dst, squared != 0);
+            return 0;
+        });
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- houghCircles
+namespace {
+
+// dilate with a 3 x 3 rectangle (morphology.hpp:50-79): the maximum over the
+// in-image part of the window, rows then columns.
+std::vector<float> dilate3(const std::vector<float>& a, int W, int H) {
+    std::vector<float> t(a.size()), o(a.size());
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float v = a[static_cast<size_t>(y) * W + x];
+            for (int i = std::max(0, x - 1); i <= std::min(W - 1, x + 1); ++i)
+                if (v < a[static_cast<size_t>(y) * W + i]) v = a[static_cast<size_t>(y) * W + i];
+            t[static_cast<size_t>(y) * W + x] = v;
+        }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float v = t[static_cast<size_t>(y) * W + x];
+            for (int j = std::max(0, y - 1); j <= std::min(H - 1, y + 1); ++j)
+                if (v < t[static_cast<size_t>(j) * W + x]) v = t[static_cast<size_t>(j) * W + x];
+            o[static_cast<size_t>(y) * W + x] = v;
+        }
+    return o;
+}
+
+struct Circle {
+    double cx, cy, r;
+    int votes;
+};
+
+// houghCircles (features.hpp:388-504): canny edges vote for centres along
+// their gradient; accumulator peaks (3 x 3 box sums, findPeaks
+// features.hpp:164-187) are refined: vote-weighted centroid, best radius by
+// radially aligned support, two Kasa least-squares fits, angular coverage in
+// 36 sectors; then sorted by support and de-duplicated by minDist.
+template <class T>
+std::vector<Circle> hough_circles(const T* src, int W, int H, int minR, int maxR, double high, int centerVotes,
+                                  double minDist, double minCoverage) {
+    const size_t n = static_cast<size_t>(W) * H;
+    std::vector<uint8_t> edges(n);
+    canny<T>(src, W, H, 1, edges.data(), high / 2, high, 3, false);
+    const int ks = ipfo_gaussian_kernel_1d(1.0, 0, nullptr, 0);
+    std::vector<double> k(static_cast<size_t>(ks));
+    ipfo_gaussian_kernel_1d(1.0, 0, k.data(), ks);
+    std::vector<float> smooth(n), dx(n), dy(n);
+    sep_filter<T, float>(src, W, H, 1, smooth.data(), k.data(), ks, k.data(), ks, ks / 2, ks / 2, B_REFLECT101, 0.0,
+                         0.0);
+    sobel<float>(smooth.data(), W, H, 1, dx.data(), 1, 0, 3, 1.0, B_REFLECT101);
+    sobel<float>(smooth.data(), W, H, 1, dy.data(), 0, 1, 3, 1.0, B_REFLECT101);
+    struct Pt {
+        double x, y, nx, ny;
+    };
+    std::vector<Pt> pts;
+    std::vector<float> acc(n, 0.f);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t p = static_cast<size_t>(y) * W + x;
+            if (!edges[p]) continue;
+            const double gx = dx[p], gy = dy[p], g = std::hypot(gx, gy);
+            if (g < 1e-9) continue;
+            pts.push_back({static_cast<double>(x), static_cast<double>(y), gx / g, gy / g});
+            for (int sgn = -1; sgn <= 1; sgn += 2)
+                for (int r = minR; r <= maxR; ++r) {
+                    const int cx = static_cast<int>(std::lround(x + sgn * r * gx / g));
+                    const int cy = static_cast<int>(std::lround(y + sgn * r * gy / g));
+                    if (cx >= 0 && cy >= 0 && cx < W && cy < H) acc[static_cast<size_t>(cy) * W + cx] += 1.f;
+                }
+        }
+    const double kAlign = 0.92;
+    auto radial = [](const Pt& e, double cx, double cy, double& dist) {
+        const double ux = e.x - cx, uy = e.y - cy;
+        dist = std::hypot(ux, uy);
+        return dist > 1e-9 ? std::fabs(ux * e.nx + uy * e.ny) / dist : 0.0;
+    };
+    // boxFilter(acc, 3, 3, false, Constant): sums of small integers, exact
+    std::vector<float> sm(n);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double s = 0;
+            for (int j = -1; j <= 1; ++j)
+                for (int i = -1; i <= 1; ++i) {
+                    const int sx = x + i, sy = y + j;
+                    if (sx >= 0 && sy >= 0 && sx < W && sy < H) s += acc[static_cast<size_t>(sy) * W + sx];
+                }
+            sm[static_cast<size_t>(y) * W + x] = static_cast<float>(s);
+        }
+    // findPeaks(sm, centerVotes - 0.5, minDist)
+    const std::vector<float> dil = dilate3(sm, W, H);
+    const float thr = static_cast<float>(centerVotes) - 0.5f;
+    struct Peak {
+        int x, y;
+        float v;
+    };
+    std::vector<Peak> cand, peaks;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const float v = sm[static_cast<size_t>(y) * W + x];
+            if (v > thr && v >= dil[static_cast<size_t>(y) * W + x]) cand.push_back({x, y, v});
+        }
+    std::stable_sort(cand.begin(), cand.end(), [](const Peak& a, const Peak& b) { return a.v > b.v; });
+    const double md2 = minDist * minDist;
+    for (const Peak& p : cand) {
+        bool ok = true;
+        for (const Peak& o : peaks) {
+            const double ddx = p.x - o.x, ddy = p.y - o.y;
+            if (ddx * ddx + ddy * ddy < md2) {
+                ok = false;
+                break;
+            }
+        }
+        if (ok) peaks.push_back(p);
+    }
+    constexpr double kPi = 3.14159265358979323846;
+    std::vector<Circle> circles;
+    for (const Peak& p : peaks) {
+        double sw = 0, scx = 0, scy = 0;
+        for (int j = -2; j <= 2; ++j)
+            for (int i = -2; i <= 2; ++i) {
+                const int x = p.x + i, y = p.y + j;
+                if (x < 0 || y < 0 || x >= W || y >= H) continue;
+                const double v = acc[static_cast<size_t>(y) * W + x];
+                sw += v;
+                scx += v * x;
+                scy += v * y;
+            }
+        double cx = sw > 0 ? scx / sw : p.x, cy = sw > 0 ? scy / sw : p.y;
+        std::vector<int> hist(static_cast<size_t>(maxR + 2), 0);
+        for (const Pt& e : pts) {
+            double d;
+            if (radial(e, cx, cy, d) < kAlign) continue;
+            const int r = static_cast<int>(std::lround(d));
+            if (r >= minR && r <= maxR) ++hist[r];
+        }
+        int bestR = 0;
+        double bestScore = 0;
+        for (int r = minR; r <= maxR; ++r) {
+            const double score = (hist[r] + 0.5 * (hist[r - 1] + hist[r + 1])) / (2 * kPi * r);
+            if (score > bestScore) {
+                bestScore = score;
+                bestR = r;
+            }
+        }
+        if (!bestR) continue;
+        double radius = bestR;
+        for (int iter = 0; iter < 2; ++iter) {
+            double sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0, sz = 0, sxz = 0, syz = 0, cnt = 0;
+            for (const Pt& e : pts) {
+                double d;
+                if (radial(e, cx, cy, d) < kAlign || std::fabs(d - radius) > 2.0) continue;
+                const double x = e.x - cx, y = e.y - cy, z = x * x + y * y;
+                sx += x;
+                sy += y;
+                sxx += x * x;
+                syy += y * y;
+                sxy += x * y;
+                sz += z;
+                sxz += x * z;
+                syz += y * z;
+                cnt += 1;
+            }
+            if (cnt < 3) break;
+            const double m[9] = {sxx, sxy, sx, sxy, syy, sy, sx, sy, cnt}, rhs[3] = {-sxz, -syz, -sz};
+            auto det3 = [](const double* t) {
+                return t[0] * (t[4] * t[8] - t[5] * t[7]) - t[1] * (t[3] * t[8] - t[5] * t[6]) +
+                       t[2] * (t[3] * t[7] - t[4] * t[6]);
+            };
+            const double det = det3(m);
+            if (std::fabs(det) < 1e-12) break;
+            double sol[3];
+            for (int col = 0; col < 3; ++col) {
+                double t[9];
+                std::copy(m, m + 9, t);
+                for (int r = 0; r < 3; ++r) t[r * 3 + col] = rhs[r];
+                sol[col] = det3(t) / det;
+            }
+            const double ox = -sol[0] / 2, oy = -sol[1] / 2, r2 = ox * ox + oy * oy - sol[2];
+            if (r2 <= 0) break;
+            cx = cx + ox;
+            cy = cy + oy;
+            radius = std::sqrt(r2);
+        }
+        if (radius < minR - 1 || radius > maxR + 1) continue;
+        bool sector[36] = {};
+        int support = 0;
+        for (const Pt& e : pts) {
+            double d;
+            if (radial(e, cx, cy, d) < kAlign || std::fabs(d - radius) > 1.5) continue;
+            ++support;
+            const double a = std::atan2(e.y - cy, e.x - cx) + kPi;
+            sector[std::min(35, static_cast<int>(a / (2 * kPi) * 36))] = true;
+        }
+        const int covered = static_cast<int>(std::count(std::begin(sector), std::end(sector), true));
+        if (covered < minCoverage * 36) continue;
+        circles.push_back({cx, cy, radius, support});
+    }
+    std::stable_sort(circles.begin(), circles.end(), [](const Circle& a, const Circle& b) { return a.votes > b.votes; });
+    std::vector<Circle> unique;
+    for (const Circle& c : circles) {
+        bool dup = false;
+        for (const Circle& u : unique) dup |= std::hypot(c.cx - u.cx, c.cy - u.cy) < minDist;
+        if (!dup) unique.push_back(c);
+    }
+    return unique;
+}
+
+}  // namespace
+
+extern "C" {
+
+// houghCircles: writes up to `capacity` circles as (cx, cy, radius, votes)
+// quadruples of doubles into out and their count into *count.
+int ipfo_hough_circles(const void* src, int w, int h, int cn, int dtype, int min_r, int max_r, double canny_high,
+                       int center_votes, double min_dist, double min_coverage, double* out, int capacity, int* count) {
+    return guarded([&] {
+        check(cn == 1, "houghCircles: expected a single-channel image");
+        check(min_r > 0 && max_r >= min_r, "houghCircles: invalid radius range");
+        check_image(w, h, cn);
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const std::vector<Circle> cs = hough_circles<T>(static_cast<const T*>(src), w, h, min_r, max_r, canny_high,
+                                                            center_votes, min_dist, min_coverage);
+            *count = static_cast<int>(cs.size());
+            for (int i = 0; i < static_cast<int>(cs.size()) && i < capacity; ++i) {
+                out[4 * i] = cs[i].cx;
+                out[4 * i + 1] = cs[i].cy;
+                out[4 * i + 2] = cs[i].r;
+                out[4 * i + 3] = cs[i].votes;
             }
             return 0;
         });

@@ -72,6 +72,9 @@ class Oracle:
             "sep_filter": (_I, [_V, _I, _I, _I, _I, _V, _I, _PD, _I, _PD, _I, _I, _I, _I, _D, _D]),
             "box_filter": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _I, _I, _I, _D]),
             "filter2d": (_I, [_V, _I, _I, _I, _I, _V, _I, _PF, _I, _I, _I, _I, _I, _D, _D]),
+            "hough_circles": (_I, [_V, _I, _I, _I, _I, _I, _I, _D, _I, _D, _D, _PD, _I, C.POINTER(_I)]),
+            "integral_image": (_I, [_V, _I, _I, _I, _I, _V, _I]),
+            "filter2d_fft": (_I, [_V, _I, _I, _I, _I, _V, _I, _PF, _I, _I, _I, _D, _D]),
             "median": (_I, [_V, _I, _I, _I, _I, _V, _I, _I]),
             "bilateral": (_I, [_V, _I, _I, _I, _I, _V, _I, _D, _D, _I]),
             "sobel": (_I, [_V, _I, _I, _I, _I, _V, _I, _I, _I, _D, _I]),
@@ -174,6 +177,39 @@ class Oracle:
         self._check(self._f("filter2d")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
                                         NP2DT[out.dtype], k.ctypes.data_as(_PF), kw, kh, anchor[0], anchor[1],
                                         border, delta, bval))
+        return out
+
+    def hough_circles(self, src, min_r, max_r, canny_high=100.0, center_votes=20, min_dist=10.0, min_coverage=0.6):
+        """houghCircles (features.hpp:388-504): (n, 4) float64 rows (cx, cy, radius, votes)."""
+        src, h, w, cn = self._shape(src)
+        cap = 256
+        while True:
+            out = np.zeros((cap, 4), np.float64)
+            n = C.c_int(0)
+            self._check(self._f("hough_circles")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], int(min_r), int(max_r),
+                                                 float(canny_high), int(center_votes), float(min_dist),
+                                                 float(min_coverage), out.ctypes.data_as(_PD), cap, C.byref(n)))
+            if n.value <= cap:
+                return out[:n.value].copy()
+            cap = n.value
+
+    def integral_image(self, src, squared=False):
+        """integralImage (filter.hpp:522-538): float64 (h + 1, w + 1[, cn]) table."""
+        a = np.ascontiguousarray(src)
+        src, h, w, cn = self._shape(src)
+        out = np.empty((h + 1, w + 1) if a.ndim == 2 else (h + 1, w + 1, cn), np.float64)
+        self._check(self._f("integral_image")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                              int(bool(squared))))
+        return out
+
+    def filter2d_fft(self, src, kernel, border=REFLECT101, delta=0.0, bval=0.0, out_dtype=None):
+        """filter2DFFT (fft.hpp:196-216): filter2D through FP64 radix-2 FFTs; anchor at the centre."""
+        src, h, w, cn = self._shape(src)
+        k = np.ascontiguousarray(kernel, np.float32)
+        kh, kw = k.shape
+        out = np.empty(src.shape, out_dtype or src.dtype)
+        self._check(self._f("filter2d_fft")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
+                                            NP2DT[out.dtype], k.ctypes.data_as(_PF), kw, kh, border, delta, bval))
         return out
 
     def median(self, src, ksize, border=REPLICATE):

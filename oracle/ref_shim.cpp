@@ -2,11 +2,12 @@
 //
 // C-ABI wrapper around the reference ipf headers, compiled by oracle/Makefile
 // from the reference source tree (include/ipf/filter.hpp, geometry.hpp,
-// core.hpp, parallel.hpp, features.hpp, color.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
+// core.hpp, parallel.hpp, features.hpp, color.hpp, fft.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
 // restatement bit-for-bit against the real reference, (2) write the golden
 // fixtures under tests/golden/, and (3) serve as the "reference" CPU baseline.
 // Signatures match oracle/ipf_oracle.cpp's ipfo_* functions one for one.
 #include <ipf/features.hpp>
+#include <ipf/fft.hpp>
 #include <ipf/filter.hpp>
 #include <ipf/geometry.hpp>
 #include <ipf/metrics.hpp>
@@ -139,6 +140,56 @@ int ipfr_filter2d(const void* src, int w, int h, int cn, int tin, void* dst, int
                 unwrap(out, dst);
                 return 0;
             });
+        });
+    });
+}
+
+int ipfr_filter2d_fft(const void* src, int w, int h, int cn, int tin, void* dst, int tout, const float* K, int kw,
+                      int kh, int border, double delta, double bval) {
+    return guarded([&] {
+        return with_type(tin, [&](auto* ti) {
+            using Ti = std::remove_pointer_t<decltype(ti)>;
+            return with_type(tout, [&](auto* to) {
+                using To = std::remove_pointer_t<decltype(to)>;
+                const auto img = wrap<Ti>(src, w, h, cn);
+                const ipf::ImageF k = ipf::ImageF::fromData(kw, kh, 1, K);
+                const auto out = timed([&] { return ipf::filter2DFFT<To>(img, k, bm(border), delta, bval); });
+                unwrap(out, dst);
+                return 0;
+            });
+        });
+    });
+}
+
+int ipfr_integral_image(const void* src, int w, int h, int cn, int dtype, double* dst, int squared) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            const auto out = timed([&] { return ipf::integralImage(img, squared != 0); });
+            unwrap(out, dst);
+            return 0;
+        });
+    });
+}
+
+int ipfr_hough_circles(const void* src, int w, int h, int cn, int dtype, int min_r, int max_r, double canny_high,
+                       int center_votes, double min_dist, double min_coverage, double* out, int capacity, int* count) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const auto img = wrap<T>(src, w, h, cn);
+            const auto cs = timed([&] {
+                return ipf::houghCircles(img, min_r, max_r, canny_high, center_votes, min_dist, min_coverage);
+            });
+            *count = static_cast<int>(cs.size());
+            for (int i = 0; i < static_cast<int>(cs.size()) && i < capacity; ++i) {
+                out[4 * i] = cs[i].center.x;
+                out[4 * i + 1] = cs[i].center.y;
+                out[4 * i + 2] = cs[i].radius;
+                out[4 * i + 3] = cs[i].votes;
+            }
+            return 0;
         });
     });
 }

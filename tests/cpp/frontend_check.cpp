@@ -60,6 +60,10 @@ int errors() {
     bad += expect_error("prewitt", [&] { prewitt(img, 0, 0); }, "ipf: prewitt: (dx, dy) must be (1,0) or (0,1)");
     bad += expect_error("magnitude", [&] { magnitude(ImageF(3, 2), ImageF(2, 3)); }, "ipf: magnitude: shape mismatch");
     bad += expect_error("phase", [&] { phase(ImageF(3, 2), ImageF(3, 2, 2)); }, "ipf: phase: shape mismatch");
+    bad += expect_error("filter2DFFT kernel", [&] { filter2DFFT(img, ImageF()); },
+                        "ipf: filter2DFFT: kernel must be single-channel");
+    bad += expect_error("filter2DFFT empty", [&] { filter2DFFT(ImageU8(0, 3, 1), ImageF(3, 3, 1, 1.f)); },
+                        "ipf: copyMakeBorder: empty source image");
     bad += expect_error("canny thresholds", [&] { canny(img, 9.0, 3.0); },
                         "ipf: canny: lowThreshold must be <= highThreshold");
     bad += expect_error("canny ksize", [&] { canny(img, 1.0, 3.0, 2); }, "ipf: derivKernel: ksize must be odd in [3, 31]");
@@ -159,6 +163,17 @@ int run(const std::string& dir) {
     save(dir + "/o26.bin", cannyAuto(img));
     save(dir + "/o27.bin", cornerHarris(gray, 3, 3, 0.04));
     save(dir + "/o28.bin", cornerMinEigenVal(imgf, 5, 5));
+    save(dir + "/o29.bin", filter2DFFT(img, makeKernel(3, 3, {0, -1, 0, -1, 5, -1, 0, -1, 0}), BorderMode::Replicate, 1.0));
+    save(dir + "/o31.bin", integralImage(img));
+    save(dir + "/o32.bin", integralImage(imgf, true));
+    {
+        const ImageD t = integralImage(img);
+        double direct = 0;
+        for (int y = 3; y < 3 + 20; ++y)
+            for (int x = 5; x < 5 + 30; ++x) direct += img(x, y, 1);
+        if (integralSum(t, Rect(5, 3, 30, 20), 1) != direct) return 1;
+    }
+    save(dir + "/o30.bin", filter2DFFT<double>(imgf, gaussianKernel2D(2.0), BorderMode::Constant, 0.5, 0.25));
     std::printf("RUN OK\n");
     return 0;
 }

@@ -37,6 +37,8 @@ def run_oracle(o, case):
         return getattr(o, c["op"])(inp)
     if c["op"] == "ssim":
         return o.ssim(inp, ex["b"])[1]
+    if c["op"] == "integral":
+        return o.integral_image(inp, bool(p["squared"]))
     if c["op"] in FEATURE_OPS:
         return _run_feature(o, case)
     out_dt = np.dtype(p["out"]) if "out" in p else None
@@ -50,6 +52,8 @@ def run_oracle(o, case):
         return o.box_filter(inp, p["kw"], p["kh"], p["normalize"], b, p["bval"], out_dt)
     if c["op"] == "filter2d":
         return o.filter2d(inp, ex["k"], b, p.get("delta", 0.0), tuple(p.get("anchor", (-1, -1))), p["bval"], out_dt)
+    if c["op"] == "filter2d_fft":
+        return o.filter2d_fft(inp, ex["k"], b, p["delta"], p["bval"], out_dt)
     if c["op"] == "median":
         return o.median(inp, p["k"], b)
     if c["op"] == "bilateral":
@@ -113,6 +117,8 @@ def run_gpu(ipf, case, precision=0):
         return getattr(ipf, c["op"])(inp)
     if c["op"] == "ssim":
         return ipf.ssim(inp, ex["b"], return_map=True)[1]
+    if c["op"] == "integral":
+        return ipf.integral_image(inp, bool(p["squared"]))
     if c["op"] in FEATURE_OPS:
         return _run_feature(ipf, case, gpu=True)
     out_dt = np.dtype(p["out"]) if "out" in p else None
@@ -127,6 +133,8 @@ def run_gpu(ipf, case, precision=0):
     if c["op"] == "filter2d":
         return ipf.filter2d(inp, ex["k"], b, p.get("delta", 0.0), tuple(p.get("anchor", (-1, -1))), p["bval"],
                             out_dt, precision)
+    if c["op"] == "filter2d_fft":
+        return ipf.filter2d_fft(inp, ex["k"], b, p["delta"], p["bval"], out_dt)
     if c["op"] == "median":
         return ipf.median_blur(inp, p["k"], b)
     if c["op"] == "bilateral":

@@ -205,6 +205,34 @@ def main():
                 add("min_eigen", {"block": blk, "ksize": k, "src": name}, img, ref.corner_min_eigen(img, blk, k))
         add("harris", {"block": 3, "ksize": 3, "k": -0.15, "src": name}, img, ref.corner_harris(img, 3, 3, -0.15))
 
+    # filter2DFFT (fft.hpp:196-216): FP64 radix-2 FFTs of pow2-padded buffers, anchor at the centre
+    kf = wl.normal(seed, 60, 35).reshape(5, 7).astype(np.float32)
+    kf31 = wl.normal(seed, 61, 961).reshape(31, 31).astype(np.float32)
+    kf31 /= np.float32(kf31.sum())
+    ke = wl.normal(seed, 62, 24).reshape(4, 6).astype(np.float32)  # even sides: anchor kw / 2, kh / 2
+    k1 = np.array([[1.75]], np.float32)
+    for b in BORDERS:
+        add("filter2d_fft", {"border": b, "bval": 7.0, "delta": 0.0}, f32_1,
+            ref.filter2d_fft(f32_1, kf, b, 0.0, 7.0), k=kf)
+        add("filter2d_fft", {"border": b, "bval": 2.0, "delta": 1.5}, u8_3,
+            ref.filter2d_fft(u8_3, kf, b, 1.5, 2.0), k=kf)
+        add("filter2d_fft", {"border": b, "bval": 0.0, "delta": 0.0}, u8_1, ref.filter2d_fft(u8_1, kf31, b), k=kf31)
+        add("filter2d_fft", {"border": b, "bval": -3.0, "delta": 0.25, "out": "float32"}, i16_1,
+            ref.filter2d_fft(i16_1, ke, b, 0.25, -3.0, np.float32), k=ke)
+        add("filter2d_fft", {"border": b, "bval": 0.0, "delta": 0.0}, tiny, ref.filter2d_fft(tiny, kf, b), k=kf)
+        add("filter2d_fft", {"border": b, "bval": 0.0, "delta": 0.0}, poly, ref.filter2d_fft(poly, k1, b), k=k1)
+        add("filter2d_fft", {"border": b, "bval": 0.0, "delta": 0.0, "out": "float64"}, f32_3,
+            ref.filter2d_fft(f32_3, kf31, b, 0.0, 0.0, np.float64), k=kf31)
+    px = wl.uniform_u8((1, 1), seed=seed, stream=15)
+    add("filter2d_fft", {"border": 3, "bval": 0.0, "delta": 0.0}, px, ref.filter2d_fft(px, kf, 3), k=kf)
+
+    # integralImage (filter.hpp:522-538): running FP64 sums; float images keep their order
+    i32_1 = (wl.uniform_f32((15, 21), seed=seed, stream=16) * 4e9 - 2e9).astype(np.int32)
+    for img, name in ((u8_3, "u8x3"), (f32_1, "f32x1"), (f32_3, "f32x3"), (i16_1, "i16x1"), (i32_1, "i32x1"),
+                      (tiny, "tiny"), (px, "px"), (poly, "poly")):
+        for sq in (0, 1):
+            add("integral", {"squared": sq, "src": name}, img, ref.integral_image(img, sq))
+
     np.savez_compressed(OUT / "fixtures.npz", **arrays)
     (OUT / "manifest.json").write_text(json.dumps(cases, indent=0))
 
@@ -249,6 +277,10 @@ def main():
         full[f"c2_box15_{NAMES[b]}_f32"] = sha(ref.box_filter(c2, 15, 15, True, b))
     full["c3_pyrdown_u8"] = sha(ref.pyr_down(c3))
     full["c3_pyrup_u8"] = sha(ref.pyr_up(c3))
+    full["c4_filter2dfft31_reflect101_u8"] = sha(ref.filter2d_fft(u8, k31, 3))
+    full["c4_filter2dfft31_reflect101_f32"] = sha(ref.filter2d_fft(f32, k31, 3))
+    full["c2_integral_f32"] = sha(ref.integral_image(c2))
+    full["c2_integral_sq_f32"] = sha(ref.integral_image(c2, True))
     full["c3_canny_50_150_u8"] = sha(ref.canny(c3, 50.0, 150.0))
     full["c3_canny_auto_u8"] = sha(ref.canny_auto(c3))
     c3g = np.ascontiguousarray(c3[..., 0])

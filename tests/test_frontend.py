@@ -79,6 +79,11 @@ def test_frontend_results_match_oracle(exe, oracle, tmp_path):
     assert np.array_equal(out(26, np.uint8, u8.shape[:2]), oracle.canny_auto(u8))
     assert np.array_equal(out(27, np.float32, gray.shape), oracle.corner_harris(gray, 3, 3, 0.04))
     assert np.array_equal(out(28, np.float32, f.shape), oracle.corner_min_eigen(f, 5, 5))
+    assert np.array_equal(out(29, np.uint8, u8.shape), oracle.filter2d_fft(u8, k, 1, 1.0))
+    k2 = np.outer(oracle.gaussian_kernel_1d(2.0), oracle.gaussian_kernel_1d(2.0)).astype(np.float32)
+    assert np.array_equal(out(30, np.float64, f.shape), oracle.filter2d_fft(f, k2, 0, 0.5, 0.25, np.float64))
+    assert np.array_equal(out(31, np.float64, (62, 98, 3)), oracle.integral_image(u8))
+    assert np.array_equal(out(32, np.float64, (60, 84)), oracle.integral_image(f, True))
     level = u8
     for i in range(7):
         assert np.array_equal(np.fromfile(tmp_path / f"p{i}.bin", np.uint8).reshape(level.shape), level), i

@@ -910,3 +910,70 @@ def test_corner_responses_vs_oracle(ipf, oracle, dtype):
         for k in (1, 3, 5, 7):
             assert np.array_equal(ipf.corner_harris(img, blk, k, 0.04), oracle.corner_harris(img, blk, k, 0.04)), (blk, k)
             assert np.array_equal(ipf.corner_min_eigen_val(img, blk, k), oracle.corner_min_eigen(img, blk, k)), (blk, k)
+
+
+# --------------------------------------------------------------- filter2DFFT
+def test_full_c4_filter2d_fft_vs_reference(ipf):
+    # 4096 x 4096 uint8 and float32 with the 31 x 31 kernel: 8192 x 8192-point
+    # FP64 transforms; SHA-256 of the reference's own outputs
+    from imageprocessingflex_amd import workloads as wl
+    u8, f32 = wl.c4_images()
+    k, _ = wl.c4_kernel()
+    assert _sha(ipf.filter2d_fft(ipf.DeviceImage.from_array(u8), k).download()) == FULL["c4_filter2dfft31_reflect101_u8"]
+    assert _sha(ipf.filter2d_fft(ipf.DeviceImage.from_array(f32), k).download()) == FULL["c4_filter2dfft31_reflect101_f32"]
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 300), (300, 1), (2, 2, 3), (37, 23, 4), (129, 257, 3), (600, 70),
+                                   (70, 1100)], ids=lambda s: "x".join(map(str, s)))
+def test_filter2d_fft_vs_oracle(ipf, oracle, shape):
+    # one- and two-pass axes (up to 2^11 points), rows shorter than a tile,
+    # every border, odd and even kernel sides, every output type
+    from imageprocessingflex_amd import workloads as wl
+    rng = np.random.default_rng(sum(shape))
+    u8 = wl.uniform_u8(shape, seed=12)
+    f = wl.uniform_f32(shape, seed=13) * 8 - 4
+    for ks in ((3, 3), (1, 7), (6, 4), (17, 15)):
+        k = rng.standard_normal(ks).astype(np.float32)
+        for b in range(5):
+            for img, to in ((u8, None), (u8, np.float32), (f, None), (f, np.float64), (u8, np.int16)):
+                got = ipf.filter2d_fft(img, k, b, 0.5, 3.0, to)
+                want = oracle.filter2d_fft(img, k, b, 0.5, 3.0, to)
+                assert np.array_equal(got, want), (ks, b, img.dtype, to)
+
+
+def test_filter2d_fft_device_strided_and_repeat(ipf, oracle):
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.uniform_u8((260, 300, 3), seed=14)
+    view = img[7:250, 11:290]
+    k = np.arange(1, 10, dtype=np.float32).reshape(3, 3) / 45
+    want = oracle.filter2d_fft(np.ascontiguousarray(view), k, 4)
+    assert np.array_equal(ipf.filter2d_fft(view, k, 4), want)
+    d = ipf.DeviceImage.from_array(np.ascontiguousarray(view))
+    s = ipf.Stream()
+    outs = [ipf.filter2d_fft(d, k, 4, stream=s) for _ in range(3)]
+    s.synchronize()
+    for o in outs:
+        assert np.array_equal(o.download(), want)
+
+
+# ------------------------------------------------------------ integralImage
+def test_full_c2_integral_vs_reference(ipf):
+    # 7680 x 4320 float32: both tables (sums and squared sums) bit for bit
+    from imageprocessingflex_amd import workloads as wl
+    d = ipf.DeviceImage.from_array(wl.sinusoid_image())
+    assert _sha(ipf.integral_image(d).download()) == FULL["c2_integral_f32"]
+    assert _sha(ipf.integral_image(d, True).download()) == FULL["c2_integral_sq_f32"]
+
+
+def test_integral_image_types_and_sums(ipf, oracle):
+    from imageprocessingflex_amd import workloads as wl
+    for shape in ((1, 1), (3, 700, 4), (513, 2), (129, 257, 3)):
+        for img in (wl.uniform_u8(shape, seed=21), wl.uniform_f32(shape, seed=22) * 2 - 1,
+                    (wl.uniform_f32(shape, seed=23) * 4e9 - 2e9).astype(np.int32)):
+            for sq in (False, True):
+                got = ipf.integral_image(img, sq)
+                assert np.array_equal(got, oracle.integral_image(img, sq)), (shape, img.dtype, sq)
+    img = wl.uniform_u8((200, 300, 3), seed=24)
+    t = ipf.integral_image(img[5:150, 7:260])  # a strided view
+    assert np.array_equal(t, oracle.integral_image(np.ascontiguousarray(img[5:150, 7:260])))
+    assert ipf.integral_sum(t, 10, 20, 30, 40, 2) == img[5:150, 7:260][20:60, 10:40, 2].astype(np.int64).sum()

@@ -627,3 +627,49 @@ def test_feature_fixtures_catch_planted_mistakes(oracle):
     assert differs("min_eigen", lambda p: p["src"] == "poly1",
                    lambda p, img: (oracle.corner_min_eigen(img, p["block"], p["ksize"]) * 2).astype(np.float32))
     assert differs("canny_auto", lambda p: p["src"] == "poly", lambda p, img: oracle.canny_auto(img, p["sigma"] * 1.1))
+
+
+# ------------------------------------------------------------- filter2DFFT
+def test_filter2d_fft_equals_spatial_filter2d(oracle):
+    # test_fft.cpp:46-56: filter2DFFT agrees with filter2D (every border mode,
+    # delta and a border value) to FFT rounding; uint8 outputs within one unit.
+    # The bound here is the radix-2 FFT's: a few ulps of log2(n) times the
+    # largest sum of |k x| involved.
+    from imageprocessingflex_amd import workloads as wl
+    img = wl.uniform_f32((41, 53, 2), seed=9)
+    g = oracle.gaussian_kernel_1d(3.0)
+    k = np.outer(g, g).astype(np.float32)
+    for m in range(5):
+        a = oracle.filter2d(img, k, m, 0.5, (-1, -1), 0.25, np.float64)
+        b = oracle.filter2d_fft(img, k, m, 0.5, 0.25, np.float64)
+        assert np.abs(a - b).max() < 1e-12, m
+    u = wl.uniform_u8((30, 40, 3), seed=1)
+    assert np.abs(oracle.filter2d(u, k).astype(int) - oracle.filter2d_fft(u, k).astype(int)).max() <= 1
+    # a signed 31 x 31 kernel on the full uint8 range
+    k31 = wl.normal(5, 0, 961).reshape(31, 31).astype(np.float32)
+    u1 = wl.uniform_u8((70, 90), seed=4)
+    a = oracle.filter2d(u1, k31, 3, 0.0, (-1, -1), 0.0, np.float64)
+    b = oracle.filter2d_fft(u1, k31, 3, 0.0, 0.0, np.float64)
+    assert np.abs(a - b).max() < 1e-9 * np.abs(k31).sum() * 255
+
+
+def test_filter2d_fft_impulse_and_planted_mistakes(oracle):
+    # an impulse under a Constant-0 border returns the kernel flipped about
+    # its anchor (correlation), to FFT rounding
+    k = np.arange(1, 16, dtype=np.float32).reshape(3, 5) / 8
+    img = np.zeros((9, 11), np.float64)
+    img[4, 5] = 1.0
+    out = oracle.filter2d_fft(img, k, 0)
+    want = np.zeros_like(img)
+    want[3:6, 3:8] = k[::-1, ::-1]
+    assert np.abs(out - want).max() < 1e-14
+    # a wrong border, a transposed kernel and a dropped delta change fixtures
+    c, inp, want, ex = next(x for x in fixtures.load_cases() if x[0]["op"] == "filter2d_fft"
+                            and x[0]["params"]["border"] == 3 and x[1].dtype == np.uint8
+                            and x[3]["k"].shape == (5, 7))
+    p = c["params"]
+    assert np.array_equal(oracle.filter2d_fft(inp, ex["k"], 3, p["delta"], p["bval"]), want)
+    assert not np.array_equal(oracle.filter2d_fft(inp, ex["k"], 2, p["delta"], p["bval"]), want)
+    assert not np.array_equal(oracle.filter2d_fft(inp, np.ascontiguousarray(ex["k"][::-1, ::-1]), 3, p["delta"],
+                                                  p["bval"]), want)
+    assert not np.array_equal(oracle.filter2d_fft(inp, ex["k"], 3, 0.0, p["bval"]), want)
