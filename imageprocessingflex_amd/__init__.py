@@ -22,7 +22,7 @@ __all__ = [
     "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
     "sobel", "scharr", "prewitt", "laplacian", "magnitude", "phase", "gradient_magnitude", "unsharp_mask",
     "sharpen", "emboss", "adaptive_threshold", "motion_blur_kernel", "AdaptiveMethod", "ThresholdType",
-    "pyr_down", "pyr_up", "build_pyramid", "ssim", "filter2d_fft", "integral_image", "integral_sum", "canny", "canny_auto", "corner_harris",
+    "pyr_down", "pyr_up", "build_pyramid", "ssim", "filter2d_fft", "integral_image", "integral_sum", "hough_circles", "canny", "canny_auto", "corner_harris",
     "corner_min_eigen_val",
 ]
 
@@ -292,6 +292,31 @@ def filter2d(src, kernel, border=BorderMode.REFLECT101, delta=0.0, anchor=(-1, -
                                          int(anchor[0]), int(anchor[1]), int(border), float(delta),
                                          float(border_value), int(precision), _sh(stream)))
     return _ret(holder)
+
+
+def hough_circles(src, min_radius, max_radius, canny_high=100.0, center_votes=20, min_dist=10.0, min_coverage=0.6,
+                  stream: Stream | None = None) -> np.ndarray:
+    """houghCircles (features.hpp:388-504): (n, 4) float64 rows (cx, cy, radius, votes),
+    strongest first."""
+    if isinstance(src, DeviceImage):
+        s, hold = src.descriptor(), src
+    else:
+        a = np.asarray(src)
+        if not _rows_contiguous(a):
+            a = np.ascontiguousarray(a)
+        s, hold = _desc(a), a
+    cap = 256
+    while True:
+        out = np.zeros((cap, 4), np.float64)
+        n = C.c_int(0)
+        _lib.check(_lib.load().ipfg_hough_circles(C.byref(s), int(min_radius), int(max_radius), float(canny_high),
+                                                  int(center_votes), float(min_dist), float(min_coverage),
+                                                  out.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n),
+                                                  _sh(stream)))
+        if n.value <= cap:
+            del hold
+            return out[:n.value].copy()
+        cap = n.value
 
 
 def integral_image(src, squared=False, stream: Stream | None = None):

@@ -99,6 +99,7 @@ SIGNATURES = {
     "ipfg_pyr_down": (_I, [_IMG, _IMG, _V]),
     "ipfg_pyr_up": (_I, [_IMG, _IMG, _V]),
     "ipfg_ssim": (_I, [_IMG, _IMG, _IMG, _P(_D), _V]),
+    "ipfg_hough_circles": (_I, [_IMG, _I, _I, _D, _I, _D, _D, _P(_D), _I, _P(_I), _V]),
     "ipfg_integral_image": (_I, [_IMG, _IMG, _I, _V]),
     "ipfg_filter2d_fft": (_I, [_IMG, _IMG, _P(C.c_float), _I, _I, _I, _D, _D, _V]),
     "ipfg_canny": (_I, [_IMG, _IMG, _D, _D, _I, _I, _V]),

@@ -22,6 +22,7 @@
 #include "arena.hpp"
 #include "batch.hpp"
 #include "fft.hpp"
+#include "hough.hpp"
 #include "kernels.hpp"
 #include "sep_kernels.hpp"
 #include "tuner.hpp"
@@ -1838,6 +1839,124 @@ void corner_impl(const ipfg_image* src, const ipfg_image* dst, int block, int ks
     io.finish();
 }
 
+// ------------------------------------------------------------------ houghCircles
+// A device copy of a host image (or the device image itself).
+DevImage device_source(const ipfg_image* im, CallScratch& sc) {
+    if (im->on_device) return to_dev(*im);
+    const size_t row = packed_stride(*im);
+    DevImage d = scratch_image(sc, im->width, im->height, im->channels, im->dtype);
+    HIPCALL(hipMemcpy2DAsync(d.data, d.stride, im->data, im->row_stride_bytes ? im->row_stride_bytes : row, row,
+                             im->height, hipMemcpyHostToDevice, sc.stream()));
+    return d;
+}
+
+// houghCircles (features.hpp:388-504): the filters, votes and per-peak
+// refinement on the device; peak ordering, the minimum-distance pass and the
+// final de-duplication on the host, in the reference's order. Writes up to
+// `capacity` (cx, cy, radius, votes) quadruples; *count receives the number
+// of circles. Blocks until the result is on the host.
+void hough_impl(const ipfg_image* src, int min_r, int max_r, double canny_high, int center_votes, double min_dist,
+                double min_coverage, double* out, int capacity, int* count, ipfg_stream stream) {
+    check_image(src, "null src image");
+    abi_check(count != nullptr && (capacity <= 0 || out != nullptr), "houghCircles: null result");
+    require(src->channels == 1, "houghCircles: expected a single-channel image");
+    require(min_r > 0 && max_r >= min_r, "houghCircles: invalid radius range");
+    require(canny_high / 2 <= canny_high, "canny: lowThreshold must be <= highThreshold");
+    if (max_r > kHoughMaxRadius) throw Unsupported{"ipfg: houghCircles: maxRadius above 15000 is not supported"};
+    *count = 0;
+    ensure_init();
+    const int W = src->width, H = src->height;
+    const size_t n = static_cast<size_t>(W) * H;
+    if (n == 0) return;
+    const hipStream_t s = pick_stream(stream);
+    const ipfg_stream st = reinterpret_cast<ipfg_stream>(s);
+    CallScratch sc(s);
+    const DevImage gray = device_source(src, sc);
+    const DevImage edges = scratch_image(sc, W, H, 1, IPFG_U8);
+    canny_device(gray, edges, canny_high / 2, canny_high, 3, false, nullptr, sc);
+    const std::vector<double> k = gaussian_kernel(1.0, 0);
+    const int ks = static_cast<int>(k.size());
+    const DevImage smooth = scratch_image(sc, W, H, 1, IPFG_F32);
+    const DevImage dx = scratch_image(sc, W, H, 1, IPFG_F32), dy = scratch_image(sc, W, H, 1, IPFG_F32);
+    const ipfg_image dg = dev_view(gray), dsm = dev_view(smooth), ddx = dev_view(dx), ddy = dev_view(dy);
+    sep_impl(&dg, &dsm, k.data(), ks, k.data(), ks, ks / 2, ks / 2, IPFG_BORDER_REFLECT101, 0.0, 0.0,
+             IPFG_PREC_DEFAULT, st);
+    sobel_impl(&dsm, &ddx, 1, 0, 3, 1.0, IPFG_BORDER_REFLECT101, IPFG_PREC_DEFAULT, st);
+    sobel_impl(&dsm, &ddy, 0, 1, 3, 1.0, IPFG_BORDER_REFLECT101, IPFG_PREC_DEFAULT, st);
+    const size_t chunks = hough_chunks(W, H);
+    HoughScratch hs;
+    hs.pts = static_cast<HoughPoint*>(sc.alloc(n * sizeof(HoughPoint)));
+    hs.grad = static_cast<double*>(sc.alloc(n * 3 * sizeof(double)));
+    hs.counts = static_cast<unsigned*>(sc.alloc(2 * chunks * sizeof(unsigned)));
+    hs.total = static_cast<unsigned*>(sc.alloc(2 * sizeof(unsigned)));
+    hs.acc = static_cast<float*>(sc.alloc(n * sizeof(float)));
+    hs.sm = static_cast<float*>(sc.alloc(n * sizeof(float)));
+    hs.cand_xy = static_cast<int*>(sc.alloc(n * 2 * sizeof(int)));
+    hs.cand_v = static_cast<float*>(sc.alloc(n * sizeof(float)));
+    HIPCALL(launch_hough_votes(edges, dx, dy, min_r, max_r, static_cast<float>(center_votes) - 0.5f, hs, s));
+    unsigned totals[2];
+    HIPCALL(hipMemcpyAsync(totals, hs.total, sizeof(totals), hipMemcpyDeviceToHost, s));
+    HIPCALL(hipStreamSynchronize(s));
+    const unsigned npts = totals[0], ncand = totals[1];
+    std::vector<int> cxy(2 * static_cast<size_t>(ncand));
+    std::vector<float> cv(ncand);
+    if (ncand) {
+        HIPCALL(hipMemcpyAsync(cxy.data(), hs.cand_xy, cxy.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCALL(hipMemcpyAsync(cv.data(), hs.cand_v, cv.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCALL(hipStreamSynchronize(s));
+    }
+    // findPeaks (features.hpp:174-187): strongest first (ties in raster
+    // order), at least minDist apart
+    std::vector<unsigned> order(ncand);
+    for (unsigned i = 0; i < ncand; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned b) { return cv[a] > cv[b]; });
+    const double md2 = min_dist * min_dist;
+    std::vector<int> peaks;
+    for (const unsigned c : order) {
+        bool ok = true;
+        for (size_t q = 0; q < peaks.size(); q += 2) {
+            const double ddx = cxy[2 * c] - peaks[q], ddy = cxy[2 * c + 1] - peaks[q + 1];
+            if (ddx * ddx + ddy * ddy < md2) {
+                ok = false;
+                break;
+            }
+        }
+        if (ok) {
+            peaks.push_back(cxy[2 * c]);
+            peaks.push_back(cxy[2 * c + 1]);
+        }
+    }
+    const int np = static_cast<int>(peaks.size() / 2);
+    std::vector<HoughFit> fits(static_cast<size_t>(np));
+    if (np) {
+        int* dpeaks = static_cast<int*>(sc.alloc(peaks.size() * sizeof(int)));
+        HoughFit* dfits = static_cast<HoughFit*>(sc.alloc(fits.size() * sizeof(HoughFit)));
+        HIPCALL(hipMemcpyAsync(dpeaks, peaks.data(), peaks.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCALL(launch_hough_refine(hs.pts, npts, hs.acc, W, H, dpeaks, np, min_r, max_r, min_coverage, dfits, s));
+        HIPCALL(hipMemcpyAsync(fits.data(), dfits, fits.size() * sizeof(HoughFit), hipMemcpyDeviceToHost, s));
+        HIPCALL(hipStreamSynchronize(s));
+    }
+    std::vector<HoughFit> circles;
+    for (const HoughFit& f : fits)
+        if (f.accepted) circles.push_back(f);
+    std::stable_sort(circles.begin(), circles.end(),
+                     [](const HoughFit& a, const HoughFit& b) { return a.support > b.support; });
+    std::vector<HoughFit> unique;
+    for (const HoughFit& c : circles) {
+        bool dup = false;
+        for (const HoughFit& u : unique) dup |= std::hypot(c.cx - u.cx, c.cy - u.cy) < min_dist;
+        if (!dup) unique.push_back(c);
+    }
+    *count = static_cast<int>(unique.size());
+    for (int i = 0; i < static_cast<int>(unique.size()) && i < capacity; ++i) {
+        out[4 * i] = unique[i].cx;
+        out[4 * i + 1] = unique[i].cy;
+        out[4 * i + 2] = unique[i].r;
+        out[4 * i + 3] = unique[i].support;
+    }
+    t_ctx.fix_valid = false;
+}
+
 // ------------------------------------------------------------------ integralImage
 // integralImage (filter.hpp:522-538): dst is a float64 (W + 1) x (H + 1) image
 // with src's channels.
@@ -2776,6 +2895,15 @@ int ipfg_emboss(const ipfg_image* src, const ipfg_image* dst, int precision, ipf
 int ipfg_adaptive_threshold(const ipfg_image* src, const ipfg_image* dst, double maxval, int method, int type,
                             int block_size, double c, int precision, ipfg_stream stream) {
     return guarded([&] { adaptive_impl(src, dst, maxval, method, type, block_size, c, precision, stream); });
+}
+
+int ipfg_hough_circles(const ipfg_image* src, int min_radius, int max_radius, double canny_high, int center_votes,
+                       double min_dist, double min_coverage, double* circles, int capacity, int* count,
+                       ipfg_stream stream) {
+    return guarded([&] {
+        hough_impl(src, min_radius, max_radius, canny_high, center_votes, min_dist, min_coverage, circles, capacity,
+                   count, stream);
+    });
 }
 
 int ipfg_integral_image(const ipfg_image* src, const ipfg_image* dst, int squared, ipfg_stream stream) {

@@ -166,3 +166,18 @@ def serving_stream(n: int = 20000, rate: float = 1000.0, seed: int = 6, min_side
 
 def request_image(req: Request, seed: int = 6) -> np.ndarray:
     return squeeze(uniform_u8((req.height, req.width, req.channels), seed=seed, stream=100 + req.rid))
+
+
+def circles_scene(h: int = 1080, w: int = 1920, n: int = 24, seed: int = 7, noise: float = 8.0) -> np.ndarray:
+    """Filled discs (radius 12 .. min(h, w) / 8 + 12, grey levels 90 .. 250) on a
+    40 background plus N(0, noise) noise, uint8: a houghCircles workload."""
+    u = uniform(seed, 0, 4 * n)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = np.full((h, w), 40.0)
+    for i in range(n):
+        rad = 12 + u[4 * i] * (min(h, w) / 8)
+        cx = rad + u[4 * i + 1] * (w - 2 * rad)
+        cy = rad + u[4 * i + 2] * (h - 2 * rad)
+        img[(xx - cx) ** 2 + (yy - cy) ** 2 <= rad ** 2] = 90 + 160 * u[4 * i + 3]
+    img += normal(seed, 1, h * w).reshape(h, w) * noise
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)

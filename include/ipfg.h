@@ -184,6 +184,21 @@ int ipfg_pyr_up(const ipfg_image* src, const ipfg_image* dst, ipfg_stream stream
 int ipfg_ssim(const ipfg_image* a, const ipfg_image* b, const ipfg_image* ssim_map, double* result,
               ipfg_stream stream);
 
+/* houghCircles(gray, minRadius, maxRadius, cannyHigh, centerVotes, minDist,
+ * minCoverage), features.hpp:388-504: one-channel src of any pixel type.
+ * Writes up to `capacity` circles as (cx, cy, radius, votes) quadruples of
+ * doubles into `circles`, strongest first, and their number into *count
+ * (call again with a larger capacity when *count exceeds it). canny, the
+ * smoothed gradient, the votes and every peak's refinement run on the
+ * device; the sums of the least-squares fits follow the reference's point
+ * order, so the circles are the reference's bit for bit unless a distance or
+ * an alignment falls exactly on one of the reference's thresholds (distances
+ * use a device hypot whose last bit can differ from glibc's). Blocks until
+ * the result is on the host. maxRadius above 15000: IPFG_ERR_UNSUPPORTED. */
+int ipfg_hough_circles(const ipfg_image* src, int min_radius, int max_radius, double canny_high, int center_votes,
+                       double min_dist, double min_coverage, double* circles, int capacity, int* count,
+                       ipfg_stream stream);
+
 /* ---------------------------------------------------------------- integral images
  * integralImage(src, squared), filter.hpp:522-538: dst is a float64
  * (W + 1) x (H + 1) image with src's channels, I(x, y) = sum of src (or of

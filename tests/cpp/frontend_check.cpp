@@ -64,6 +64,9 @@ int errors() {
                         "ipf: filter2DFFT: kernel must be single-channel");
     bad += expect_error("filter2DFFT empty", [&] { filter2DFFT(ImageU8(0, 3, 1), ImageF(3, 3, 1, 1.f)); },
                         "ipf: copyMakeBorder: empty source image");
+    bad += expect_error("hough channels", [&] { houghCircles(ImageU8(8, 8, 3), 2, 4); },
+                        "ipf: houghCircles: expected a single-channel image");
+    bad += expect_error("hough radii", [&] { houghCircles(img, 5, 4); }, "ipf: houghCircles: invalid radius range");
     bad += expect_error("canny thresholds", [&] { canny(img, 9.0, 3.0); },
                         "ipf: canny: lowThreshold must be <= highThreshold");
     bad += expect_error("canny ksize", [&] { canny(img, 1.0, 3.0, 2); }, "ipf: derivKernel: ksize must be odd in [3, 31]");
@@ -164,6 +167,22 @@ int run(const std::string& dir) {
     save(dir + "/o27.bin", cornerHarris(gray, 3, 3, 0.04));
     save(dir + "/o28.bin", cornerMinEigenVal(imgf, 5, 5));
     save(dir + "/o29.bin", filter2DFFT(img, makeKernel(3, 3, {0, -1, 0, -1, 5, -1, 0, -1, 0}), BorderMode::Replicate, 1.0));
+    {
+        // houghCircles on a drawn disc (radius 14 at (48, 30)) with a little noise
+        ImageU8 disc(97, 61, 1, 40);
+        for (int y = 0; y < 61; ++y)
+            for (int x = 0; x < 97; ++x) {
+                const int dx = x - 48, dy = y - 30;
+                disc(x, y) = static_cast<std::uint8_t>((dx * dx + dy * dy <= 14 * 14 ? 200 : 40) + (img(x, y, 0) & 7));
+            }
+        save(dir + "/in_disc.bin", disc);
+        const std::vector<HoughCircle> cs = houghCircles(disc, 8, 24);
+        std::vector<double> flat;
+        for (const HoughCircle& c : cs) flat.insert(flat.end(), {c.center.x, c.center.y, c.radius, double(c.votes)});
+        std::ofstream f(dir + "/o33.bin", std::ios::binary);
+        f.write(reinterpret_cast<const char*>(flat.data()), static_cast<std::streamsize>(flat.size() * sizeof(double)));
+        std::printf("CIRCLES %zu\n", cs.size());
+    }
     save(dir + "/o31.bin", integralImage(img));
     save(dir + "/o32.bin", integralImage(imgf, true));
     {

@@ -61,7 +61,7 @@ def run_oracle(o, case):
     return _run_composite(o, case)
 
 
-FEATURE_OPS = ("canny", "canny_auto", "harris", "min_eigen")
+FEATURE_OPS = ("canny", "canny_auto", "harris", "min_eigen", "hough")
 
 
 def _run_feature(m, case, gpu=False):
@@ -76,6 +76,8 @@ def _run_feature(m, case, gpu=False):
         return m.canny_auto(inp, p["sigma"])
     if c["op"] == "harris":
         return m.corner_harris(inp, p["block"], p["ksize"], p["k"])
+    if c["op"] == "hough":
+        return m.hough_circles(inp, p["rmin"], p["rmax"], p["high"], p["votes"], p["min_dist"], p["coverage"])
     fn = m.corner_min_eigen_val if gpu else m.corner_min_eigen
     return fn(inp, p["block"], p["ksize"])
 

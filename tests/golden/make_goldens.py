@@ -233,6 +233,20 @@ def main():
         for sq in (0, 1):
             add("integral", {"squared": sq, "src": name}, img, ref.integral_image(img, sq))
 
+    # houghCircles (features.hpp:388-504): disc scenes, noise, several parameter sets; outputs are
+    # (n, 4) rows (cx, cy, radius, votes)
+    sc1 = wl.circles_scene(120, 160, 3, seed=seed, noise=6.0)
+    sc2 = wl.circles_scene(150, 110, 5, seed=seed + 1, noise=12.0)
+    flat = np.full((40, 50), 77, np.uint8)
+    for img, name in ((sc1, "sc1"), (sc2, "sc2"), ((sc1.astype(np.float32) / 255).astype(np.float32), "sc1f"),
+                      ((sc2.astype(np.uint16) * 200).astype(np.uint16), "sc2u16"), (flat, "flat"), (px, "px")):
+        scale = {"sc1f": 1.0 / 255.0, "sc2u16": 200.0}.get(name, 1.0)
+        for (rmin, rmax, high, votes, md, cov) in ((8, 40, 100.0, 20, 10.0, 0.6), (5, 25, 60.0, 10, 6.0, 0.4),
+                                                   (12, 30, 150.0, 25, 15.0, 0.75)):
+            add("hough", {"rmin": rmin, "rmax": rmax, "high": high * scale, "votes": votes, "min_dist": md,
+                          "coverage": cov, "src": name}, img,
+                ref.hough_circles(img, rmin, rmax, high * scale, votes, md, cov))
+
     np.savez_compressed(OUT / "fixtures.npz", **arrays)
     (OUT / "manifest.json").write_text(json.dumps(cases, indent=0))
 
@@ -281,6 +295,8 @@ def main():
     full["c4_filter2dfft31_reflect101_f32"] = sha(ref.filter2d_fft(f32, k31, 3))
     full["c2_integral_f32"] = sha(ref.integral_image(c2))
     full["c2_integral_sq_f32"] = sha(ref.integral_image(c2, True))
+    scene = wl.circles_scene(1080, 1920, 24, 7, 3.0)
+    full["hough_1080p_circles"] = sha(ref.hough_circles(scene, 20, 120, 100.0, 30, 10.0, 0.6))
     full["c3_canny_50_150_u8"] = sha(ref.canny(c3, 50.0, 150.0))
     full["c3_canny_auto_u8"] = sha(ref.canny_auto(c3))
     c3g = np.ascontiguousarray(c3[..., 0])

@@ -83,6 +83,10 @@ def test_frontend_results_match_oracle(exe, oracle, tmp_path):
     k2 = np.outer(oracle.gaussian_kernel_1d(2.0), oracle.gaussian_kernel_1d(2.0)).astype(np.float32)
     assert np.array_equal(out(30, np.float64, f.shape), oracle.filter2d_fft(f, k2, 0, 0.5, 0.25, np.float64))
     assert np.array_equal(out(31, np.float64, (62, 98, 3)), oracle.integral_image(u8))
+    disc = np.fromfile(tmp_path / "in_disc.bin", np.uint8).reshape(61, 97)
+    want = oracle.hough_circles(disc, 8, 24)
+    assert len(want) >= 1 and abs(want[0, 0] - 48) < 1 and abs(want[0, 1] - 30) < 1 and abs(want[0, 2] - 14) < 1
+    assert np.array_equal(np.fromfile(tmp_path / "o33.bin", np.float64).reshape(-1, 4), want)
     assert np.array_equal(out(32, np.float64, (60, 84)), oracle.integral_image(f, True))
     level = u8
     for i in range(7):
