@@ -22,7 +22,8 @@ __all__ = [
     "median_blur", "bilateral_filter", "last_fixups", "synchronize", "init",
     "sobel", "scharr", "prewitt", "laplacian", "magnitude", "phase", "gradient_magnitude", "unsharp_mask",
     "sharpen", "emboss", "adaptive_threshold", "motion_blur_kernel", "AdaptiveMethod", "ThresholdType",
-    "pyr_down", "pyr_up", "build_pyramid", "ssim", "filter2d_fft", "integral_image", "integral_sum", "hough_circles", "canny", "canny_auto", "corner_harris",
+    "pyr_down", "pyr_up", "build_pyramid", "ssim", "filter2d_fft", "integral_image", "integral_sum", "hough_circles", "pnm_info", "decode_pnm", "encode_pnm",
+    "encode_pfm", "canny", "canny_auto", "corner_harris",
     "corner_min_eigen_val",
 ]
 
@@ -317,6 +318,65 @@ def hough_circles(src, min_radius, max_radius, canny_high=100.0, center_votes=20
             del hold
             return out[:n.value].copy()
         cap = n.value
+
+
+def pnm_info(data: bytes):
+    """Header of a Netpbm (P1-P6) or PFM file: (height, width, channels, is_pfm)."""
+    w, h, c, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    b = bytes(data)
+    _lib.check(_lib.load().ipfg_pnm_info(b, len(b), C.byref(w), C.byref(h), C.byref(c), C.byref(f)))
+    return h.value, w.value, c.value, bool(f.value)
+
+
+def decode_pnm(data: bytes, dtype=np.uint8, device: bool = False, stream: Stream | None = None):
+    """decodePNM<T> / decodePFM (codec/pnm.hpp:58-107, 143-176): the image of a
+    Netpbm or PFM file; binary rasters are unpacked on the GPU. dtype: uint8 or
+    uint16 (PNM; PFM files give float32). device=True returns a DeviceImage."""
+    b = bytes(data)
+    h, w, c, pfm = pnm_info(b)
+    dt = np.dtype(np.float32) if pfm else np.dtype(dtype)
+    if device:
+        out = DeviceImage(h, w, c, dt)
+        d, ret = out.descriptor(), out
+    else:
+        arr = np.empty((h, w) if c == 1 else (h, w, c), dt)
+        d, ret = _desc(arr), arr
+    _lib.check(_lib.load().ipfg_decode_pnm(b, len(b), C.byref(d), _sh(stream)))
+    return ret
+
+
+def encode_pnm(img: np.ndarray, binary: bool = True) -> bytes:
+    """encodePNM (codec/pnm.hpp:109-141): P5 / P6 (binary) or P2 / P3 from a uint8
+    or uint16 image (2 and 4 channels lose alpha). Host-side formatting."""
+    a = np.asarray(img.download() if isinstance(img, DeviceImage) else img)
+    if a.dtype not in (np.uint8, np.uint16):
+        raise IpfError("ipfg: encodePNM: image must be uint8 or uint16")
+    if a.size == 0:
+        raise IpfError("ipf: encodePNM: empty image")
+    if a.ndim == 3:
+        a = a[..., 0] if a.shape[2] <= 2 else a[..., :3]
+    a = np.ascontiguousarray(a)
+    one = a.ndim == 2
+    maxval = 255 if a.dtype == np.uint8 else 65535
+    magic = (5 if binary else 2) if one else (6 if binary else 3)
+    head = f"P{magic}\n{a.shape[1]} {a.shape[0]}\n{maxval}\n".encode()
+    if binary:
+        return head + (a.astype(">u2").tobytes() if a.dtype == np.uint16 else a.tobytes())
+    per = a.shape[1] * (1 if one else 3)
+    text = np.char.mod("%d", a.reshape(-1, per))
+    return head + "".join(" ".join(r) + "\n" for r in text).encode()
+
+
+def encode_pfm(img: np.ndarray) -> bytes:
+    """encodePFM (codec/pnm.hpp:178-197): little-endian PFM, rows bottom-up."""
+    a = np.asarray(img.download() if isinstance(img, DeviceImage) else img)
+    if not (a.ndim == 2 or (a.ndim == 3 and a.shape[2] in (1, 3))):
+        raise IpfError("ipf: encodePFM: image must have 1 or 3 channels")
+    if a.size == 0:
+        raise IpfError("ipf: encodePFM: empty image")
+    three = a.ndim == 3 and a.shape[2] == 3
+    head = f"{'PF' if three else 'Pf'}\n{a.shape[1]} {a.shape[0]}\n-1.0\n".encode()
+    return head + np.ascontiguousarray(a.astype("<f4")[::-1]).tobytes()
 
 
 def integral_image(src, squared=False, stream: Stream | None = None):

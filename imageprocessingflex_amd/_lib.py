@@ -100,6 +100,8 @@ SIGNATURES = {
     "ipfg_pyr_up": (_I, [_IMG, _IMG, _V]),
     "ipfg_ssim": (_I, [_IMG, _IMG, _IMG, _P(_D), _V]),
     "ipfg_hough_circles": (_I, [_IMG, _I, _I, _D, _I, _D, _D, _P(_D), _I, _P(_I), _V]),
+    "ipfg_pnm_info": (_I, [C.c_char_p, C.c_size_t, _P(_I), _P(_I), _P(_I), _P(_I)]),
+    "ipfg_decode_pnm": (_I, [C.c_char_p, C.c_size_t, _IMG, _V]),
     "ipfg_integral_image": (_I, [_IMG, _IMG, _I, _V]),
     "ipfg_filter2d_fft": (_I, [_IMG, _IMG, _P(C.c_float), _I, _I, _I, _D, _D, _V]),
     "ipfg_canny": (_I, [_IMG, _IMG, _D, _D, _I, _I, _V]),

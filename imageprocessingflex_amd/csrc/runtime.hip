@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cctype>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -23,6 +24,7 @@
 #include "batch.hpp"
 #include "fft.hpp"
 #include "hough.hpp"
+#include "pnm.hpp"
 #include "kernels.hpp"
 #include "sep_kernels.hpp"
 #include "tuner.hpp"
@@ -1957,6 +1959,166 @@ void hough_impl(const ipfg_image* src, int min_r, int max_r, double canny_high, 
     t_ctx.fix_valid = false;
 }
 
+// ------------------------------------------------------------------ Netpbm / PFM ingest
+// The header grammar of codec/pnm.hpp:16-55: tokens separated by whitespace,
+// '#' comments to the end of the line; numbers above 2^31 are rejected.
+class PnmHeader {
+public:
+    PnmHeader(const uint8_t* b, size_t n) : b_(b), n_(n) {
+        require(n >= 3 && b[0] == 'P' && ((b[1] >= '1' && b[1] <= '6') || b[1] == 'F' || b[1] == 'f'),
+                "PNM: not a P1-P6 netpbm file");
+        pfm = b[1] == 'F' || b[1] == 'f';
+        const char* fmt = pfm ? "PFM" : "PNM";
+        kind = pfm ? 0 : b[1] - '0';
+        pos_ = 2;
+        const long long ww = number(), hh = number();
+        if (ww <= 0 || hh <= 0) throw ArgError{std::string("ipf: ") + fmt + ": invalid image dimensions"};
+        if (ww > (1 << 24) || hh > (1 << 24) || ww * hh > (1LL << 30))
+            throw ArgError{std::string("ipf: ") + fmt + ": image dimensions too large"};
+        w = static_cast<int>(ww);
+        h = static_cast<int>(hh);
+        if (pfm) {
+            cn = b[1] == 'F' ? 3 : 1;
+            skip();
+            const size_t s0 = pos_;
+            while (pos_ < n_ && !std::isspace(b_[pos_])) ++pos_;
+            const std::string word(reinterpret_cast<const char*>(b_) + s0, pos_ - s0);
+            double scale = 0;
+            try {
+                scale = std::stod(word);
+            } catch (...) {
+                require(false, "PFM: invalid scale value");
+            }
+            require(scale != 0, "PFM: invalid scale value");
+            little = scale < 0;
+            require(pos_ < n_, "PFM: truncated");
+            data = pos_ + 1;  // one whitespace byte
+            return;
+        }
+        const bool bitmap = kind == 1 || kind == 4;
+        maxval = bitmap ? 1 : static_cast<int>(std::min<long long>(number(), 1LL << 31));
+        require(maxval >= 1 && maxval <= 65535, "PNM: invalid maxval");
+        cn = (kind == 3 || kind == 6) ? 3 : 1;
+        if (kind >= 4) {
+            require(pos_ < n_ && std::isspace(b_[pos_]), "PNM: malformed header");
+            data = pos_ + 1;
+        }
+    }
+    // ASCII rasters (P1-P3), rescaled to T
+    template <class T>
+    std::vector<T> ascii_samples() {
+        const size_t count = static_cast<size_t>(w) * h * cn;
+        std::vector<T> out(count);
+        const double tmax = static_cast<double>(std::numeric_limits<T>::max());
+        for (size_t i = 0; i < count; ++i) {
+            if (kind == 1) {
+                skip();
+                require(pos_ < n_ && (b_[pos_] == '0' || b_[pos_] == '1'), "PNM: malformed P1 pixel data");
+                out[i] = b_[pos_++] == '1' ? T(0) : static_cast<T>(tmax);
+            } else {
+                long long v = number();
+                if (v > maxval) v = maxval;
+                out[i] = static_cast<T>((static_cast<double>(v) * tmax) / static_cast<double>(maxval) + 0.5);
+            }
+        }
+        return out;
+    }
+    size_t raster_bytes() const {
+        if (pfm) return static_cast<size_t>(w) * h * cn * 4;
+        if (kind == 4) return (static_cast<size_t>(w) + 7) / 8 * h;
+        return static_cast<size_t>(w) * h * cn * (maxval < 256 ? 1 : 2);
+    }
+    int kind = 0, w = 0, h = 0, cn = 1, maxval = 1;
+    bool pfm = false, little = false;
+    size_t data = 0;
+
+private:
+    void skip() {
+        while (pos_ < n_) {
+            if (b_[pos_] == '#') {
+                while (pos_ < n_ && b_[pos_] != '\n' && b_[pos_] != '\r') ++pos_;
+            } else if (std::isspace(b_[pos_])) {
+                ++pos_;
+            } else {
+                break;
+            }
+        }
+    }
+    long long number() {
+        skip();
+        require(pos_ < n_ && std::isdigit(b_[pos_]), "PNM: malformed header or pixel data");
+        long long v = 0;
+        while (pos_ < n_ && std::isdigit(b_[pos_])) {
+            v = v * 10 + (b_[pos_++] - '0');
+            require(v <= (1LL << 31), "PNM: number too large");
+        }
+        return v;
+    }
+    const uint8_t* b_;
+    size_t n_;
+    size_t pos_ = 0;
+};
+
+void pnm_info_impl(const uint8_t* bytes, size_t n, int* w, int* h, int* cn, int* is_pfm) {
+    abi_check(bytes != nullptr || n == 0, "null bytes");
+    abi_check(w && h && cn && is_pfm, "null result");
+    const PnmHeader hd(bytes, n);
+    *w = hd.w;
+    *h = hd.h;
+    *cn = hd.cn;
+    *is_pfm = hd.pfm ? 1 : 0;
+}
+
+// decodePNM<T> / decodePFM (codec/pnm.hpp:58-107, 143-176) into dst.
+void pnm_decode_impl(const uint8_t* bytes, size_t n, const ipfg_image* dst, ipfg_stream stream) {
+    abi_check(bytes != nullptr || n == 0, "null bytes");
+    check_image(dst, "null dst image");
+    PnmHeader hd(bytes, n);
+    abi_check(dst->width == hd.w && dst->height == hd.h && dst->channels == hd.cn,
+              "decode: dst must have the file's width, height and channels");
+    abi_check(hd.pfm ? dst->dtype == IPFG_F32 : (dst->dtype == IPFG_U8 || dst->dtype == IPFG_U16),
+              hd.pfm ? "decodePFM: dst must be float32" : "decodePNM: dst must be uint8 or uint16");
+    std::vector<uint8_t> ascii;
+    if (!hd.pfm && hd.kind <= 3) {
+        if (dst->dtype == IPFG_U8) {
+            const std::vector<uint8_t> v = hd.ascii_samples<uint8_t>();
+            ascii.assign(v.begin(), v.end());
+        } else {
+            const std::vector<uint16_t> v = hd.ascii_samples<uint16_t>();
+            ascii.resize(v.size() * 2);
+            std::memcpy(ascii.data(), v.data(), ascii.size());
+        }
+    } else if (hd.raster_bytes() > n - std::min(n, hd.data)) {
+        throw ArgError{std::string("ipf: ") + (hd.pfm ? "PFM" : "PNM") + ": unexpected end of data (truncated file?)"};
+    }
+    ensure_init();
+    const hipStream_t s = pick_stream(stream);
+    CallScratch sc(s);
+    DevImage d;
+    const size_t row = packed_stride(*dst);
+    if (dst->on_device) {
+        d = to_dev(*dst);
+    } else {
+        d = scratch_image(sc, dst->width, dst->height, dst->channels, dst->dtype);
+    }
+    if (!ascii.empty()) {
+        HIPCALL(hipMemcpy2DAsync(d.data, d.stride, ascii.data(), row, row, d.h, hipMemcpyHostToDevice, s));
+        HIPCALL(hipStreamSynchronize(s));  // `ascii` is pageable and leaves scope
+    } else {
+        const int layout = hd.pfm ? (hd.little ? PFM_LE : PFM_BE)
+                                  : hd.kind == 4 ? PNM_BITS : (hd.maxval < 256 ? PNM_BYTES : PNM_WORDS_BE);
+        uint8_t* raw = static_cast<uint8_t*>(sc.alloc(hd.raster_bytes()));
+        HIPCALL(hipMemcpyAsync(raw, bytes + hd.data, hd.raster_bytes(), hipMemcpyHostToDevice, s));
+        HIPCALL(launch_pnm_unpack(raw, layout, hd.maxval, d, s));
+    }
+    if (!dst->on_device) {
+        HIPCALL(hipMemcpy2DAsync(dst->data, dst->row_stride_bytes ? dst->row_stride_bytes : row, d.data, d.stride,
+                                 row, d.h, hipMemcpyDeviceToHost, s));
+        HIPCALL(hipStreamSynchronize(s));
+    }
+    t_ctx.fix_valid = false;
+}
+
 // ------------------------------------------------------------------ integralImage
 // integralImage (filter.hpp:522-538): dst is a float64 (W + 1) x (H + 1) image
 // with src's channels.
@@ -2904,6 +3066,14 @@ int ipfg_hough_circles(const ipfg_image* src, int min_radius, int max_radius, do
         hough_impl(src, min_radius, max_radius, canny_high, center_votes, min_dist, min_coverage, circles, capacity,
                    count, stream);
     });
+}
+
+int ipfg_pnm_info(const uint8_t* bytes, size_t n, int* width, int* height, int* channels, int* is_pfm) {
+    return guarded([&] { pnm_info_impl(bytes, n, width, height, channels, is_pfm); });
+}
+
+int ipfg_decode_pnm(const uint8_t* bytes, size_t n, const ipfg_image* dst, ipfg_stream stream) {
+    return guarded([&] { pnm_decode_impl(bytes, n, dst, stream); });
 }
 
 int ipfg_integral_image(const ipfg_image* src, const ipfg_image* dst, int squared, ipfg_stream stream) {

@@ -199,6 +199,21 @@ int ipfg_hough_circles(const ipfg_image* src, int min_radius, int max_radius, do
                        double min_dist, double min_coverage, double* circles, int capacity, int* count,
                        ipfg_stream stream);
 
+/* ---------------------------------------------------------------- ingest
+ * Netpbm (P1-P6) and PFM decoding, codec/pnm.hpp:58-107 (decodePNM) and
+ * 143-176 (decodePFM), for request pipelines that decode straight into device
+ * memory. ipfg_pnm_info parses the header of the file in `bytes` (host
+ * memory) and returns its size, channel count (1 or 3) and whether it is a
+ * PFM. ipfg_decode_pnm writes the image into dst (device or host memory,
+ * uint8 or uint16 for P1-P6 with the reference's maxval rescaling, float32
+ * for PFM). Headers and ASCII rasters (P1-P3) are parsed on the host; binary
+ * rasters (P4-P6, PFM) are copied to the device as stored and unpacked
+ * there. Results are bit-identical to the reference's decoders; errors carry
+ * their messages. Blocks until `bytes` may be reused (and, for a host dst,
+ * until dst holds the image). */
+int ipfg_pnm_info(const uint8_t* bytes, size_t n, int* width, int* height, int* channels, int* is_pfm);
+int ipfg_decode_pnm(const uint8_t* bytes, size_t n, const ipfg_image* dst, ipfg_stream stream);
+
 /* ---------------------------------------------------------------- integral images
  * integralImage(src, squared), filter.hpp:522-538: dst is a float64
  * (W + 1) x (H + 1) image with src's channels, I(x, y) = sum of src (or of

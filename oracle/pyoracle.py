@@ -351,3 +351,233 @@ class Oracle:
         self._check(self._f("corner_min_eigen")(src.ctypes.data, w, h, cn, NP2DT[src.dtype], out.ctypes.data,
                                                 int(block), int(ksize)))
         return out
+
+
+# ------------------------------------------------------------------ Netpbm / PFM codecs
+# Test infrastructure: a numpy restatement of the reference's codec/pnm.hpp
+# (decodePNM 58-107, encodePNM 109-141, decodePFM 143-176, encodePFM 178-197),
+# pinned in tests/test_oracle.py against bytes and images the reference itself
+# produced (tests/golden), and the reference-built codecs (RefCodec) that
+# wrote them.
+class PnmError(ValueError):
+    pass
+
+
+def _pnm_fail(msg):
+    raise PnmError("ipf: " + msg)
+
+
+class _Tok:
+    """The Netpbm header tokenizer: '#' comments to the end of the line, any
+    whitespace between tokens (pnm.hpp:16-55)."""
+
+    def __init__(self, b: bytes, pos: int):
+        self.b, self.pos = b, pos
+
+    def skip(self):
+        b = self.b
+        while self.pos < len(b):
+            if b[self.pos] == 0x23:  # '#'
+                while self.pos < len(b) and b[self.pos] not in (0x0A, 0x0D):
+                    self.pos += 1
+            elif chr(b[self.pos]).isspace():
+                self.pos += 1
+            else:
+                break
+
+    def number(self):
+        self.skip()
+        b = self.b
+        if self.pos >= len(b) or not chr(b[self.pos]).isdigit():
+            _pnm_fail("PNM: malformed header or pixel data")
+        v = 0
+        while self.pos < len(b) and chr(b[self.pos]).isdigit():
+            v = v * 10 + b[self.pos] - 48
+            self.pos += 1
+            if v > (1 << 31):
+                _pnm_fail("PNM: number too large")
+        return v
+
+    def bit(self):
+        self.skip()
+        if self.pos >= len(self.b) or self.b[self.pos] not in (0x30, 0x31):
+            _pnm_fail("PNM: malformed P1 pixel data")
+        self.pos += 1
+        return self.b[self.pos - 1] - 48
+
+    def word(self):
+        self.skip()
+        s = self.pos
+        while self.pos < len(self.b) and not chr(self.b[self.pos]).isspace():
+            self.pos += 1
+        return self.b[s:self.pos].decode("latin-1")
+
+
+def _check_size(w, h, fmt):
+    if w <= 0 or h <= 0:
+        _pnm_fail(f"{fmt}: invalid image dimensions")
+    if w > (1 << 24) or h > (1 << 24) or w * h > (1 << 30):
+        _pnm_fail(f"{fmt}: image dimensions too large")
+
+
+def pnm_decode(b: bytes, dtype=np.uint8) -> np.ndarray:
+    """decodePNM<T>: (h, w) or (h, w, 3), samples rescaled from maxval to T's range."""
+    b = bytes(b)
+    if len(b) < 3 or b[0] != 0x50 or not (0x31 <= b[1] <= 0x36):
+        _pnm_fail("PNM: not a P1-P6 netpbm file")
+    kind = b[1] - 48
+    t = _Tok(b, 2)
+    w, h = t.number(), t.number()
+    _check_size(w, h, "PNM")
+    bitmap = kind in (1, 4)
+    maxval = 1 if bitmap else t.number()
+    if maxval < 1 or maxval > 65535:
+        _pnm_fail("PNM: invalid maxval")
+    cn = 3 if kind in (3, 6) else 1
+    n = w * h * cn
+    tmax = float(np.iinfo(dtype).max)
+
+    def scale(v):
+        v = np.minimum(np.asarray(v, np.int64), maxval).astype(np.float64)
+        return ((v * tmax) / float(maxval) + 0.5).astype(dtype)
+
+    if kind == 1:
+        out = np.array([0 if t.bit() else int(tmax) for _ in range(n)], dtype)
+    elif kind in (2, 3):
+        out = scale([t.number() for _ in range(n)])
+    else:
+        if t.pos >= len(b) or not chr(b[t.pos]).isspace():
+            _pnm_fail("PNM: malformed header")
+        pos = t.pos + 1
+
+        def take(k):
+            if k > len(b) - pos:
+                _pnm_fail("PNM: unexpected end of data (truncated file?)")
+            return np.frombuffer(b, np.uint8, k, pos)
+
+        if kind == 4:
+            rb = (w + 7) // 8
+            bits = np.unpackbits(take(rb * h).reshape(h, rb), axis=1)[:, :w]
+            out = np.where(bits.astype(bool), 0, int(tmax)).astype(dtype).ravel()
+        elif maxval < 256:
+            out = scale(take(n))
+        else:
+            r = take(2 * n).astype(np.int64)
+            out = scale((r[0::2] << 8) | r[1::2])
+    return out.reshape((h, w, 3) if cn == 3 else (h, w))
+
+
+def pfm_decode(b: bytes) -> np.ndarray:
+    """decodePFM: float32, rows stored bottom-up; byte order from the scale's sign."""
+    b = bytes(b)
+    if len(b) < 3 or b[0] != 0x50 or b[1] not in (0x46, 0x66):
+        _pnm_fail("PFM: not a PFM file")
+    cn = 3 if b[1] == 0x46 else 1
+    t = _Tok(b, 2)
+    w, h = t.number(), t.number()
+    _check_size(w, h, "PFM")
+    word = t.word()
+    try:
+        sc = float(word)
+    except ValueError:
+        _pnm_fail("PFM: invalid scale value")
+    if sc == 0:
+        _pnm_fail("PFM: invalid scale value")
+    pos = t.pos
+    if pos >= len(b):
+        _pnm_fail("PFM: truncated")
+    pos += 1
+    n = w * h * cn
+    if 4 * n > len(b) - pos:
+        _pnm_fail("PFM: unexpected end of data (truncated file?)")
+    a = np.frombuffer(b, "<f4" if sc < 0 else ">f4", n, pos).astype(np.float32)
+    a = a.reshape(h, w * cn)[::-1]
+    return np.ascontiguousarray(a.reshape((h, w, 3) if cn == 3 else (h, w)))
+
+
+def _to_1_or_3(img):
+    if img.ndim == 2:
+        return img
+    c = img.shape[2]
+    if c == 1 or c == 2:
+        return img[..., 0]
+    if c == 3:
+        return img
+    return img[..., :3]
+
+
+def pnm_encode(img: np.ndarray, binary: bool = True) -> bytes:
+    """encodePNM: P5/P6 (binary) or P2/P3; uint8 maxval 255, uint16 maxval 65535."""
+    if img.size == 0:
+        _pnm_fail("encodePNM: empty image")
+    a = np.ascontiguousarray(_to_1_or_3(img))
+    one = a.ndim == 2
+    maxval = 255 if a.dtype == np.uint8 else 65535
+    hdr = f"P{(5 if binary else 2) if one else (6 if binary else 3)}\n{a.shape[1]} {a.shape[0]}\n{maxval}\n"
+    if binary:
+        body = a.astype(">u2").tobytes() if a.dtype == np.uint16 else a.tobytes()
+        return hdr.encode() + body
+    per = a.shape[1] * (1 if one else 3)
+    flat = a.ravel()
+    parts = [str(int(v)) + ("\n" if (i + 1) % per == 0 else " ") for i, v in enumerate(flat)]
+    return hdr.encode() + "".join(parts).encode()
+
+
+def pfm_encode(img: np.ndarray) -> bytes:
+    """encodePFM: little-endian (scale -1.0), rows bottom-up, samples as float."""
+    if img.ndim == 3 and img.shape[2] not in (1, 3) or img.ndim not in (2, 3):
+        _pnm_fail("encodePFM: image must have 1 or 3 channels")
+    if img.size == 0:
+        _pnm_fail("encodePFM: empty image")
+    cn = 3 if img.ndim == 3 and img.shape[2] == 3 else 1
+    h, w = img.shape[:2]
+    hdr = f"{'PF' if cn == 3 else 'Pf'}\n{w} {h}\n-1.0\n"
+    return hdr.encode() + np.ascontiguousarray(img.astype("<f4")[::-1]).tobytes()
+
+
+class RefCodec:
+    """The reference's own Netpbm / PFM codecs (oracle/_ref)."""
+
+    def __init__(self):
+        if not REF_SO.exists():
+            raise OracleError(f"{REF_SO} not built (make -C oracle)")
+        self.lib = C.CDLL(str(REF_SO))
+        self.lib.ipfr_last_error.restype = C.c_char_p
+
+    def _err(self, rc):
+        if rc != 0:
+            raise PnmError(self.lib.ipfr_last_error().decode())
+
+    def decode_pnm(self, b: bytes, dtype=np.uint8):
+        w, h, cn = C.c_int(), C.c_int(), C.c_int()
+        self._err(self.lib.ipfr_decode_pnm(b, C.c_size_t(len(b)), int(dtype == np.uint16), None, C.c_size_t(0),
+                                           C.byref(w), C.byref(h), C.byref(cn)))
+        out = np.empty((h.value, w.value, cn.value), dtype)
+        self._err(self.lib.ipfr_decode_pnm(b, C.c_size_t(len(b)), int(dtype == np.uint16), out.ctypes.data_as(_V),
+                                           C.c_size_t(out.size), C.byref(w), C.byref(h), C.byref(cn)))
+        return out[..., 0] if cn.value == 1 else out
+
+    def decode_pfm(self, b: bytes):
+        w, h, cn = C.c_int(), C.c_int(), C.c_int()
+        self._err(self.lib.ipfr_decode_pfm(b, C.c_size_t(len(b)), None, C.c_size_t(0), C.byref(w), C.byref(h),
+                                           C.byref(cn)))
+        out = np.empty((h.value, w.value, cn.value), np.float32)
+        self._err(self.lib.ipfr_decode_pfm(b, C.c_size_t(len(b)), out.ctypes.data_as(_V), C.c_size_t(out.size),
+                                           C.byref(w), C.byref(h), C.byref(cn)))
+        return out[..., 0] if cn.value == 1 else out
+
+    def _encode(self, fn, img, *args):
+        a = np.ascontiguousarray(img)
+        h, w = a.shape[:2]
+        cn = 1 if a.ndim == 2 else a.shape[2]
+        need = C.c_size_t(0)
+        self._err(fn(a.ctypes.data_as(_V), w, h, cn, NP2DT[a.dtype], *args, None, C.c_size_t(0), C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        self._err(fn(a.ctypes.data_as(_V), w, h, cn, NP2DT[a.dtype], *args, buf, need, C.byref(need)))
+        return buf.raw[:need.value]
+
+    def encode_pnm(self, img, binary=True):
+        return self._encode(self.lib.ipfr_encode_pnm, img, int(bool(binary)))
+
+    def encode_pfm(self, img):
+        return self._encode(self.lib.ipfr_encode_pfm, img)

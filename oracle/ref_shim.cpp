@@ -2,12 +2,13 @@
 //
 // C-ABI wrapper around the reference ipf headers, compiled by oracle/Makefile
 // from the reference source tree (include/ipf/filter.hpp, geometry.hpp,
-// core.hpp, parallel.hpp, features.hpp, color.hpp, fft.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
+// core.hpp, parallel.hpp, features.hpp, color.hpp, fft.hpp, codec/pnm.hpp) into oracle/_ref/libipf_ref.so. It exists to (1) pin the oracle
 // restatement bit-for-bit against the real reference, (2) write the golden
 // fixtures under tests/golden/, and (3) serve as the "reference" CPU baseline.
 // Signatures match oracle/ipf_oracle.cpp's ipfo_* functions one for one.
 #include <ipf/features.hpp>
 #include <ipf/fft.hpp>
+#include <ipf/io.hpp>
 #include <ipf/filter.hpp>
 #include <ipf/geometry.hpp>
 #include <ipf/metrics.hpp>
@@ -189,6 +190,59 @@ int ipfr_hough_circles(const void* src, int w, int h, int cn, int dtype, int min
                 out[4 * i + 2] = cs[i].radius;
                 out[4 * i + 3] = cs[i].votes;
             }
+            return 0;
+        });
+    });
+}
+
+// Netpbm / PFM codecs (codec/pnm.hpp): decode into out (capacity samples),
+// shape into w, h, cn; encode into out (capacity bytes), size into *needed.
+int ipfr_decode_pnm(const unsigned char* bytes, size_t n, int depth16, void* out, size_t capacity, int* w, int* h,
+                    int* cn) {
+    return guarded([&] {
+        const std::vector<std::uint8_t> b(bytes, bytes + n);
+        auto put = [&](const auto& img) {
+            *w = img.width();
+            *h = img.height();
+            *cn = img.channels();
+            if (img.sampleCount() <= capacity) unwrap(img, out);
+            return 0;
+        };
+        return depth16 ? put(ipf::decodePNM<std::uint16_t>(b)) : put(ipf::decodePNM<std::uint8_t>(b));
+    });
+}
+
+int ipfr_decode_pfm(const unsigned char* bytes, size_t n, float* out, size_t capacity, int* w, int* h, int* cn) {
+    return guarded([&] {
+        const ipf::ImageF img = ipf::decodePFM(std::vector<std::uint8_t>(bytes, bytes + n));
+        *w = img.width();
+        *h = img.height();
+        *cn = img.channels();
+        if (img.sampleCount() <= capacity) unwrap(img, out);
+        return 0;
+    });
+}
+
+int ipfr_encode_pnm(const void* src, int w, int h, int cn, int dtype, int binary, unsigned char* out,
+                    size_t capacity, size_t* needed) {
+    return guarded([&] {
+        std::vector<std::uint8_t> b;
+        if (dtype == 0) b = ipf::encodePNM(wrap<std::uint8_t>(src, w, h, cn), binary != 0);
+        else b = ipf::encodePNM(wrap<std::uint16_t>(src, w, h, cn), binary != 0);
+        *needed = b.size();
+        if (b.size() <= capacity) std::memcpy(out, b.data(), b.size());
+        return 0;
+    });
+}
+
+int ipfr_encode_pfm(const void* src, int w, int h, int cn, int dtype, unsigned char* out, size_t capacity,
+                    size_t* needed) {
+    return guarded([&] {
+        return with_type(dtype, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const std::vector<std::uint8_t> b = ipf::encodePFM(wrap<T>(src, w, h, cn));
+            *needed = b.size();
+            if (b.size() <= capacity) std::memcpy(out, b.data(), b.size());
             return 0;
         });
     });

@@ -6,6 +6,7 @@
 //   frontend_check run <dir>     writes inputs and outputs as raw files
 //   frontend_check motion n a..  prints motionBlurKernel(n, a) (no GPU needed)
 #include <ipf/ipf.hpp>
+#include <ipf/npy.hpp>
 
 #include <cstdio>
 #include <cstdlib>
@@ -64,6 +65,11 @@ int errors() {
                         "ipf: filter2DFFT: kernel must be single-channel");
     bad += expect_error("filter2DFFT empty", [&] { filter2DFFT(ImageU8(0, 3, 1), ImageF(3, 3, 1, 1.f)); },
                         "ipf: copyMakeBorder: empty source image");
+    bad += expect_error("pnm magic", [&] { decodePNM(std::vector<std::uint8_t>{'P', '7', ' '}); },
+                        "ipf: PNM: not a P1-P6 netpbm file");
+    bad += expect_error("pnm truncated", [&] { decodePNM(std::vector<std::uint8_t>{'P', '5', ' ', '2', ' ', '1', ' ', '9', ' ', 0}); },
+                        "ipf: PNM: unexpected end of data (truncated file?)");
+    bad += expect_error("pfm channels", [&] { encodePFM(ImageF(2, 2, 2)); }, "ipf: encodePFM: image must have 1 or 3 channels");
     bad += expect_error("hough channels", [&] { houghCircles(ImageU8(8, 8, 3), 2, 4); },
                         "ipf: houghCircles: expected a single-channel image");
     bad += expect_error("hough radii", [&] { houghCircles(img, 5, 4); }, "ipf: houghCircles: invalid radius range");
@@ -183,6 +189,15 @@ int run(const std::string& dir) {
         f.write(reinterpret_cast<const char*>(flat.data()), static_cast<std::streamsize>(flat.size() * sizeof(double)));
         std::printf("CIRCLES %zu\n", cs.size());
     }
+    {
+        // decoding on the device: the files the encoders write come back unchanged
+        const ImageU8 p6 = decodePNM(encodePNM(img));
+        const ImageU16 p5 = decodePNM<std::uint16_t>(encodePNM(gray, false));
+        const ImageF pf = decodePFM(encodePFM(imgf));
+        if (!(p6 == img) || !(pf == imgf)) return 1;
+        for (std::size_t i = 0; i < gray.sampleCount(); ++i)
+            if (p5.data()[i] != gray.data()[i] * 257u) return 1;
+    }
     save(dir + "/o31.bin", integralImage(img));
     save(dir + "/o32.bin", integralImage(imgf, true));
     {
@@ -198,7 +213,33 @@ int run(const std::string& dir) {
 }
 }  // namespace
 
+// Host-only codec paths: PNM / PFM encoders and .npy files (no device work).
+int codec(const std::string& dir) {
+    ImageU8 rgb(13, 7, 3);
+    ImageU16 g16(9, 4, 1);
+    ImageF f(6, 5, 3);
+    for (std::size_t i = 0; i < rgb.sampleCount(); ++i) rgb.data()[i] = static_cast<std::uint8_t>(i * 37 + 5);
+    for (std::size_t i = 0; i < g16.sampleCount(); ++i) g16.data()[i] = static_cast<std::uint16_t>(i * 2741 + 11);
+    for (std::size_t i = 0; i < f.sampleCount(); ++i) f.data()[i] = static_cast<float>(i) * 0.37f - 3.0f;
+    writeFile(dir + "/rgb_bin.ppm", encodePNM(rgb));
+    writeFile(dir + "/rgb_txt.ppm", encodePNM(rgb, false));
+    writeFile(dir + "/g16_bin.pgm", encodePNM(g16));
+    writeFile(dir + "/g16_txt.pgm", encodePNM(g16, false));
+    writeFile(dir + "/f.pfm", encodePFM(f));
+    npy::save(rgb, dir + "/rgb.npy");
+    npy::save(g16, dir + "/g16.npy");
+    npy::save(f, dir + "/f.npy");
+    const ImageF back = npy::load<float>(dir + "/f.npy");
+    const ImageU8 rgb2 = npy::load<std::uint8_t>(dir + "/rgb.npy");
+    if (!(back == f) || !(rgb2 == rgb)) return 1;
+    const ImageD wide = npy::load<double>(dir + "/in.npy");  // written by numpy
+    save(dir + "/npy_in.bin", wide);
+    std::printf("CODEC OK\n");
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 3 && std::strcmp(argv[1], "codec") == 0) return codec(argv[2]);
     if (argc >= 2 && std::strcmp(argv[1], "errors") == 0) return errors();
     if (argc >= 3 && std::strcmp(argv[1], "run") == 0) return run(argv[2]);
     if (argc >= 2 && std::strcmp(argv[1], "motion") == 0) return motion(argc - 2, argv + 2);

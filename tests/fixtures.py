@@ -29,10 +29,33 @@ def case_id(case) -> str:
     return f"{c['id']}:{c['op']}:{inp.dtype}:{'x'.join(map(str, inp.shape))}:{tag}"
 
 
+CODEC_OPS = ("pnm_dec", "pnm_enc", "pfm_enc")
+
+
+def _run_codec(case, gpu_ipf=None):
+    """Netpbm / PFM cases: the numpy restatement of the reference's codecs, or
+    (gpu_ipf) the package (decoding unpacks on the GPU)."""
+    from oracle import pyoracle as po
+    c, inp, _, _ = case
+    p = c["params"]
+    if c["op"] == "pnm_dec":
+        b = inp.tobytes()
+        if gpu_ipf is not None:
+            return gpu_ipf.decode_pnm(b, np.dtype(p["dtype"]))
+        return po.pfm_decode(b) if p["dtype"] == "float32" else po.pnm_decode(b, np.dtype(p["dtype"]))
+    if c["op"] == "pnm_enc":
+        b = (gpu_ipf.encode_pnm if gpu_ipf is not None else po.pnm_encode)(inp, bool(p["binary"]))
+    else:
+        b = (gpu_ipf.encode_pfm if gpu_ipf is not None else po.pfm_encode)(inp)
+    return np.frombuffer(b, np.uint8)
+
+
 def run_oracle(o, case):
     """Runs a fixture case through the oracle (oracle.pyoracle.Oracle)."""
     c, inp, _, ex = case
     p = c["params"]
+    if c["op"] in CODEC_OPS:
+        return _run_codec(case)
     if c["op"] in ("pyr_down", "pyr_up"):
         return getattr(o, c["op"])(inp)
     if c["op"] == "ssim":
@@ -115,6 +138,8 @@ def run_gpu(ipf, case, precision=0):
     """Runs a fixture case through libipfg (the imageprocessingflex_amd API)."""
     c, inp, _, ex = case
     p = c["params"]
+    if c["op"] in CODEC_OPS:
+        return _run_codec(case, ipf)
     if c["op"] in ("pyr_down", "pyr_up"):
         return getattr(ipf, c["op"])(inp)
     if c["op"] == "ssim":

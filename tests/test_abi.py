@@ -144,3 +144,36 @@ def test_tuning_table_round_trip():
             ipf.tune_import(bad)
     ipf.tune_clear()
     assert ipf.tune_export() == ""
+
+
+def test_pnm_encoders_match_reference_bytes():
+    # encodePNM / encodePFM run on the host: the package's bytes equal the
+    # reference encoder's for every fixture image
+    n = 0
+    for case in fixtures.load_cases():
+        if case[0]["op"] in ("pnm_enc", "pfm_enc"):
+            got = fixtures.run_gpu(ipf, case)
+            assert np.array_equal(got, case[2]), fixtures.case_id(case)
+            n += 1
+    assert n >= 10
+
+
+def test_pnm_header_errors_precede_device_work():
+    cases = [
+        (b"Q5 1 1 255\n\x00", "ipf: PNM: not a P1-P6 netpbm file"),
+        (b"P5 0 1 255\n", "ipf: PNM: invalid image dimensions"),
+        (b"P5 2 x", "ipf: PNM: malformed header or pixel data"),
+        (b"P5 99999999999 1 255\n", "ipf: PNM: number too large"),
+        (b"P5 2 1 70000\n\x00\x00", "ipf: PNM: invalid maxval"),
+        (b"P5 2 1 255\n\x00", "ipf: PNM: unexpected end of data (truncated file?)"),
+        (b"P5 2 1 255", "ipf: PNM: malformed header"),
+        (b"P1 2 1 02", "ipf: PNM: malformed P1 pixel data"),
+        (b"Pf 2 1 abc\n", "ipf: PFM: invalid scale value"),
+        (b"Pf 2 1 0\n", "ipf: PFM: invalid scale value"),
+        (b"PF 2 1 -1.0\n\x00\x00", "ipf: PFM: unexpected end of data (truncated file?)"),
+        (b"PF 20000000 1 -1.0\n", "ipf: PFM: image dimensions too large"),
+    ]
+    for b, msg in cases:
+        with pytest.raises(ipf.IpfError) as e:
+            ipf.decode_pnm(b)
+        assert str(e.value) == msg, (b, str(e.value))

@@ -28,6 +28,29 @@ def test_frontend_argument_errors(exe):
     assert r.returncode == 0 and "ERRORS OK" in r.stdout, r.stdout + r.stderr
 
 
+def test_frontend_codecs_on_host(exe, tmp_path):
+    # encodePNM / encodePFM and .npy files of the C++ front end: bytes equal the
+    # restatement of the reference's encoders (pinned to its own output), numpy
+    # reads and writes the arrays
+    from oracle import pyoracle as po
+    np.save(tmp_path / "in.npy", np.arange(24, dtype=np.int16).reshape(2, 3, 4) - 7)
+    r = subprocess.run([str(exe), "codec", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "CODEC OK" in r.stdout, r.stdout + r.stderr
+    rgb = (np.arange(13 * 7 * 3) * 37 + 5).astype(np.uint8).reshape(7, 13, 3)
+    g16 = (np.arange(36) * 2741 + 11).astype(np.uint16).reshape(4, 9)
+    f = (np.arange(90, dtype=np.float32) * np.float32(0.37) - np.float32(3.0)).reshape(5, 6, 3)
+    assert (tmp_path / "rgb_bin.ppm").read_bytes() == po.pnm_encode(rgb, True)
+    assert (tmp_path / "rgb_txt.ppm").read_bytes() == po.pnm_encode(rgb, False)
+    assert (tmp_path / "g16_bin.pgm").read_bytes() == po.pnm_encode(g16, True)
+    assert (tmp_path / "g16_txt.pgm").read_bytes() == po.pnm_encode(g16, False)
+    assert (tmp_path / "f.pfm").read_bytes() == po.pfm_encode(f)
+    assert np.array_equal(np.load(tmp_path / "rgb.npy"), rgb)
+    assert np.array_equal(np.load(tmp_path / "g16.npy"), g16)
+    assert np.array_equal(np.load(tmp_path / "f.npy"), f)
+    got = np.fromfile(tmp_path / "npy_in.bin", np.float64).reshape(2, 3, 4)
+    assert np.array_equal(got, np.arange(24).reshape(2, 3, 4) - 7.0)
+
+
 @pytest.mark.gpu
 def test_frontend_results_match_oracle(exe, oracle, tmp_path):
     r = subprocess.run([str(exe), "run", str(tmp_path)], capture_output=True, text=True, timeout=120)

@@ -1002,3 +1002,26 @@ def test_hough_circles_vs_oracle(ipf, oracle, args):
     a[2] /= 255.0
     got = ipf.hough_circles(ipf.DeviceImage.from_array(f), *a)
     assert np.array_equal(got, oracle.hough_circles(f, *a))
+
+
+# ------------------------------------------------------------ Netpbm / PFM ingest
+def test_decode_pnm_full_size_vs_restatement(ipf):
+    # 3840 x 2160 RGB files in every binary layout, decoded on the device and
+    # on the host restatement of the reference's decoder
+    from imageprocessingflex_amd import workloads as wl
+    from oracle import pyoracle as po
+    img = wl.polygon_image()
+    g16 = (img[..., 1].astype(np.uint16) * 257 - 3).astype(np.uint16)
+    for b, dt in ((po.pnm_encode(img), np.uint8), (po.pnm_encode(img), np.uint16),
+                  (po.pnm_encode(g16), np.uint16), (po.pnm_encode(g16), np.uint8),
+                  (b"P5 3840 2160 1000\n" + (g16 // 66).astype(">u2").tobytes(), np.uint8)):
+        want = po.pnm_decode(b, dt)
+        assert np.array_equal(ipf.decode_pnm(b, dt), want)
+        d = ipf.decode_pnm(b, dt, device=True)
+        assert np.array_equal(d.download(), want)
+    f = (img.astype(np.float32) / 255).astype(np.float32)
+    for b in (po.pfm_encode(f), b"PF\n3840 2160\n1.0\n" + f[::-1].astype(">f4").tobytes()):
+        assert np.array_equal(ipf.decode_pnm(b), po.pfm_decode(b))
+    bits = np.packbits(img[..., 0] > 128, axis=1)
+    b = b"P4\n3840 2160\n" + bits.tobytes()
+    assert np.array_equal(ipf.decode_pnm(b), po.pnm_decode(b))
