@@ -552,6 +552,18 @@ def traffic_from_profiles(group: str):
     return d.get(group, {}).get("bytes_per_launch")
 
 
+def calibrate_counters(nbytes: int) -> None:
+    """Kernels of known traffic for the HBM byte counters (rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE): torch's fill of an nbytes float tensor writes
+    nbytes, its sum reads them once. Run outside every timed region."""
+    import torch
+    t = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    t.fill_(1.0)
+    float(t.sum())
+    del t
+    torch.cuda.synchronize()
+
+
 def measure(wd, ipf, steps: int, warmup: int, cpu: bool, cpu_runs: int = 5):
     """One workload: the step's calls back to back (value), per-call launch
     times from HIP events (per filter, roofline of the dominant one), call
@@ -663,6 +675,7 @@ def measure(wd, ipf, steps: int, warmup: int, cpu: bool, cpu_runs: int = 5):
     copy_ms = ev.ms(evs[0], evs[1]) / 5
     copy = {"bytes_per_launch": 2 * big.nbytes, "mean_ms": round(copy_ms, 4),
             "gb_s": round(2 * big.nbytes / (copy_ms * 1e-3) / 1e9, 1)}
+    calibrate_counters(big.nbytes)
     cpu_line = cpu_baseline(wd, cpu_runs) if cpu else None
     extra = {}
     if "post" in wd and cpu:

@@ -99,5 +99,38 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
     return LIB
 
 
+ASAN_EXE = BUILD / "abi_check_asan"
+SANITIZE = ["-fsanitize=address", "-fsanitize=undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer"]
+
+
+def build_abi_check_asan() -> Path:
+    """tests/cpp/abi_check.cpp linked with the library's objects, the runtime's
+    host code compiled under AddressSanitizer and UndefinedBehaviorSanitizer
+    (device code is built as usual). Needs the objects of build()."""
+    objs = [BUILD / (s.stem + ".o") for s in sources() if s.stem != "runtime"]
+    missing = [o for o in objs if not o.exists()]
+    if missing:
+        build(incremental=True, verbose=False)
+    rt = BUILD / "runtime_asan.o"
+    host_san = [x for f in SANITIZE for x in ("-Xarch_host", f)]
+    cmd = [HIPCC, *[f for f in CFLAGS if f != "-O3"], "-O1", "-g", *host_san, "-c", str(CSRC / "runtime.hip"),
+           "-o", str(rt)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for runtime.hip (sanitized):\n{r.stderr}")
+    clang = Path(HIPCC).resolve().parent.parent / "llvm" / "bin" / "clang++"
+    drv = BUILD / "abi_check.o"
+    r = subprocess.run([str(clang), "-std=c++17", "-O1", "-g", *SANITIZE, "-I", str(PKG.parent / "include"), "-c",
+                        str(PKG.parent / "tests" / "cpp" / "abi_check.cpp"), "-o", str(drv)],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"clang++ failed for abi_check.cpp:\n{r.stderr}")
+    r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", *host_san[:4], str(drv), str(rt), *map(str, objs),
+                        "-Wl,-rpath,/opt/rocm/lib", "-pthread", "-o", str(ASAN_EXE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed (sanitized ABI check):\n{r.stderr}")
+    return ASAN_EXE
+
+
 if __name__ == "__main__":
     build(incremental="--incremental" in sys.argv)
