@@ -44,6 +44,7 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
+
 constexpr int MT = 1;              // 16-row tiles per wave (weight fragments are shared by MT x NT tiles)
 constexpr int TYo = 4 * 16 * MT;   // output rows per workgroup
 constexpr int TXo = 64;            // output columns per workgroup (4 tiles of 16)
@@ -259,6 +260,228 @@ hipError_t mf_by_prec(const MfArgs& a, int precision, hipStream_t s) {
     return mf_launch<Ti, true, 3, XS>(a, s);
 }
 
+// ---------------------------------------------------------------------------
+// Rows-on-M formulation (kernels up to 32 x 31). One wave owns a strip of 128
+// output columns and walks down its rows, one staged input row per step. For
+// input row R and a tile s (s = 0..7) of the 16 output columns x0 + s + 8 n,
+// one MFMA pair forms C[M][n] += sum_k A[M][k] B[k][n] with
+//   B[k][n] = S_R[s + 8 n + k]       (the row, 32 taps per output column),
+//   A[M][k] = K[(ph - M) mod 32][k]  (a rotation of the kernel rows),
+// M = 0..31 over two 16-row halves and ph = step mod 32. Accumulator row M
+// thereby collects output row y with kernel row j = (ph - M) mod 32 at every
+// step: one output row per accumulator row, j advancing with the steps. At
+// j = 31 (a zero kernel row) the row is complete: its 16 x 8 = 128 values
+// (8 consecutive columns per lane) are stored and the row starts over at zero.
+// Every weight fragment of the 32 rotations is laid out once by the host
+// (32 x 2 halves x parts x 1 KB); the data fragments are 16-byte LDS reads
+// from 8 copies of the staged row, copy s shifted left by s, so that every
+// lane's 8 taps start 16-byte aligned. Executed work: 32 x 32 products per
+// 31 x 31 taps (1.066) times (rows + 31) / rows per strip, times the parts.
+constexpr int RW_TX = 128;               // output columns per wave
+constexpr int RW_SPAN = 160;             // staged samples per row (128 + 31 taps, rounded up)
+constexpr int RW_PAD = 8;                // copy s keeps sample c at c - s + RW_PAD (no bounds tests)
+constexpr int RW_CP = RW_SPAN + RW_PAD;  // elements per shifted copy (pitch, a multiple of 8)
+constexpr int RW_ROWB = 8 * RW_CP;       // elements of one staged row (8 copies)
+constexpr int RW_AHEAD = 4;              // rows in flight from global memory
+
+struct RwArgs {
+    DevImage src, dst;
+    const uint4* rot;    // [32][2][parts][64] weight fragments
+    int kw, kh, ax, ay, border, sh;
+    double delta, bval, wscale;
+};
+
+// Stores the 8 consecutive outputs of one lane (columns x .. x + 7 of row y).
+template <class To>
+__device__ __forceinline__ void rw_store8(const DevImage& d, int y, int x, int ch, const float (&v)[8], double unscale,
+                                          double delta) {
+    To* row = reinterpret_cast<To*>(static_cast<char*>(d.data) + static_cast<size_t>(y) * d.stride);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+        if (x + t < d.w)
+            row[static_cast<size_t>(x + t) * d.cn + ch] =
+                saturate_from_double<To>(static_cast<double>(v[t]) * unscale + delta);
+}
+
+template <class Ti, bool BF, int WP, bool XS>
+__global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
+    using F = Fmt<BF>;
+    using T = typename F::T;
+    using V = typename F::V;
+    // per wave: two staged-row buffers (hi and, with XS, lo); waves never
+    // touch each other's buffers, so they synchronise with themselves only
+    __shared__ __align__(16) T stage[4][2][XS ? 2 : 1][RW_ROWB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int W = a.src.w, H = a.src.h, cn = a.src.cn, ch = blockIdx.z;
+    const int x0 = (blockIdx.x * 4 + wave) * RW_TX;
+    const int y0 = blockIdx.y * a.sh;
+    const int rows = min(a.sh, H - y0);
+    const int nsteps = rows + 31;
+    const int cx0 = x0 - a.ax;  // image column of staged sample 0
+    const char* __restrict__ src = static_cast<const char*>(a.src.data);
+    // lane l stages samples l, l + 64, l + 128 (< RW_SPAN) of every row: their
+    // byte offsets in a row are fixed for the strip (-1: Constant border)
+    long long coff[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int sx = border_index(cx0 + lane + 64 * i, W, a.border);
+        coff[i] = sx < 0 ? -1 : (static_cast<long long>(sx) * cn + ch) * static_cast<long long>(sizeof(Ti));
+    }
+    const float bvalf = static_cast<float>(a.bval);
+    auto fetch = [&](int q, float (&v)[3]) {
+        const int sy = border_index(y0 - a.ay + q, H, a.border);  // wave-uniform
+        const char* rp = src + static_cast<size_t>(sy < 0 ? 0 : sy) * a.src.stride;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const bool use = sy >= 0 && coff[i] >= 0;
+            v[i] = use ? static_cast<float>(*reinterpret_cast<const Ti*>(rp + (coff[i] < 0 ? 0 : coff[i]))) : bvalf;
+        }
+    };
+    // fp16 keeps float inputs in its normal range: one power-of-two scale per
+    // strip, max |x| of the strip's staged samples in [2^13, 2^14)
+    float xscale = 1.0f;
+    if constexpr (XS && !BF) {
+        float m = 0.f;
+        for (int q = 0; q < nsteps; ++q) {
+            float v[3];
+            fetch(q, v);
+            m = fmaxf(m, fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), lane + 128 < RW_SPAN ? fabsf(v[2]) : 0.f)));
+        }
+        const float mx = __uint_as_float(wave_max_u32(__float_as_uint(m)));
+        if (mx > 0.f && mx < INFINITY) {
+            int ex;
+            frexpf(mx, &ex);
+            xscale = ldexpf(1.0f, 14 - ex);
+        }
+    }
+    // rows in flight: ring of RW_AHEAD register sets
+    float ring[RW_AHEAD][3];
+    auto store_row = [&](const float (&v)[3], int buf) {
+        T* hi = stage[wave][buf][0] + RW_PAD;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int c = lane + 64 * i;
+            if (i == 2 && c >= RW_SPAN) break;
+            const float x = v[i] * xscale;
+            const T h = static_cast<T>(x);
+#pragma unroll
+            for (int sft = 0; sft < 8; ++sft) hi[sft * RW_CP + c - sft] = h;
+            if constexpr (XS) {
+                const T l = static_cast<T>(x - static_cast<float>(h));
+                T* lo = stage[wave][buf][1] + RW_PAD;
+#pragma unroll
+                for (int sft = 0; sft < 8; ++sft) lo[sft * RW_CP + c - sft] = l;
+            }
+        }
+    };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    float4v acc[8][2];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t][0] = acc[t][1] = float4v{0.f, 0.f, 0.f, 0.f};
+    const int n = lane & 15, g = lane >> 4;
+    const int boff = RW_PAD + 8 * n + 8 * g;  // element offset of this lane's 8 taps in a copy
+    const double unscale = 1.0 / (static_cast<double>(xscale) * a.wscale);
+#pragma unroll
+    for (int r = 0; r < RW_AHEAD; ++r) fetch(r, ring[r]);
+    store_row(ring[0], 0);
+    fetch(RW_AHEAD, ring[0]);
+    wave_sync();
+    // steps in groups of 4 (= RW_AHEAD): the accumulator element that
+    // finishes and the register set of the prefetch ring are compile-time
+    // indices after unrolling; which half finishes is uniform
+    for (int q0 = 0; q0 < nsteps; q0 += 4) {
+#pragma unroll
+        for (int E = 0; E < 4; ++E) {
+            const int q = q0 + E;
+            if (q < nsteps) {
+                const int buf = q & 1, ph = q & 31;
+                V wa[2][WP];
+#pragma unroll
+                for (int mh = 0; mh < 2; ++mh)
+#pragma unroll
+                    for (int p = 0; p < WP; ++p)
+                        wa[mh][p] = __builtin_bit_cast(V, a.rot[((ph * 2 + mh) * WP + p) * 64 + lane]);
+                V xb[8], xl[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    xb[t] = *reinterpret_cast<const V*>(&stage[wave][buf][0][t * RW_CP + boff]);
+                    if constexpr (XS) xl[t] = *reinterpret_cast<const V*>(&stage[wave][buf][1][t * RW_CP + boff]);
+                }
+#pragma unroll
+                for (int p = 0; p < WP; ++p)
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+#pragma unroll
+                        for (int mh = 0; mh < 2; ++mh) acc[t][mh] = F::mfma(wa[mh][p], xb[t], acc[t][mh]);
+                if constexpr (XS) {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+#pragma unroll
+                        for (int mh = 0; mh < 2; ++mh) acc[t][mh] = F::mfma(wa[mh][0], xl[t], acc[t][mh]);
+                }
+                // the next row into the other buffer (read two steps ago), the
+                // row after the prefetched ones into the freed register set
+                store_row(ring[(E + 1) & 3], buf ^ 1);
+                fetch(q + 1 + RW_AHEAD, ring[(E + 1) & 3]);
+                // accumulator row (ph + 1) mod 32 has seen kernel rows 0..31: output row y0 + q - 31
+                const int ef = (E + 1) & 3;
+                const int Mf = (ph + 1) & 31, gf = (Mf & 15) >> 2;
+                const bool upper = (Mf >> 4) != 0, mine = g == gf;
+                const int y = y0 + q - 31;
+                float v[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const float v0 = acc[t][0][ef], v1 = acc[t][1][ef];
+                    v[t] = upper ? v1 : v0;
+                    acc[t][0][ef] = (mine && !upper) ? 0.f : v0;
+                    acc[t][1][ef] = (mine && upper) ? 0.f : v1;
+                }
+                if (mine && y >= y0) {
+                    const int x = x0 + 8 * n;
+                    switch (a.dst.dt) {
+                        case IPFG_U8: rw_store8<uint8_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                        case IPFG_U16: rw_store8<uint16_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                        case IPFG_I16: rw_store8<int16_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                        case IPFG_I32: rw_store8<int32_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                        case IPFG_F32: rw_store8<float>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                        default: rw_store8<double>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                    }
+                }
+                wave_sync();
+            }
+        }
+    }
+}
+
+template <class Ti, bool BF, int WP, bool XS>
+hipError_t rw_launch(const RwArgs& a0, hipStream_t s) {
+    RwArgs a = a0;
+    const int strips = (a.src.w + RW_TX - 1) / RW_TX;
+    const int gx = (strips + 3) / 4;
+    // about two waves per SIMD on 256 CUs, strips at least 32 rows tall
+    long long nby = 2048 / std::max(1, strips);
+    int sh = static_cast<int>((a.src.h + std::max(1LL, nby) - 1) / std::max(1LL, nby));
+    sh = std::max(sh, 32);
+    a.sh = sh;
+    const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>((a.src.h + sh - 1) / sh),
+                    static_cast<unsigned>(a.src.cn));
+    hipLaunchKernelGGL((f2d_mfma_rows_kernel<Ti, BF, WP, XS>), grid, dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class Ti, bool XS>
+hipError_t rw_by_prec(const RwArgs& a, int precision, hipStream_t s) {
+    if (precision == IPFG_PREC_FP16X2) return rw_launch<Ti, false, 2, XS>(a, s);
+    return rw_launch<Ti, true, 3, XS>(a, s);
+}
+
+bool rows_fit(int kw, int kh) { return kw <= 32 && kh <= 31; }
+
 }  // namespace
 
 size_t f2d_mfma_lds_bytes(int kw, int kh, bool split_input) {
@@ -267,12 +490,36 @@ size_t f2d_mfma_lds_bytes(int kw, int kh, bool split_input) {
 
 int f2d_mfma_pairs(int kw, int kh) { return (((kh * mf_nc(kw) + 1) / 2 + PD - 1) / PD) * PD; }
 
-// Weight fragments in register order: for part p, pair s, lane l, element e:
-// chunk q = 2 s + (l >= 32), row j = q / NC, chunk column cc = q % NC,
-// k = 8 ((l % 32) / 16) + e, n = l % 16, tap i = 16 cc + k - n.
+namespace {
+// The parts (hi, lo[, lo2]) of a scaled weight in the operand format, each
+// rounded from what the earlier parts left.
+void split_parts(double w, bool bf, int parts, uint16_t* bits) {
+    double rest = w;
+    for (int p = 0; p < parts; ++p) {
+        double got;
+        if (bf) {
+            const __bf16 h = static_cast<__bf16>(static_cast<float>(rest));
+            std::memcpy(&bits[p], &h, 2);
+            got = static_cast<double>(static_cast<float>(h));
+        } else {
+            const _Float16 h = static_cast<_Float16>(static_cast<float>(rest));
+            std::memcpy(&bits[p], &h, 2);
+            got = static_cast<double>(static_cast<float>(h));
+        }
+        rest -= got;
+    }
+}
+}  // namespace
+
+// Weight fragments in register order.
+// Rows-on-M kernels (kw <= 32, kh <= 31): for rotation r, half mh, part p,
+// lane l, element e: accumulator row M = 16 mh + l % 16, kernel row
+// j = (r - M) mod 32, tap k = 8 (l / 16) + e.
+// Otherwise: for part p, pair s, lane l, element e: chunk q = 2 s + (l >= 32),
+// row j = q / NC, chunk column cc = q % NC, k = 8 ((l % 32) / 16) + e,
+// n = l % 16, tap i = 16 cc + k - n.
 void f2d_mfma_fragments(const double* kd, int kw, int kh, int precision, std::vector<uint16_t>& out,
                         double& wscale) {
-    const int NC = mf_nc(kw), pairs = f2d_mfma_pairs(kw, kh);
     const bool bf = precision != IPFG_PREC_FP16X2;
     const int parts = bf ? 3 : 2;
     double mx = 0;
@@ -283,6 +530,22 @@ void f2d_mfma_fragments(const double* kd, int kw, int kh, int precision, std::ve
         std::frexp(mx, &ex);
         wscale = std::ldexp(1.0, 14 - ex);
     }
+    uint16_t bits[3];
+    if (rows_fit(kw, kh)) {
+        out.assign(static_cast<size_t>(32) * 2 * parts * 64 * 8, 0);
+        for (int r = 0; r < 32; ++r)
+            for (int mh = 0; mh < 2; ++mh)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int M = 16 * mh + l % 16, j = ((r - M) % 32 + 32) % 32, k = 8 * (l / 16) + e;
+                        const double w = (j < kh && k < kw) ? kd[static_cast<size_t>(j) * kw + k] * wscale : 0.0;
+                        split_parts(w, bf, parts, bits);
+                        for (int p = 0; p < parts; ++p)
+                            out[((((static_cast<size_t>(r) * 2 + mh) * parts + p) * 64 + l) * 8) + e] = bits[p];
+                    }
+        return;
+    }
+    const int NC = mf_nc(kw), pairs = f2d_mfma_pairs(kw, kh);
     out.assign(static_cast<size_t>(parts) * pairs * 64 * 8, 0);
     for (int s = 0; s < pairs; ++s)
         for (int l = 0; l < 64; ++l)
@@ -293,26 +556,35 @@ void f2d_mfma_fragments(const double* kd, int kw, int kh, int precision, std::ve
                 const int i = 16 * cc + k - n;
                 double w = 0.0;
                 if (j < kh && i >= 0 && i < kw) w = kd[static_cast<size_t>(j) * kw + i] * wscale;
-                double rest = w;
-                for (int p = 0; p < parts; ++p) {
-                    uint16_t bits;
-                    double got;
-                    if (bf) {
-                        const __bf16 h = static_cast<__bf16>(static_cast<float>(rest));
-                        std::memcpy(&bits, &h, 2);
-                        got = static_cast<double>(static_cast<float>(h));
-                    } else {
-                        const _Float16 h = static_cast<_Float16>(static_cast<float>(rest));
-                        std::memcpy(&bits, &h, 2);
-                        got = static_cast<double>(static_cast<float>(h));
-                    }
-                    out[((static_cast<size_t>(p) * pairs + s) * 64 + l) * 8 + e] = bits;
-                    rest -= got;
-                }
+                split_parts(w, bf, parts, bits);
+                for (int p = 0; p < parts; ++p) out[((static_cast<size_t>(p) * pairs + s) * 64 + l) * 8 + e] = bits[p];
             }
 }
 
 hipError_t launch_filter2d_mfma(const F2dLaunch& l, hipStream_t s) {
+    if (rows_fit(l.kw, l.kh)) {
+        RwArgs r{};
+        r.src = l.src;
+        r.dst = l.dst;
+        r.rot = static_cast<const uint4*>(l.mfma_w);
+        r.kw = l.kw;
+        r.kh = l.kh;
+        r.ax = l.ax;
+        r.ay = l.ay;
+        r.border = l.border;
+        r.delta = l.delta;
+        r.bval = l.bval;
+        r.wscale = l.mfma_wscale;
+        switch (l.src.dt) {
+            case IPFG_U8: return rw_by_prec<uint8_t, false>(r, l.precision, s);
+            case IPFG_U16: return rw_by_prec<uint16_t, true>(r, l.precision, s);
+            case IPFG_I16: return rw_by_prec<int16_t, true>(r, l.precision, s);
+            case IPFG_I32: return rw_by_prec<int32_t, true>(r, l.precision, s);
+            case IPFG_F32: return rw_by_prec<float, true>(r, l.precision, s);
+            case IPFG_F64: return rw_by_prec<double, true>(r, l.precision, s);
+        }
+        return hipErrorInvalidValue;
+    }
     MfArgs a{};
     a.src = l.src;
     a.dst = l.dst;
