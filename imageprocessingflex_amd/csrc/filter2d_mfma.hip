@@ -32,6 +32,7 @@
 // host, float inputs per tile from the tile's max |x|) so that they stay in
 // fp16's normal range; the scale is removed exactly in the epilogue.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -283,36 +284,61 @@ constexpr int RW_PAD = 8;                // copy s keeps sample c at c - s + RW_
 constexpr int RW_CP = RW_SPAN + RW_PAD;  // elements per shifted copy (pitch, a multiple of 8)
 constexpr int RW_ROWB = 8 * RW_CP;       // elements of one staged row (8 copies)
 constexpr int RW_AHEAD = 4;              // rows in flight from global memory
+constexpr int RW_WK = 40;                // weight table pitch (32 taps + 8: rows 80 bytes apart, no bank conflicts)
 
 struct RwArgs {
     DevImage src, dst;
-    const uint4* rot;    // [32][2][parts][64] weight fragments
+    const uint4* wt;     // [parts][32 kernel rows][RW_WK] weight table (zero outside kh x kw)
     int kw, kh, ax, ay, border, sh;
     double delta, bval, wscale;
 };
 
-// Stores the 8 consecutive outputs of one lane (columns x .. x + 7 of row y).
+// Stores outputs x and x + 1 of row y (columns past the image are skipped).
 template <class To>
-__device__ __forceinline__ void rw_store8(const DevImage& d, int y, int x, int ch, const float (&v)[8], double unscale,
-                                          double delta) {
+__device__ __forceinline__ void rw_store2(const DevImage& d, int y, int x, int ch, float v0, float v1,
+                                          double unscale, double delta) {
     To* row = reinterpret_cast<To*>(static_cast<char*>(d.data) + static_cast<size_t>(y) * d.stride);
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-        if (x + t < d.w)
-            row[static_cast<size_t>(x + t) * d.cn + ch] =
-                saturate_from_double<To>(static_cast<double>(v[t]) * unscale + delta);
+    const To o0 = saturate_from_double<To>(static_cast<double>(v0) * unscale + delta);
+    const To o1 = saturate_from_double<To>(static_cast<double>(v1) * unscale + delta);
+    struct alignas(2 * sizeof(To)) Pair {
+        To a, b;
+    };
+    if (d.cn == 1 && x + 1 < d.w && (reinterpret_cast<uintptr_t>(row) & (2 * sizeof(To) - 1)) == 0) {
+        *reinterpret_cast<Pair*>(row + x) = Pair{o0, o1};
+    } else {
+        if (x < d.w) row[static_cast<size_t>(x) * d.cn + ch] = o0;
+        if (x + 1 < d.w) row[static_cast<size_t>(x + 1) * d.cn + ch] = o1;
+    }
+}
+
+// Waves per SIMD the rows kernels are compiled for (registers) and launched
+// at (one full round of workgroups): 3, or 2 for bf16 with split inputs
+// (whose LDS traffic gains nothing from a third wave) and for double inputs
+// (which would spill at 3).
+template <class Ti, bool BF, bool XS>
+constexpr int rw_occupancy() {
+    return ((BF && XS) || sizeof(Ti) == 8) ? 2 : 3;
 }
 
 template <class Ti, bool BF, int WP, bool XS>
-__global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(rw_occupancy<Ti, BF, XS>(), rw_occupancy<Ti, BF, XS>())))
+void f2d_mfma_rows_kernel(RwArgs a) {
     using F = Fmt<BF>;
     using T = typename F::T;
     using V = typename F::V;
-    // per wave: two staged-row buffers (hi and, with XS, lo); waves never
-    // touch each other's buffers, so they synchronise with themselves only
+    // per wave: two staged-row buffers (hi and, with XS, lo) and a parking
+    // row for the finished sums; waves never touch each other's, so after the
+    // weight table is in place they synchronise with themselves only
     __shared__ __align__(16) T stage[4][2][XS ? 2 : 1][RW_ROWB];
+    __shared__ __align__(16) T wtab[WP][32][RW_WK];
+    __shared__ __align__(16) float park[4][RW_TX];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    {
+        constexpr int n16 = WP * 32 * RW_WK * 2 / 16;
+        uint4* w = reinterpret_cast<uint4*>(&wtab[0][0][0]);
+        for (int i = tid; i < n16; i += kBlock) w[i] = a.wt[i];
+    }
     const int W = a.src.w, H = a.src.h, cn = a.src.cn, ch = blockIdx.z;
     const int x0 = (blockIdx.x * 4 + wave) * RW_TX;
     const int y0 = blockIdx.y * a.sh;
@@ -321,49 +347,62 @@ __global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
     const int cx0 = x0 - a.ax;  // image column of staged sample 0
     const char* __restrict__ src = static_cast<const char*>(a.src.data);
     // lane l stages samples l, l + 64, l + 128 (< RW_SPAN) of every row: their
-    // byte offsets in a row are fixed for the strip (-1: Constant border)
+    // byte offsets in a row are fixed for the strip. Loads always read inside
+    // the image (offset 0 stands in for a Constant-border column, row 0 for a
+    // Constant-border row) and the border value replaces them when the row is
+    // staged, so no load sits under a branch and the ring keeps them in flight.
     long long coff[3];
+    bool cok[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int sx = border_index(cx0 + lane + 64 * i, W, a.border);
-        coff[i] = sx < 0 ? -1 : (static_cast<long long>(sx) * cn + ch) * static_cast<long long>(sizeof(Ti));
+        cok[i] = sx >= 0;
+        coff[i] = cok[i] ? (static_cast<long long>(sx) * cn + ch) * static_cast<long long>(sizeof(Ti)) : 0;
     }
     const float bvalf = static_cast<float>(a.bval);
-    auto fetch = [&](int q, float (&v)[3]) {
+    auto fetch = [&](int q, Ti (&v)[3]) -> bool {
         const int sy = border_index(y0 - a.ay + q, H, a.border);  // wave-uniform
         const char* rp = src + static_cast<size_t>(sy < 0 ? 0 : sy) * a.src.stride;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const bool use = sy >= 0 && coff[i] >= 0;
-            v[i] = use ? static_cast<float>(*reinterpret_cast<const Ti*>(rp + (coff[i] < 0 ? 0 : coff[i]))) : bvalf;
-        }
+        for (int i = 0; i < 3; ++i) v[i] = *reinterpret_cast<const Ti*>(rp + coff[i]);
+        return sy >= 0;
+    };
+    auto value = [&](const Ti (&v)[3], bool row_ok, int i) {
+        return (row_ok && cok[i]) ? static_cast<float>(v[i]) : bvalf;
     };
     // fp16 keeps float inputs in its normal range: one power-of-two scale per
     // strip, max |x| of the strip's staged samples in [2^13, 2^14)
     float xscale = 1.0f;
     if constexpr (XS && !BF) {
         float m = 0.f;
-        for (int q = 0; q < nsteps; ++q) {
-            float v[3];
-            fetch(q, v);
-            m = fmaxf(m, fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), lane + 128 < RW_SPAN ? fabsf(v[2]) : 0.f)));
+        Ti pv[RW_AHEAD][3];
+        bool pok[RW_AHEAD];
+#pragma unroll
+        for (int r = 0; r < RW_AHEAD; ++r) pok[r] = fetch(r, pv[r]);
+        for (int q0 = 0; q0 < nsteps; q0 += RW_AHEAD) {
+#pragma unroll
+            for (int E = 0; E < RW_AHEAD; ++E) {
+                if (q0 + E < nsteps) {
+                    m = fmaxf(m, fmaxf(fabsf(value(pv[E], pok[E], 0)),
+                                       fmaxf(fabsf(value(pv[E], pok[E], 1)),
+                                             lane + 128 < RW_SPAN ? fabsf(value(pv[E], pok[E], 2)) : 0.f)));
+                    pok[E] = fetch(q0 + E + RW_AHEAD, pv[E]);
+                }
+            }
         }
         const float mx = __uint_as_float(wave_max_u32(__float_as_uint(m)));
-        if (mx > 0.f && mx < INFINITY) {
-            int ex;
-            frexpf(mx, &ex);
-            xscale = ldexpf(1.0f, 14 - ex);
-        }
+        if (mx > 0.f && mx < INFINITY) xscale = ldexpf(1.0f, 13 - ilogbf(mx));  // mx in [2^e, 2^(e+1))
     }
-    // rows in flight: ring of RW_AHEAD register sets
-    float ring[RW_AHEAD][3];
-    auto store_row = [&](const float (&v)[3], int buf) {
+    // rows in flight: ring of RW_AHEAD raw sample sets and their row flags
+    Ti ring[RW_AHEAD][3];
+    bool rok[RW_AHEAD];
+    auto store_row = [&](const Ti (&v)[3], bool row_ok, int buf) {
         T* hi = stage[wave][buf][0] + RW_PAD;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const int c = lane + 64 * i;
             if (i == 2 && c >= RW_SPAN) break;
-            const float x = v[i] * xscale;
+            const float x = value(v, row_ok, i) * xscale;
             const T h = static_cast<T>(x);
 #pragma unroll
             for (int sft = 0; sft < 8; ++sft) hi[sft * RW_CP + c - sft] = h;
@@ -387,10 +426,21 @@ __global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
     const int boff = RW_PAD + 8 * n + 8 * g;  // element offset of this lane's 8 taps in a copy
     const double unscale = 1.0 / (static_cast<double>(xscale) * a.wscale);
 #pragma unroll
-    for (int r = 0; r < RW_AHEAD; ++r) fetch(r, ring[r]);
-    store_row(ring[0], 0);
-    fetch(RW_AHEAD, ring[0]);
-    wave_sync();
+    for (int r = 0; r < RW_AHEAD; ++r) rok[r] = fetch(r, ring[r]);
+    store_row(ring[0], rok[0], 0);
+    rok[0] = fetch(RW_AHEAD, ring[0]);
+    __syncthreads();  // weight table and every wave's first row
+    // A fragments of the next step (read one step ahead): accumulator row
+    // M = 16 mh + n takes kernel row (ph - M) mod 32
+    V wnext[2][WP];
+    auto load_weights = [&](int ph) {
+#pragma unroll
+        for (int mh = 0; mh < 2; ++mh)
+#pragma unroll
+            for (int p = 0; p < WP; ++p)
+                wnext[mh][p] = *reinterpret_cast<const V*>(&wtab[p][(ph - 16 * mh - n) & 31][8 * g]);
+    };
+    load_weights(0);
     // steps in groups of 4 (= RW_AHEAD): the accumulator element that
     // finishes and the register set of the prefetch ring are compile-time
     // indices after unrolling; which half finishes is uniform
@@ -404,35 +454,49 @@ __global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
 #pragma unroll
                 for (int mh = 0; mh < 2; ++mh)
 #pragma unroll
-                    for (int p = 0; p < WP; ++p)
-                        wa[mh][p] = __builtin_bit_cast(V, a.rot[((ph * 2 + mh) * WP + p) * 64 + lane]);
+                    for (int p = 0; p < WP; ++p) wa[mh][p] = wnext[mh][p];
                 V xb[8], xl[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     xb[t] = *reinterpret_cast<const V*>(&stage[wave][buf][0][t * RW_CP + boff]);
                     if constexpr (XS) xl[t] = *reinterpret_cast<const V*>(&stage[wave][buf][1][t * RW_CP + boff]);
                 }
-#pragma unroll
-                for (int p = 0; p < WP; ++p)
-#pragma unroll
-                    for (int t = 0; t < 8; ++t)
-#pragma unroll
-                        for (int mh = 0; mh < 2; ++mh) acc[t][mh] = F::mfma(wa[mh][p], xb[t], acc[t][mh]);
                 if constexpr (XS) {
+#pragma unroll
+                    for (int p = 0; p < WP; ++p)
+#pragma unroll
+                        for (int t = 0; t < 8; ++t)
+#pragma unroll
+                            for (int mh = 0; mh < 2; ++mh) acc[t][mh] = F::mfma(wa[mh][p], xb[t], acc[t][mh]);
 #pragma unroll
                     for (int t = 0; t < 8; ++t)
 #pragma unroll
                         for (int mh = 0; mh < 2; ++mh) acc[t][mh] = F::mfma(wa[mh][0], xl[t], acc[t][mh]);
+                } else {
+#pragma unroll
+                    for (int p = 0; p < WP; ++p)
+#pragma unroll
+                        for (int t = 0; t < 8; ++t) acc[t][0] = F::mfma(wa[0][p], xb[t], acc[t][0]);
+                    // rows M >= 16 finish (as warm-up rows, discarded) at step
+                    // M - 1 >= 15 and start over: their earlier products are unused
+                    if (q >= 16) {
+#pragma unroll
+                        for (int p = 0; p < WP; ++p)
+#pragma unroll
+                            for (int t = 0; t < 8; ++t) acc[t][1] = F::mfma(wa[1][p], xb[t], acc[t][1]);
+                    }
                 }
                 // the next row into the other buffer (read two steps ago), the
                 // row after the prefetched ones into the freed register set
-                store_row(ring[(E + 1) & 3], buf ^ 1);
-                fetch(q + 1 + RW_AHEAD, ring[(E + 1) & 3]);
-                // accumulator row (ph + 1) mod 32 has seen kernel rows 0..31: output row y0 + q - 31
+                store_row(ring[(E + 1) & 3], rok[(E + 1) & 3], buf ^ 1);
+                load_weights(ph + 1);
+                rok[(E + 1) & 3] = fetch(q + 1 + RW_AHEAD, ring[(E + 1) & 3]);
+                // accumulator row (ph + 1) mod 32 has seen kernel rows 0..31:
+                // output row y0 + q - 31, held by lane group gf (16 lanes x 8
+                // columns); they park it, then every lane converts two columns
                 const int ef = (E + 1) & 3;
                 const int Mf = (ph + 1) & 31, gf = (Mf & 15) >> 2;
                 const bool upper = (Mf >> 4) != 0, mine = g == gf;
-                const int y = y0 + q - 31;
                 float v[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
@@ -441,15 +505,22 @@ __global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
                     acc[t][0][ef] = (mine && !upper) ? 0.f : v0;
                     acc[t][1][ef] = (mine && upper) ? 0.f : v1;
                 }
-                if (mine && y >= y0) {
-                    const int x = x0 + 8 * n;
+                if (q >= 31) {
+                    if (mine) {
+                        float4* pk = reinterpret_cast<float4*>(&park[wave][8 * n]);
+                        pk[0] = make_float4(v[0], v[1], v[2], v[3]);
+                        pk[1] = make_float4(v[4], v[5], v[6], v[7]);
+                    }
+                    wave_sync();
+                    const float2 pv = *reinterpret_cast<const float2*>(&park[wave][2 * lane]);
+                    const int y = y0 + q - 31, x = x0 + 2 * lane;
                     switch (a.dst.dt) {
-                        case IPFG_U8: rw_store8<uint8_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
-                        case IPFG_U16: rw_store8<uint16_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
-                        case IPFG_I16: rw_store8<int16_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
-                        case IPFG_I32: rw_store8<int32_t>(a.dst, y, x, ch, v, unscale, a.delta); break;
-                        case IPFG_F32: rw_store8<float>(a.dst, y, x, ch, v, unscale, a.delta); break;
-                        default: rw_store8<double>(a.dst, y, x, ch, v, unscale, a.delta); break;
+                        case IPFG_U8: rw_store2<uint8_t>(a.dst, y, x, ch, pv.x, pv.y, unscale, a.delta); break;
+                        case IPFG_U16: rw_store2<uint16_t>(a.dst, y, x, ch, pv.x, pv.y, unscale, a.delta); break;
+                        case IPFG_I16: rw_store2<int16_t>(a.dst, y, x, ch, pv.x, pv.y, unscale, a.delta); break;
+                        case IPFG_I32: rw_store2<int32_t>(a.dst, y, x, ch, pv.x, pv.y, unscale, a.delta); break;
+                        case IPFG_F32: rw_store2<float>(a.dst, y, x, ch, pv.x, pv.y, unscale, a.delta); break;
+                        default: rw_store2<double>(a.dst, y, x, ch, pv.x, pv.y, unscale, a.delta); break;
                     }
                 }
                 wave_sync();
@@ -458,14 +529,28 @@ __global__ __launch_bounds__(kBlock) void f2d_mfma_rows_kernel(RwArgs a) {
     }
 }
 
+// Compute units of the current device (cached per device; 256 if the query fails).
+int device_cus() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
 template <class Ti, bool BF, int WP, bool XS>
 hipError_t rw_launch(const RwArgs& a0, hipStream_t s) {
     RwArgs a = a0;
     const int strips = (a.src.w + RW_TX - 1) / RW_TX;
     const int gx = (strips + 3) / 4;
-    // about two waves per SIMD on 256 CUs, strips at least 32 rows tall
-    long long nby = 2048 / std::max(1, strips);
-    int sh = static_cast<int>((a.src.h + std::max(1LL, nby) - 1) / std::max(1LL, nby));
+    // one round of workgroups: rw_occupancy per CU (one wave per SIMD each);
+    // strips at least 32 rows tall (each strip spends 31 steps warming up)
+    const long long nby = std::max(1LL, static_cast<long long>(device_cus()) * rw_occupancy<Ti, BF, XS>() / gx);
+    int sh = static_cast<int>((a.src.h + nby - 1) / nby);
     sh = std::max(sh, 32);
     a.sh = sh;
     const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>((a.src.h + sh - 1) / sh),
@@ -512,9 +597,8 @@ void split_parts(double w, bool bf, int parts, uint16_t* bits) {
 }  // namespace
 
 // Weight fragments in register order.
-// Rows-on-M kernels (kw <= 32, kh <= 31): for rotation r, half mh, part p,
-// lane l, element e: accumulator row M = 16 mh + l % 16, kernel row
-// j = (r - M) mod 32, tap k = 8 (l / 16) + e.
+// Rows-on-M kernels (kw <= 32, kh <= 31): the scaled weights as a table
+// [part][32 kernel rows][RW_WK], zero outside the kernel.
 // Otherwise: for part p, pair s, lane l, element e: chunk q = 2 s + (l >= 32),
 // row j = q / NC, chunk column cc = q % NC, k = 8 ((l % 32) / 16) + e,
 // n = l % 16, tap i = 16 cc + k - n.
@@ -532,17 +616,12 @@ void f2d_mfma_fragments(const double* kd, int kw, int kh, int precision, std::ve
     }
     uint16_t bits[3];
     if (rows_fit(kw, kh)) {
-        out.assign(static_cast<size_t>(32) * 2 * parts * 64 * 8, 0);
-        for (int r = 0; r < 32; ++r)
-            for (int mh = 0; mh < 2; ++mh)
-                for (int l = 0; l < 64; ++l)
-                    for (int e = 0; e < 8; ++e) {
-                        const int M = 16 * mh + l % 16, j = ((r - M) % 32 + 32) % 32, k = 8 * (l / 16) + e;
-                        const double w = (j < kh && k < kw) ? kd[static_cast<size_t>(j) * kw + k] * wscale : 0.0;
-                        split_parts(w, bf, parts, bits);
-                        for (int p = 0; p < parts; ++p)
-                            out[((((static_cast<size_t>(r) * 2 + mh) * parts + p) * 64 + l) * 8) + e] = bits[p];
-                    }
+        out.assign(static_cast<size_t>(parts) * 32 * RW_WK, 0);
+        for (int j = 0; j < kh; ++j)
+            for (int k = 0; k < kw; ++k) {
+                split_parts(kd[static_cast<size_t>(j) * kw + k] * wscale, bf, parts, bits);
+                for (int p = 0; p < parts; ++p) out[(static_cast<size_t>(p) * 32 + j) * RW_WK + k] = bits[p];
+            }
         return;
     }
     const int NC = mf_nc(kw), pairs = f2d_mfma_pairs(kw, kh);
@@ -566,7 +645,7 @@ hipError_t launch_filter2d_mfma(const F2dLaunch& l, hipStream_t s) {
         RwArgs r{};
         r.src = l.src;
         r.dst = l.dst;
-        r.rot = static_cast<const uint4*>(l.mfma_w);
+        r.wt = static_cast<const uint4*>(l.mfma_w);
         r.kw = l.kw;
         r.kh = l.kh;
         r.ax = l.ax;
