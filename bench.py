@@ -1,6 +1,6 @@
 """Benchmark of the MI355X filtering path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|deriv|pyr]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|deriv|pyr|features]
                     [--no-cpu] [--no-extra] [--cpu-budget S]
 
 The default run (c2) prints the c2 headline line with the other
@@ -26,6 +26,10 @@ Cache):
   pyr           3840x2160x3 uint8 polygons + noise (the c3 image): pyrDown and
                 pyrUp (exact FP64 in the reference's operation order); pixels
                 are counted at the input.
+  features      1920x1080x1 uint8 disc scene: canny, cannyAuto, cornerHarris,
+                cornerMinEigenVal, integralImage, houghCircles; filter2DFFT
+                31x31 on its float32 version. Bytes count the input and the
+                output image (houghCircles: the input only).
 All calls use the default precision: FP64 accumulation, certified bit-identical
 to the reference (median is compare-only).
 
@@ -270,7 +274,53 @@ def _pyr():
             "workload": "pyrDown + pyrUp, 3840x2160x3 uint8 polygons + noise (pixels counted at the input)"}
 
 
-WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None, "deriv": _deriv, "pyr": _pyr}
+def _features():
+    from imageprocessingflex_amd import workloads as wl
+    gray = wl.circles_scene(1080, 1920, 24, 7, 3.0)
+    grayf = (gray.astype(np.float32) / np.float32(255)).astype(np.float32)
+    k, _ = wl.c4_kernel(31)
+    kp = np.ascontiguousarray(k, np.float32)
+    kptr = kp.ctypes.data_as(C.POINTER(C.c_float))
+    circles = np.zeros((256, 4), np.float64)
+    count = C.c_int(0)
+    u8, f32, f64 = np.uint8, np.float32, np.float64
+
+    def hough(lib, s, d, st):  # returns after the circles are in host memory
+        return lib.ipfg_hough_circles(s, 20, 120, 100.0, 30, 10.0, 0.6, circles.ctypes.data_as(C.POINTER(C.c_double)),
+                                      256, C.byref(count), st)
+
+    calls = [
+        Call("canny", "canny", "gray", u8, lambda lib, s, d, st: lib.ipfg_canny(s, d, 50.0, 150.0, 3, 0, st),
+             lambda o, im: o.canny(im, 50.0, 150.0), "canny_nms_kernel + union-find hysteresis"),
+        Call("canny_auto", "canny_auto", "gray", u8, lambda lib, s, d, st: lib.ipfg_canny_auto(s, d, 1.4, st),
+             lambda o, im: o.canny_auto(im, 1.4), "gaussian + sobel + select_* (70th percentile) + canny"),
+        Call("corner_harris", "corner_harris", "gray", f32,
+             lambda lib, s, d, st: lib.ipfg_corner_harris(s, d, 3, 3, 0.04, st),
+             lambda o, im: o.corner_harris(im, 3, 3, 0.04), "sobel + tensor_rows_kernel + tensor_cols_kernel"),
+        Call("corner_min_eigen", "corner_min_eigen", "gray", f32,
+             lambda lib, s, d, st: lib.ipfg_corner_min_eigen_val(s, d, 3, 3, st),
+             lambda o, im: o.corner_min_eigen(im, 3, 3), "sobel + tensor_rows_kernel + tensor_cols_kernel"),
+        Call("integral", "integral", "gray", f64, lambda lib, s, d, st: lib.ipfg_integral_image(s, d, 0, st),
+             lambda o, im: o.integral_image(im), "integral_rows_kernel + integral_cols_kernel",
+             out_shape=lambda h, w, c: (h + 1, w + 1)),
+        Call("filter2d_fft31", "filter2d_fft31", "grayf", f32,
+             lambda lib, s, d, st: lib.ipfg_filter2d_fft(s, d, kptr, 31, 31, 3, 0.0, 0.0, st),
+             lambda o, im: o.filter2d_fft(im, kp, border=3), "fft_stages_kernel (radix-2, bit-exact)",
+             flops=2.0 * 1080 * 1920 * 31 * 31),
+        Call("hough_circles", "hough_circles", "gray", u8, hough,
+             lambda o, im: o.hough_circles(im, 20, 120, 100.0, 30, 10.0, 0.6),
+             "canny + hough votes / peaks / refine (synchronous: circles copied to the host)",
+             out_shape=lambda h, w, c: (1, 1)),
+    ]
+    return {"images": {"gray": gray, "grayf": grayf}, "calls": calls, "dtype": "f64",
+            "workload": "canny(50, 150), cannyAuto(1.4), cornerHarris(3, 3, 0.04), cornerMinEigenVal(3, 3), "
+                        "integralImage, houghCircles(20..120) on a 1920x1080x1 uint8 disc scene; filter2DFFT "
+                        "31x31 on its float32 version",
+            "keep": (kp, circles, count)}
+
+
+WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": None, "deriv": _deriv, "pyr": _pyr,
+             "features": _features}
 
 C5_BUDGET = 256 << 20  # device-memory budget of the batched server (>= the largest request, 2 x 67 MB)
 C5_MAX_BATCH = 256
