@@ -297,46 +297,86 @@ __global__ __launch_bounds__(kBlock) void select_pick_kernel(unsigned int* __res
 // static_cast<float>(sum * 1.0). The passes below keep that order exactly.
 // rows: 3 planes of H x W doubles (xx, xy, yy horizontal sums of image row y;
 // padded rows reuse the row they reflect).
+//
+// Rows: one wave per image row. Lanes load 64 consecutive window positions
+// (coalesced) and form the entering-minus-leaving products in parallel; the
+// running sums then pass through the lanes in order (lane i's term broadcast
+// with v_readlane, added, kept by lane i), the reference's sequential
+// additions exactly. Columns: one thread per column, a dependent chain whose
+// loads two register buffers of kTensorChunk rows keep in flight.
+constexpr int kTensorChunk = 8;
+
 __global__ __launch_bounds__(kBlock) void tensor_rows_kernel(DevImage dx, DevImage dy, int kw,
                                                              double* __restrict__ rows) {
-    const int y = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int y = blockIdx.x * (blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = dx.w, H = dx.h;
-    if (y >= H) return;
-    const float* a = frow(dx, y);
-    const float* b = frow(dy, y);
+    if (y >= H) return;  // whole waves
+    const float* __restrict__ a = frow(dx, y);
+    const float* __restrict__ b = frow(dy, y);
     const size_t plane = static_cast<size_t>(W) * H;
-    double* rxx = rows + static_cast<size_t>(y) * W;
-    double* rxy = rxx + plane;
-    double* ryy = rxy + plane;
+    double* __restrict__ rxx = rows + static_cast<size_t>(y) * W;
+    double* __restrict__ rxy = rxx + plane;
+    double* __restrict__ ryy = rxy + plane;
     const int ax = kw / 2;
-    auto at = [&](int px, double& xx, double& xy, double& yy) {
-        const int sx = border_index(px - ax, W, IPFG_BORDER_REFLECT101);
-        const float u = a[sx], v = b[sx];
+    auto src_x = [&](int px) { return border_index(px - ax, W, IPFG_BORDER_REFLECT101); };
+    auto prod = [](float u, float v, double& xx, double& xy, double& yy) {
         xx = static_cast<double>(u * u);
         xy = static_cast<double>(u * v);
         yy = static_cast<double>(v * v);
     };
+    // the first window, summed in order (every lane holds the sums)
     double sxx = 0.0, sxy = 0.0, syy = 0.0;
     for (int i = 0; i < kw; ++i) {
+        const int sx = src_x(i);
         double p, q, r;
-        at(i, p, q, r);
+        prod(a[sx], b[sx], p, q, r);
         sxx += p;
         sxy += q;
         syy += r;
     }
-    rxx[0] = sxx;
-    rxy[0] = sxy;
-    ryy[0] = syy;
-    for (int x = 1; x < W; ++x) {
+    if (lane == 0) {
+        rxx[0] = sxx;
+        rxy[0] = sxy;
+        ryy[0] = syy;
+    }
+    // step x (1 <= x < W) adds window position x + kw - 1 and drops x - 1;
+    // the next chunk's samples are loaded before this chunk's chain runs
+    float ui, vi, uo, vo;
+    auto load = [&](int x0) {
+        const int x = min(x0 + lane, W - 1);  // lanes past the row load a valid sample (unused)
+        const int si = src_x(x + kw - 1), so = src_x(x - 1);
+        ui = a[si];
+        vi = b[si];
+        uo = a[so];
+        vo = b[so];
+    };
+    load(1);
+    for (int x0 = 1; x0 < W; x0 += 64) {
         double p1, q1, r1, p0, q0, r0;
-        at(x + kw - 1, p1, q1, r1);
-        at(x - 1, p0, q0, r0);
-        sxx += p1 - p0;
-        sxy += q1 - q0;
-        syy += r1 - r0;
-        rxx[x] = sxx;
-        rxy[x] = sxy;
-        ryy[x] = syy;
+        prod(ui, vi, p1, q1, r1);
+        prod(uo, vo, p0, q0, r0);
+        const double dxx = p1 - p0, dxy = q1 - q0, dyy = r1 - r0;
+        if (x0 + 64 < W) load(x0 + 64);
+        const int n = min(64, W - x0);
+        double oxx = 0.0, oxy = 0.0, oyy = 0.0;
+#pragma unroll 8
+        for (int i = 0; i < n; ++i) {
+            sxx += bcast_d(dxx, i);
+            sxy += bcast_d(dxy, i);
+            syy += bcast_d(dyy, i);
+            if (lane == i) {
+                oxx = sxx;
+                oxy = sxy;
+                oyy = syy;
+            }
+        }
+        const int x = x0 + lane;
+        if (x < W) {
+            rxx[x] = oxx;
+            rxy[x] = oxy;
+            ryy[x] = oyy;
+        }
     }
 }
 
@@ -344,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void tensor_rows_kernel(DevImage dx, DevIma
 // minimum eigenvalue of [[A/2, B], [B, C/2]] form (141-153).
 __global__ __launch_bounds__(kBlock) void tensor_cols_kernel(const double* __restrict__ rows, int W, int H, int kh,
                                                              int harris, double k, DevImage dst) {
-    const int x = blockIdx.x * kBlock + threadIdx.x;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= W) return;
     const size_t plane = static_cast<size_t>(W) * H;
     const int ay = kh / 2;
@@ -356,25 +396,56 @@ __global__ __launch_bounds__(kBlock) void tensor_cols_kernel(const double* __res
         cb += r[plane];
         cc += r[2 * plane];
     }
-    for (int y = 0; y < H; ++y) {
-        const float fa = static_cast<float>(ca * 1.0), fb = static_cast<float>(cb * 1.0),
-                    fc = static_cast<float>(cc * 1.0);
-        float out;
-        if (harris) {
-            const double A = fa, B = fb, C = fc;
-            out = static_cast<float>(A * C - B * B - k * (A + C) * (A + C));
-        } else {
-            const double A = fa * 0.5, B = fb, C = fc * 0.5;
-            out = static_cast<float>((A + C) - __builtin_sqrt((A - C) * (A - C) + B * B));
-        }
-        reinterpret_cast<float*>(static_cast<char*>(dst.data) + static_cast<size_t>(y) * dst.stride)[x] = out;
-        if (y + 1 < H) {
+    // after output row y the window adds row y + kh and drops row y
+    struct Chunk {
+        double ia[kTensorChunk], ib[kTensorChunk], ic[kTensorChunk];
+        double oa[kTensorChunk], ob[kTensorChunk], oc[kTensorChunk];
+    };
+    auto load = [&](Chunk& c, int y0) {  // steps past the last row load valid rows (unused)
+#pragma unroll
+        for (int j = 0; j < kTensorChunk; ++j) {
+            const int y = min(y0 + j, H - 1);
             const double* in = row(y + kh);
-            const double* outr = row(y);
-            ca += in[0] - outr[0];
-            cb += in[plane] - outr[plane];
-            cc += in[2 * plane] - outr[2 * plane];
+            const double* out = row(y);
+            c.ia[j] = in[0];
+            c.ib[j] = in[plane];
+            c.ic[j] = in[2 * plane];
+            c.oa[j] = out[0];
+            c.ob[j] = out[plane];
+            c.oc[j] = out[2 * plane];
         }
+    };
+    auto run = [&](const Chunk& c, int y0) {
+#pragma unroll
+        for (int j = 0; j < kTensorChunk; ++j) {
+            const int y = y0 + j;
+            if (y < H) {
+                const float fa = static_cast<float>(ca * 1.0), fb = static_cast<float>(cb * 1.0),
+                            fc = static_cast<float>(cc * 1.0);
+                float out;
+                if (harris) {
+                    const double A = fa, B = fb, C = fc;
+                    out = static_cast<float>(A * C - B * B - k * (A + C) * (A + C));
+                } else {
+                    const double A = fa * 0.5, B = fb, C = fc * 0.5;
+                    out = static_cast<float>((A + C) - __builtin_sqrt((A - C) * (A - C) + B * B));
+                }
+                reinterpret_cast<float*>(static_cast<char*>(dst.data) + static_cast<size_t>(y) * dst.stride)[x] = out;
+                if (y + 1 < H) {
+                    ca += c.ia[j] - c.oa[j];
+                    cb += c.ib[j] - c.ob[j];
+                    cc += c.ic[j] - c.oc[j];
+                }
+            }
+        }
+    };
+    Chunk qa, qb;
+    load(qa, 0);
+    for (int y0 = 0; y0 < H; y0 += 2 * kTensorChunk) {
+        load(qb, y0 + kTensorChunk);
+        run(qa, y0);
+        load(qa, y0 + 2 * kTensorChunk);
+        run(qb, y0 + kTensorChunk);
     }
 }
 
@@ -439,10 +510,11 @@ hipError_t launch_structure_response(const DevImage& dx, const DevImage& dy, int
                                      double* rows, const DevImage& dst, hipStream_t s) {
     const int W = dx.w, H = dx.h;
     if (static_cast<long long>(W) * H == 0) return hipSuccess;
-    hipLaunchKernelGGL(tensor_rows_kernel, dim3(static_cast<unsigned>((H + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       dx, dy, block, rows);
-    hipLaunchKernelGGL(tensor_cols_kernel, dim3(static_cast<unsigned>((W + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       rows, W, H, block, harris ? 1 : 0, k, dst);
+    // one image row per one-wave workgroup (spread evenly over the CUs)
+    hipLaunchKernelGGL(tensor_rows_kernel, dim3(static_cast<unsigned>(H)), dim3(64), 0, s, dx, dy, block, rows);
+    // one wave per workgroup: the few column waves spread over as many CUs
+    hipLaunchKernelGGL(tensor_cols_kernel, dim3(static_cast<unsigned>((W + 63) / 64)), dim3(64), 0, s, rows, W, H, block,
+                       harris ? 1 : 0, k, dst);
     return hipGetLastError();
 }
 
