@@ -128,10 +128,11 @@ std::vector<int> sep_sh_candidates(const SepLaunch& l) {
     const int h = l.src.h;
     std::vector<int> c;
     if (l.mode == MODE_CERTIFY && sep1_fits(l.src, l.dst, l.kw, l.kh, l.ax, l.ay)) {
-        // the one-channel float32 kernel: 16-row steps, then 8-row steps
-        for (int v : {128, 256, 512})
+        // the one-channel float32 kernel: 16-row steps, then 8-row steps;
+        // short strips give small images enough workgroups to fill the CUs
+        for (int v : {16, 32, 64, 128, 256, 512})
             if (v < 2 * h) c.push_back(v);
-        for (int v : {64, 128, 256})
+        for (int v : {8, 16, 32, 64, 128, 256})
             if (v < 2 * h) c.push_back(sep1::kHalfStep + v);
         return c;
     }
