@@ -14,6 +14,7 @@
 #include <list>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -132,6 +133,21 @@ void ensure_init() {
     if (cur != g_device.load()) HIPCALL(hipSetDevice(g_device.load()));
 }
 
+// A stream the library owns and never destroys. The runtime keeps a pointer
+// to the stream an event was last recorded on and consults that stream when
+// the event is synchronised or queried, so a cached event (pinned ring slot,
+// parameter or scratch block, batch buffer) must not outlive the stream it
+// was recorded on: when a stream is destroyed through the library (its work
+// has completed by then) those events are re-recorded here.
+hipStream_t home_stream() {
+    static hipStream_t h = [] {
+        hipStream_t st = nullptr;
+        HIPCALL(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        return st;
+    }();
+    return h;
+}
+
 // Per-thread default stream (default priority) and a pinned slot receiving the
 // fix-up count of the last certified call.
 struct ThreadCtx {
@@ -183,6 +199,7 @@ public:
             events_.resize(kSlots, nullptr);
             for (auto& e : events_) HIPCALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             armed_.assign(kSlots, 0);
+            rec_.assign(kSlots, nullptr);
         }
         const int slot = next_;
         next_ = (next_ + 1) % kSlots;
@@ -192,7 +209,18 @@ public:
         HIPCALL(hipMemcpyAsync(dev, h, bytes, hipMemcpyHostToDevice, s));
         HIPCALL(hipEventRecord(events_[slot], s));
         armed_[slot] = 1;
+        rec_[slot] = s;
         return true;
+    }
+
+    // s has completed its work and is being destroyed: its slots are free.
+    void forget(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (size_t i = 0; i < rec_.size(); ++i)
+            if (rec_[i] == s) {
+                armed_[i] = 0;
+                rec_[i] = nullptr;
+            }
     }
 
 private:
@@ -200,6 +228,7 @@ private:
     char* base_ = nullptr;
     std::vector<hipEvent_t> events_;
     std::vector<char> armed_;
+    std::vector<hipStream_t> rec_;  // stream of each slot's last copy
     int next_ = 0;
 };
 PinnedRing g_ring;
@@ -217,6 +246,7 @@ public:
         size_t bytes = 0;
         hipEvent_t ready = nullptr;
         hipEvent_t last_use = nullptr;
+        hipStream_t ready_on = nullptr, used_on = nullptr;  // streams the two events were recorded on
         int users = 0;
         bool used = false;
         std::list<const std::string*>::iterator pos;
@@ -243,6 +273,7 @@ public:
         if (e.bytes && !g_ring.copy(e.dev, key.data(), e.bytes, s))
             HIPCALL(hipMemcpyAsync(e.dev, key.data(), e.bytes, hipMemcpyHostToDevice, s));
         HIPCALL(hipEventRecord(e.ready, s));
+        e.ready_on = s;
         e.users = 1;
         auto ins = map_.emplace(key, e).first;
         lru_.push_front(&ins->first);
@@ -255,8 +286,25 @@ public:
     void release(Entry* e, hipStream_t s) {
         std::lock_guard<std::mutex> lock(mu_);
         (void)hipEventRecord(e->last_use, s);
+        e->used_on = s;
         e->used = true;
         --e->users;
+    }
+
+    // s has completed its work and is being destroyed.
+    void forget(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (auto& kv : map_) {
+            Entry& e = kv.second;
+            if (e.ready_on == s) {
+                HIPCALL(hipEventRecord(e.ready, home_stream()));
+                e.ready_on = home_stream();
+            }
+            if (e.used_on == s) {
+                HIPCALL(hipEventRecord(e.last_use, home_stream()));
+                e.used_on = home_stream();
+            }
+        }
     }
 
 private:
@@ -342,6 +390,7 @@ public:
         void* ptr;
         size_t size;
         hipEvent_t ev;
+        hipStream_t on = nullptr;  // stream ev was last recorded on
     };
     static constexpr size_t kGranule = size_t(2) << 20;
     static constexpr size_t kKeepBytes = size_t(8) << 30;  // cached beyond this is freed
@@ -359,7 +408,7 @@ public:
                 return b;
             }
         }
-        Block b{nullptr, need, nullptr};
+        Block b{nullptr, need, nullptr, nullptr};
         HIPCALL(hipMalloc(&b.ptr, need));
         HIPCALL(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
         return b;
@@ -375,7 +424,9 @@ public:
         std::vector<Block> drop;
         {
             std::lock_guard<std::mutex> lock(mu_);
-            free_.emplace(b.size, b);
+            Block kept = b;
+            kept.on = s;
+            free_.emplace(kept.size, kept);
             cached_ += b.size;
             while (cached_ > kKeepBytes && !free_.empty()) {
                 auto big = std::prev(free_.end());
@@ -385,6 +436,16 @@ public:
             }
         }
         for (const Block& d : drop) release(d);
+    }
+
+    // s has completed its work and is being destroyed.
+    void forget(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (auto& kv : free_)
+            if (kv.second.on == s) {
+                HIPCALL(hipEventRecord(kv.second.ev, home_stream()));
+                kv.second.on = home_stream();
+            }
     }
 
     void clear() {
@@ -2348,6 +2409,11 @@ void build_job(const ipfg_job& j, BatchJob& out, JobParams& p) {
 // Runs batches on one stream: the job table goes from a pinned host buffer
 // (two, alternating) to a fixed device arena, then one persistent launch per
 // class; with use_graph the copy and the launches are a captured hipGraph.
+class BatchExecutor;
+// Live executors (their buffer events may be recorded on caller streams).
+std::mutex g_exec_mu;
+std::set<BatchExecutor*> g_execs;
+
 class BatchExecutor {
 public:
     BatchExecutor(int max_jobs, bool use_graph) : max_jobs_(max_jobs), use_graph_(use_graph) {
@@ -2373,8 +2439,14 @@ public:
             lds_[c] = batch_lds_bytes(c);
             grid_[c] = batch_grid(c, lds_[c]);
         }
+        std::lock_guard<std::mutex> lock(g_exec_mu);
+        g_execs.insert(this);
     }
     ~BatchExecutor() {
+        {
+            std::lock_guard<std::mutex> lock(g_exec_mu);
+            g_execs.erase(this);
+        }
         for (int b = 0; b < 2; ++b) {
             if (armed_[b]) (void)hipEventSynchronize(free_[b]);
             if (exec_[b]) (void)hipGraphExecDestroy(exec_[b]);
@@ -2449,14 +2521,31 @@ public:
             enqueue(b, s);
         }
         next_ ^= 1;
+        std::lock_guard<std::mutex> lock(ev_mu_);
         HIPCALL(hipEventRecord(free_[b], s));
         armed_[b] = true;
+        free_on_[b] = s;
         HIPCALL(hipEventRecord(last_done_, s));
         last_stream_ = s;
         last_armed_ = true;
         held.armed = false;
         for (auto* e : held.v) g_params.release(e, s);
         held_[b] = std::move(held.v);
+    }
+
+    // s has completed its work and is being destroyed: events recorded on it
+    // move to the library's own stream.
+    void forget(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(ev_mu_);
+        for (int b = 0; b < 2; ++b)
+            if (armed_[b] && free_on_[b] == s) {
+                HIPCALL(hipEventRecord(free_[b], home_stream()));
+                free_on_[b] = home_stream();
+            }
+        if (last_armed_ && last_stream_ == s) {
+            HIPCALL(hipEventRecord(last_done_, home_stream()));
+            last_stream_ = home_stream();
+        }
     }
 
 private:
@@ -2503,6 +2592,8 @@ private:
     void* host_arena_[2] = {nullptr, nullptr};
     hipEvent_t free_[2] = {nullptr, nullptr};
     bool armed_[2] = {false, false};
+    hipStream_t free_on_[2] = {nullptr, nullptr};  // streams free_ was recorded on
+    std::mutex ev_mu_;                              // free_ / last_done_ bookkeeping vs forget()
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
     std::vector<ParamCache::Entry*> held_[2];
     int next_ = 0;
@@ -2914,9 +3005,18 @@ int ipfg_stream_destroy(ipfg_stream stream) {
     return guarded([&] {
         ensure_init();
         if (!stream) return;
-        HIPCALL(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-        forget_stream(static_cast<hipStream_t>(stream));
-        HIPCALL(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        HIPCALL(hipStreamSynchronize(st));
+        forget_stream(st);
+        // cached events last recorded on st move to the library's own stream
+        g_ring.forget(st);
+        g_params.forget(st);
+        g_scratch.forget(st);
+        {
+            std::lock_guard<std::mutex> lock(g_exec_mu);
+            for (BatchExecutor* x : g_execs) x->forget(st);
+        }
+        HIPCALL(hipStreamDestroy(st));
     });
 }
 int ipfg_stream_synchronize(ipfg_stream stream) {

@@ -6,6 +6,7 @@ reference; float outputs of the FP32 path stay within the bound computed by
 tests.fixtures.float_tolerance (written next to the formula)."""
 from __future__ import annotations
 
+import gc
 import hashlib
 import json
 import threading
@@ -173,6 +174,33 @@ def test_device_resident_async_and_in_place(ipf, oracle):
                                            s.handle))
     s.synchronize()
     assert np.array_equal(d.download(s), oracle.sep_filter(img, kx, kx))
+
+
+def test_cached_events_outlive_destroyed_streams(ipf, oracle):
+    # Parameter uploads (pinned ring slots), parameter-cache and scratch
+    # events recorded on a stream that is then destroyed are recycled by
+    # later calls on other streams: the ring wraps (64 slots) over slots last
+    # used by the destroyed stream, and the results stay the reference's.
+    rng = np.random.default_rng(14)
+    img = rng.integers(0, 256, (48, 72, 3), dtype=np.uint8)
+    sigmas = [0.5 + 0.013 * i for i in range(80)]  # distinct weights: one upload each
+    s = ipf.Stream()
+    d = ipf.DeviceImage.from_array(img, s)
+    for sig in sigmas[:70]:
+        ipf.gaussian_blur(d, sig, stream=s)
+    s.synchronize()
+    del d, s
+    gc.collect()
+    for _ in range(16):  # reuse the destroyed stream's memory
+        t = ipf.Stream()
+        del t
+    gc.collect()
+    s2 = ipf.Stream()
+    d2 = ipf.DeviceImage.from_array(img, s2)
+    outs = [(sig, ipf.gaussian_blur(d2, sig + 3.0, stream=s2)) for sig in sigmas]
+    s2.synchronize()
+    for sig, o in outs[::9]:
+        assert np.array_equal(o.download(s2), oracle.gaussian_blur(img, sig + 3.0)), sig
 
 
 def test_concurrent_host_threads(ipf, oracle):
