@@ -7,7 +7,8 @@
 // count, kernel size, arithmetic and on how many workgroups fit on a CU, so
 // it is measured: the first call of each shape class with at least
 // kMinTunedSamples samples times every candidate on the caller's stream (HIP
-// events, two launches each, the second one counted) and keeps the fastest.
+// events: a warm-up launch, then kReps launches back to back) and keeps the
+// fastest.
 // Each candidate launch computes the complete, identical output, so the call
 // returns the right result whichever launch ran last. The table can be
 // exported and imported as text, one entry per line:
@@ -48,6 +49,7 @@ inline int ceil_log2(long long v) {
 class Tuner {
 public:
     static constexpr long long kMinTunedSamples = 1LL << 20;
+    static constexpr int kReps = 3;  // timed launches per candidate
 
     Tuner() {
         const char* e = std::getenv("IPFG_AUTOTUNE");
@@ -84,9 +86,11 @@ public:
         float best_ms = 0.f;
         for (int v : candidates) {
             if (launch(v) != hipSuccess) continue;  // warm-up (and validity) launch
-            if (hipEventRecord(e0, s) != hipSuccess || launch(v) != hipSuccess ||
-                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
-                continue;
+            // kReps launches back to back: one launch's time is too noisy to
+            // rank a dozen candidates
+            bool ok = hipEventRecord(e0, s) == hipSuccess;
+            for (int r = 0; ok && r < kReps; ++r) ok = launch(v) == hipSuccess;
+            if (!ok || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) continue;
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
             if (best == 0 || ms < best_ms) {
