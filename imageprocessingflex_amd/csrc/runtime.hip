@@ -3,6 +3,7 @@
 // parameter upload, precision selection and kernel dispatch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cctype>
 #include <chrono>
@@ -1969,24 +1970,47 @@ void hough_impl(const ipfg_image* src, int min_r, int max_r, double canny_high, 
         HIPCALL(hipStreamSynchronize(s));
     }
     // findPeaks (features.hpp:174-187): strongest first (ties in raster
-    // order), at least minDist apart
+    // order: the candidates arrive in raster order), each kept unless a kept
+    // peak lies closer than minDist. Kept peaks are binned in square cells of
+    // side ceil(minDist), so a candidate tests only the kept peaks of its own
+    // and the eight neighbouring cells -- the same test, against every peak
+    // that can be that close.
     std::vector<unsigned> order(ncand);
     for (unsigned i = 0; i < ncand; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned b) { return cv[a] > cv[b]; });
+    std::sort(order.begin(), order.end(),
+              [&](unsigned a, unsigned b) { return cv[a] > cv[b] || (cv[a] == cv[b] && a < b); });
     const double md2 = min_dist * min_dist;
     std::vector<int> peaks;
-    for (const unsigned c : order) {
-        bool ok = true;
-        for (size_t q = 0; q < peaks.size(); q += 2) {
-            const double ddx = cxy[2 * c] - peaks[q], ddy = cxy[2 * c + 1] - peaks[q + 1];
-            if (ddx * ddx + ddy * ddy < md2) {
-                ok = false;
-                break;
-            }
-        }
-        if (ok) {
+    if (!(md2 > 0.0)) {  // nothing is ever closer than minDist
+        for (const unsigned c : order) {
             peaks.push_back(cxy[2 * c]);
             peaks.push_back(cxy[2 * c + 1]);
+        }
+    } else {
+        const long long cell = std::max(1LL, static_cast<long long>(std::ceil(std::min(std::fabs(min_dist), 1e9))));
+        const long long gw = (W + cell - 1) / cell + 1, gh = (H + cell - 1) / cell + 1;
+        std::vector<int> head(static_cast<size_t>(gw * gh), -1), next;
+        for (const unsigned c : order) {
+            const int x = cxy[2 * c], y = cxy[2 * c + 1];
+            const long long gx = x / cell, gy = y / cell;
+            bool ok = true;
+            for (long long j = std::max(0LL, gy - 1); ok && j <= std::min(gh - 1, gy + 1); ++j)
+                for (long long i = std::max(0LL, gx - 1); ok && i <= std::min(gw - 1, gx + 1); ++i)
+                    for (int q = head[static_cast<size_t>(j * gw + i)]; q >= 0; q = next[static_cast<size_t>(q)]) {
+                        const double ddx = x - peaks[2 * static_cast<size_t>(q)];
+                        const double ddy = y - peaks[2 * static_cast<size_t>(q) + 1];
+                        if (ddx * ddx + ddy * ddy < md2) {
+                            ok = false;
+                            break;
+                        }
+                    }
+            if (ok) {
+                const size_t cellid = static_cast<size_t>(gy * gw + gx);
+                next.push_back(head[cellid]);
+                head[cellid] = static_cast<int>(peaks.size() / 2);
+                peaks.push_back(x);
+                peaks.push_back(y);
+            }
         }
     }
     const int np = static_cast<int>(peaks.size() / 2);

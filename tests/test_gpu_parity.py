@@ -1017,7 +1017,8 @@ def test_full_hough_circles_1080p_vs_reference(ipf):
 
 
 @pytest.mark.parametrize("args", [(8, 60, 100.0, 20, 10.0, 0.6), (15, 40, 80.0, 12, 8.0, 0.5),
-                                  (30, 90, 120.0, 40, 20.0, 0.7)], ids=str)
+                                  (30, 90, 120.0, 40, 20.0, 0.7), (8, 60, 100.0, 20, 1.5, 0.6),
+                                  (15, 40, 80.0, 25, 0.0, 0.5)], ids=str)
 def test_hough_circles_vs_oracle(ipf, oracle, args):
     from imageprocessingflex_amd import workloads as wl
     for img in (wl.circles_scene(360, 480, 8, 11, 6.0), wl.circles_scene(301, 257, 5, 12, 14.0)):
