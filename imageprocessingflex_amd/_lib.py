@@ -77,6 +77,7 @@ SIGNATURES = {
     "ipfg_stream_create": (_I, [_P(_V)]),
     "ipfg_stream_destroy": (_I, [_V]),
     "ipfg_stream_synchronize": (_I, [_V]),
+    "ipfg_stream_forget": (_I, [_V]),
     "ipfg_device_synchronize": (_I, []),
     "ipfg_gaussian_kernel_1d": (_I, [_D, _I, _P(_D), _I]),
     "ipfg_sep_filter": (_I, [_IMG, _IMG, _P(_D), _I, _P(_D), _I, _I, _I, _I, _D, _D, _I, _V]),

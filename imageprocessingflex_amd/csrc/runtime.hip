@@ -3025,22 +3025,31 @@ int ipfg_stream_create(ipfg_stream* stream) {
         *stream = s;
     });
 }
+// Waits for st and moves the cached events last recorded on it to the
+// library's own stream (their work is complete once st has drained).
+void detach_stream(hipStream_t st) {
+    HIPCALL(hipStreamSynchronize(st));
+    forget_stream(st);
+    g_ring.forget(st);
+    g_params.forget(st);
+    g_scratch.forget(st);
+    std::lock_guard<std::mutex> lock(g_exec_mu);
+    for (BatchExecutor* x : g_execs) x->forget(st);
+}
+
 int ipfg_stream_destroy(ipfg_stream stream) {
     return guarded([&] {
         ensure_init();
         if (!stream) return;
         const hipStream_t st = static_cast<hipStream_t>(stream);
-        HIPCALL(hipStreamSynchronize(st));
-        forget_stream(st);
-        // cached events last recorded on st move to the library's own stream
-        g_ring.forget(st);
-        g_params.forget(st);
-        g_scratch.forget(st);
-        {
-            std::lock_guard<std::mutex> lock(g_exec_mu);
-            for (BatchExecutor* x : g_execs) x->forget(st);
-        }
+        detach_stream(st);
         HIPCALL(hipStreamDestroy(st));
+    });
+}
+int ipfg_stream_forget(ipfg_stream stream) {
+    return guarded([&] {
+        ensure_init();
+        if (stream) detach_stream(static_cast<hipStream_t>(stream));
     });
 }
 int ipfg_stream_synchronize(ipfg_stream stream) {

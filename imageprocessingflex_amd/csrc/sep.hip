@@ -183,6 +183,11 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     const int cn = l.src.cn;
     const bool planar = sep_planar(cn, l.src.dt, l.dst.dt);
     if (planar && l.src.dt == IPFG_U8) {
+        if (cert && cn == 1 && l.dst.dt == IPFG_F32) {
+            const hipError_t e = plane_u8f_k1(a, s);
+            if (e != hipErrorNotSupported) return e;
+            (void)hipGetLastError();
+        }
         if (cert && plane_ring_is_float(l.dst.dt)) {
             // the ring holds the FP64 horizontal sums rounded to float
             a.bcoef += 1.01 * 5.9604644775390625e-08 * l.kabs;

@@ -866,6 +866,7 @@ hipError_t plane_f32(const SepArgs& a, bool cert, hipStream_t s);
 hipError_t plane_u8_k1(const SepArgs& a, hipStream_t s);
 hipError_t plane_u8_k3(const SepArgs& a, hipStream_t s);
 hipError_t plane_u8_k4(const SepArgs& a, hipStream_t s);
+hipError_t plane_u8f_k1(const SepArgs& a, hipStream_t s);  // certified uint8 -> float32, 1 channel, 3 / 5 / 7 taps
 hipError_t plane_f32_k1(const SepArgs& a, hipStream_t s);
 hipError_t generic_any(const SepArgs& a, bool cert, hipStream_t s);
 

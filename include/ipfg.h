@@ -73,6 +73,13 @@ int ipfg_memset(void* dst, int value, size_t bytes, ipfg_stream stream);
 int ipfg_stream_create(ipfg_stream* stream);
 int ipfg_stream_destroy(ipfg_stream stream);
 int ipfg_stream_synchronize(ipfg_stream stream);
+/* No reference counterpart. Waits for `stream` and detaches the library's
+ * cached state from it (events of parameter uploads, scratch blocks and
+ * batch buffers last recorded on it), so that the caller may then destroy a
+ * stream it created itself; the HIP runtime consults an event's recording
+ * stream when the event is later synchronised. ipfg_stream_destroy does this
+ * for streams from ipfg_stream_create. */
+int ipfg_stream_forget(ipfg_stream stream);
 int ipfg_device_synchronize(void);
 
 /* ---------------------------------------------------------------- filters */
