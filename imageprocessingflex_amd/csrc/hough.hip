@@ -201,7 +201,23 @@ __global__ __launch_bounds__(kBlock) void zero_f32_kernel(float* __restrict__ p,
         p[i] = 0.0f;
 }
 
-// One workgroup per peak (features.hpp:419-488).
+// First index in [0, n) whose point lies on row y or below (the points are
+// in row-major order, so their rows never decrease).
+__device__ unsigned first_row_at(const HoughPoint* __restrict__ pts, unsigned n, double y) {
+    unsigned lo = 0, hi = n;
+    while (lo < hi) {
+        const unsigned mid = lo + (hi - lo) / 2;
+        if (pts[mid].y < y) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One workgroup per peak (features.hpp:419-488). Every pass takes only points
+// within (a radius of) the current centre, so it scans the rows
+// [cy - reach, cy + reach] with reach >= that radius + 3: a contiguous range
+// of the row-major points holding every point the pass can take, in the same
+// order.
 __global__ __launch_bounds__(kBlock) void hough_refine_kernel(const HoughPoint* __restrict__ pts, unsigned npts,
                                                               const float* __restrict__ acc, int w, int h,
                                                               const int* __restrict__ peaks, int min_r, int max_r,
@@ -209,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void hough_refine_kernel(const HoughPoint* 
     extern __shared__ int hist[];  // max_r + 2
     __shared__ double s_cx, s_cy, s_radius;
     __shared__ int s_best, s_stop, s_support;
+    __shared__ unsigned s_lo, s_hi;
     __shared__ unsigned long long s_mask[kBlock / 64];
     __shared__ int s_sector[36];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -230,7 +247,22 @@ __global__ __launch_bounds__(kBlock) void hough_refine_kernel(const HoughPoint* 
     for (int i = tid; i < max_r + 2; i += kBlock) hist[i] = 0;
     __syncthreads();
     double cx = s_cx, cy = s_cy;
-    for (unsigned i = tid; i < npts; i += kBlock) {  // radially aligned support per radius
+    // rows [cy - reach, cy + reach] as a point range [lo, hi); every point if
+    // the centre is not finite (the distance tests then take NaN paths)
+    unsigned lo = 0, hi = npts;
+    auto window = [&](double reach) {
+        if (tid == 0) {
+            const bool finite = isfinite(cy) && isfinite(reach);
+            s_lo = finite ? first_row_at(pts, npts, cy - reach) : 0u;
+            s_hi = finite ? first_row_at(pts, npts, nextafter(cy + reach, INFINITY)) : npts;
+        }
+        __syncthreads();
+        lo = s_lo;
+        hi = s_hi;
+        __syncthreads();  // every thread has its copy before the words are rewritten
+    };
+    window(static_cast<double>(max_r) + 3.0);  // lround(d) <= max_r
+    for (unsigned i = lo + tid; i < hi; i += kBlock) {  // radially aligned support per radius
         double d;
         if (radial(pts[i], cx, cy, &d) < kAlign) continue;
         const long r = lround(d);
@@ -260,10 +292,11 @@ __global__ __launch_bounds__(kBlock) void hough_refine_kernel(const HoughPoint* 
     for (int iter = 0; iter < 2; ++iter) {
         const double radius = s_radius;
         double sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0, sz = 0, sxz = 0, syz = 0, cnt = 0;
-        for (unsigned base = 0; base < npts; base += kBlock) {
+        window(radius + 3.0);  // |d - radius| <= 2
+        for (unsigned base = lo; base < hi; base += kBlock) {
             const unsigned i = base + tid;
             bool take = false;
-            if (i < npts) {
+            if (i < hi) {
                 double d;
                 take = !(radial(pts[i], cx, cy, &d) < kAlign) && !(fabs(d - radius) > 2.0);
             }
@@ -338,7 +371,8 @@ __global__ __launch_bounds__(kBlock) void hough_refine_kernel(const HoughPoint* 
     if (tid == 0) s_support = 0;
     __syncthreads();
     int mine = 0;
-    for (unsigned i = tid; i < npts; i += kBlock) {
+    window(radius + 3.0);  // |d - radius| <= 1.5
+    for (unsigned i = lo + tid; i < hi; i += kBlock) {
         const HoughPoint e = pts[i];
         double d;
         if (radial(e, cx, cy, &d) < kAlign || fabs(d - radius) > 1.5) continue;
