@@ -6,11 +6,16 @@
 // the reference's FP64 arithmetic for matrix-core throughput: the result
 // carries a small error against the FP64 value, which the tests measure and
 // bound (max and RMS against the FP64 oracle). Zero weights, and the zero
-// band of the Toeplitz blocks below, take part as zeros: a non-finite input
-// (which the reference would skip under a zero tap) can reach outputs up to
-// 16 columns outside its window. Finite inputs are unaffected.
+// padding of the kernels below (the Toeplitz band; the rows kernel's 32 x 32
+// weight table), take part as zeros: a non-finite input (which the reference
+// would skip under a zero tap) can reach outputs up to 16 columns (Toeplitz)
+// or 32 - kw columns and 32 - kh rows (rows kernel) outside its window.
+// Finite inputs are unaffected.
 //
-// Formulation. For one output row block of 16 rows and 16 columns, kernel row
+// Two formulations: kernels up to 32 x 31 run the rows-on-M kernel (further
+// below), larger ones the Toeplitz blocks described here.
+//
+// Toeplitz blocks. For one output row block of 16 rows and 16 columns, kernel row
 // j contributes a 1-D correlation of input rows with K[j][.]; with the input
 // columns split into chunks of 16, each (row j, chunk c) pair is a
 // 16 x 16 x 16 product In[16 rows][16 cols] * T[16 cols][16 outputs] with the
@@ -273,10 +278,11 @@ hipError_t mf_by_prec(const MfArgs& a, int precision, hipStream_t s) {
 // step: one output row per accumulator row, j advancing with the steps. At
 // j = 31 (a zero kernel row) the row is complete: its 16 x 8 = 128 values
 // (8 consecutive columns per lane) are stored and the row starts over at zero.
-// Every weight fragment of the 32 rotations is laid out once by the host
-// (32 x 2 halves x parts x 1 KB); the data fragments are 16-byte LDS reads
-// from 8 copies of the staged row, copy s shifted left by s, so that every
-// lane's 8 taps start 16-byte aligned. Executed work: 32 x 32 products per
+// The A fragments come from one 32 x 40 table of the scaled weights per part
+// in LDS (a lane reads kernel row (ph - M) mod 32, taps 8 (l / 16) ..+7, one
+// step ahead); the data fragments are 16-byte LDS reads from 8 copies of the
+// staged row, copy s shifted left by s, so that every lane's 8 taps start
+// 16-byte aligned. Executed work: 32 x 32 products per
 // 31 x 31 taps (1.066) times (rows + 31) / rows per strip, times the parts.
 constexpr int RW_TX = 128;               // output columns per wave
 constexpr int RW_SPAN = 160;             // staged samples per row (128 + 31 taps, rounded up)
