@@ -203,6 +203,36 @@ def test_cached_events_outlive_destroyed_streams(ipf, oracle):
         assert np.array_equal(o.download(s2), oracle.gaussian_blur(img, sig + 3.0)), sig
 
 
+def test_stream_forget_before_caller_destroys_its_stream(ipf, oracle):
+    # a stream the caller created itself (hipStreamCreate from the runtime the
+    # library uses) carries calls with fresh parameter uploads;
+    # ipfg_stream_forget detaches the library's cached events from it before
+    # the caller destroys it, and later calls recycle those ring slots
+    import ctypes as C
+    from imageprocessingflex_amd import _lib
+    lib = _lib.load()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    rng = np.random.default_rng(15)
+    img = rng.integers(0, 256, (40, 56), dtype=np.uint8)
+    d = ipf.DeviceImage.from_array(img)
+    o = ipf.DeviceImage.like(d)
+    h = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(h)) == 0
+    sd, od = d.descriptor(), o.descriptor()
+    for i in range(70):
+        _lib.check(lib.ipfg_gaussian_blur(C.byref(sd), C.byref(od), 0.6 + 0.011 * i, 0.0, 0, 3, 0, h))
+    _lib.check(lib.ipfg_stream_forget(h))
+    assert np.array_equal(o.download(), oracle.gaussian_blur(img, 0.6 + 0.011 * 69))
+    assert hip.hipStreamDestroy(h) == 0
+    s2 = ipf.Stream()
+    for i in range(70):
+        got = ipf.gaussian_blur(d, 5.0 + 0.007 * i, stream=s2)
+    s2.synchronize()
+    assert np.array_equal(got.download(s2), oracle.gaussian_blur(img, 5.0 + 0.007 * 69))
+
+
 def test_concurrent_host_threads(ipf, oracle):
     rng = np.random.default_rng(13)
     imgs = [rng.integers(0, 256, (120 + 7 * i, 200 - 5 * i, 1 + i % 4), dtype=np.uint8) for i in range(8)]
