@@ -134,12 +134,14 @@ __device__ __forceinline__ void f2d_row(const float* __restrict__ base, int kw, 
     }
 }
 
-template <class Ti, class Acc, bool HZ, bool CERT, int P>
+// KS > 0: the kernel is KS x KS (the row's tap loop is unrolled and its
+// weights are loaded ahead); 0: sizes at run time.
+template <class Ti, class Acc, bool HZ, bool CERT, int P, int KS>
 __global__ __launch_bounds__(kBlock) void f2d_fast_kernel(F2dArgs a, const Acc* __restrict__ K) {
     extern __shared__ float smem[];
     __shared__ unsigned int tile_max_bits;
     const int W = a.src.w, H = a.src.h, cn = a.src.cn;
-    const int kw = a.kw, kh = a.kh, TH = a.TH;
+    const int kw = KS ? KS : a.kw, kh = KS ? KS : a.kh, TH = a.TH;
     const int XP = f2d_xp(kw);
     const int HR = TH + kh - 1;
     const int plane = HR * XP;
@@ -297,7 +299,7 @@ int grid_for(size_t n) {
     return static_cast<int>(b < 16384 ? (b ? b : 1) : 16384);
 }
 
-template <class Ti, class Acc, bool HZ, bool CERT, int P>
+template <class Ti, class Acc, bool HZ, bool CERT, int P, int KS>
 hipError_t f2d_fast_launch(const F2dLaunch& l, hipStream_t s) {
     const int cn = l.src.cn;
     // rows per tile: halo overhead (TH + kh - 1) / TH <= 1.25 up to 64 rows; LDS within 64 KB
@@ -328,17 +330,21 @@ hipError_t f2d_fast_launch(const F2dLaunch& l, hipStream_t s) {
     a.xbound = l.xbound;
     a.fix = l.fix;
     const dim3 grid(static_cast<unsigned>((l.src.w + TX - 1) / TX), static_cast<unsigned>((l.src.h + TH - 1) / TH));
-    hipLaunchKernelGGL((f2d_fast_kernel<Ti, Acc, HZ, CERT, P>), grid, dim3(kBlock), lds, s, a, static_cast<const Acc*>(a.K));
+    hipLaunchKernelGGL((f2d_fast_kernel<Ti, Acc, HZ, CERT, P, KS>), grid, dim3(kBlock), lds, s, a,
+                       static_cast<const Acc*>(a.K));
     return hipGetLastError();
 }
 
 template <class Ti, int P>
 hipError_t f2d_fast_dispatch_p(const F2dLaunch& a, hipStream_t s) {
-    if (a.mode == MODE_CERTIFY)
-        return a.has_zero ? f2d_fast_launch<Ti, double, true, true, P>(a, s)
-                          : f2d_fast_launch<Ti, double, false, true, P>(a, s);
-    return a.has_zero ? f2d_fast_launch<Ti, float, true, false, P>(a, s)
-                      : f2d_fast_launch<Ti, float, false, false, P>(a, s);
+    if (a.mode == MODE_CERTIFY) {
+        if (P == 8 && !a.has_zero && a.kw == 31 && a.kh == 31)  // dense 31 x 31
+            return f2d_fast_launch<Ti, double, false, true, P, 31>(a, s);
+        return a.has_zero ? f2d_fast_launch<Ti, double, true, true, P, 0>(a, s)
+                          : f2d_fast_launch<Ti, double, false, true, P, 0>(a, s);
+    }
+    return a.has_zero ? f2d_fast_launch<Ti, float, true, false, P, 0>(a, s)
+                      : f2d_fast_launch<Ti, float, false, false, P, 0>(a, s);
 }
 
 template <class Ti>
