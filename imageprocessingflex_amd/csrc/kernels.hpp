@@ -96,6 +96,10 @@ hipError_t launch_filter2d(const F2dLaunch& a, hipStream_t s);
 hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const float* k, int kw, int kh, int ay,
                              int border, double delta, double bval, hipStream_t s, const float* kx = nullptr,
                              const float* ky = nullptr);
+// medianBlur 3 x 3 of uint8 images (1, 3 or 4 channels, 16-byte aligned
+// rows, border not Constant) in the stencil layout; hipErrorNotSupported
+// otherwise.
+hipError_t launch_median3_u8(const DevImage& src, const DevImage& dst, int border, hipStream_t s);
 // unsharpMask on uint8 images (ks <= 13, 1/3/4 channels, 16-byte aligned
 // rows) in one launch; k_host / k_dev hold the ks FP64 Gaussian weights and
 // bnd bounds |FP32 blur - reference blur|. hipErrorNotSupported otherwise.

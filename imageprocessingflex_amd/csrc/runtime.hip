@@ -1068,6 +1068,15 @@ void median_impl(const ipfg_image* src, const ipfg_image* dst, int ksize, int bo
         io.finish();
         return;
     }
+    if (ksize == 3) {  // uint8, aligned rows: the register-window kernel
+        const hipError_t e = launch_median3_u8(io.src(), io.dst(), border, s);
+        if (e != hipErrorNotSupported) {
+            HIPCALL(e);
+            io.finish();
+            return;
+        }
+        (void)hipGetLastError();
+    }
     MedianLaunch a{};
     a.src = io.src();
     a.dst = io.dst();

@@ -1,5 +1,5 @@
 // Small dense stencils (3x3, 5x5, centred anchor) on interleaved uint8
-// images whose sums are exact in FP32.
+// images whose sums are exact in FP32, and the 3x3 median of uint8 images.
 //
 // Used for linear filters on uint8 images when every weight is a dyadic
 // rational m * 2^-f and 255 * sum|m| (+ |delta| in units of 2^-f) stays below
@@ -51,9 +51,9 @@ __device__ __forceinline__ float win_byte(const unsigned (&w)[N], int q) {
     return static_cast<float>((w[q >> 2] >> (8 * (q & 3))) & 0xffu);
 }
 
-template <class To, int KW, int KH, int CN>
-__device__ __forceinline__ void frame_sample(const StencilArgs& a, long long idx) {
-    int y, s;
+// Row y and flat sample s of frame sample idx: the whole top rows, the whole
+// bottom rows, then the first lside and last rside samples of each interior row.
+__device__ __forceinline__ void frame_pos(const StencilArgs& a, long long idx, int& y, int& s) {
     const long long full_top = static_cast<long long>(a.ntop) * a.L;
     const long long full = full_top + static_cast<long long>(a.nbot) * a.L;
     if (idx < full_top) {
@@ -72,6 +72,12 @@ __device__ __forceinline__ void frame_sample(const StencilArgs& a, long long idx
         y = a.ay + rr;
         s = t < a.lside ? t : a.L - a.rside + (t - a.lside);
     }
+}
+
+template <class To, int KW, int KH, int CN>
+__device__ __forceinline__ void frame_sample(const StencilArgs& a, long long idx) {
+    int y, s;
+    frame_pos(a, idx, y, s);
     const int px = s / CN, c = s - px * CN;
     float v = a.delta;
 #pragma unroll
@@ -176,16 +182,16 @@ __global__ __launch_bounds__(kBlock) void stencil_u8_kernel(StencilArgs a) {
     }
 }
 
-template <class To, int KW, int KH, bool SEP>
-hipError_t stencil_by_cn(StencilArgs a, int cn, hipStream_t s) {
-    const int ah = (KW / 2) * cn;
+// Interior and frame of a KW x KH window over a (fields L, H, ay set) and the
+// grid: interior groups g with 4 g >= 4 dh and 4 g + 4 + 4 dh <= L, on rows
+// whose window stays inside the image; the frame is everything else.
+hipError_t stencil_geometry(StencilArgs& a, int kw, int kh, int cn, dim3& grid) {
+    const int ah = (kw / 2) * cn;
     const int dh = (ah + 3) / 4;
-    // interior groups g: 4 g >= 4 dh and 4 g + 4 + 4 dh <= L; rows whose
-    // window stays inside the image
     a.glo = dh;
     const int ghi = (a.L - 4 - 4 * dh) >= 0 ? (a.L - 4 - 4 * dh) / 4 : -1;
     a.GI = ghi >= a.glo ? ghi - a.glo + 1 : 0;
-    a.HI = a.H >= KH ? a.H - KH + 1 : 0;
+    a.HI = a.H >= kh ? a.H - kh + 1 : 0;
     if (a.GI == 0 || a.HI == 0) {
         a.GI = a.HI = 0;
         a.ntop = a.H;
@@ -204,7 +210,15 @@ hipError_t stencil_by_cn(StencilArgs a, int cn, hipStream_t s) {
     if (nint > (1LL << 31) - 1 || nb + nbf + 8 > (1LL << 31) - 1) return hipErrorNotSupported;
     a.nblocks = static_cast<int>(nb);
     a.nbi = static_cast<int>((nb + 7) / 8 * 8);
-    const dim3 grid(static_cast<unsigned>(a.nbi + nbf));
+    grid = dim3(static_cast<unsigned>(a.nbi + nbf));
+    return hipSuccess;
+}
+
+template <class To, int KW, int KH, bool SEP>
+hipError_t stencil_by_cn(StencilArgs a, int cn, hipStream_t s) {
+    dim3 grid;
+    const hipError_t e = stencil_geometry(a, KW, KH, cn, grid);
+    if (e != hipSuccess) return e;
     switch (cn) {
         case 1: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 1, SEP>), grid, dim3(kBlock), 0, s, a); break;
         case 3: hipLaunchKernelGGL((stencil_u8_kernel<To, KW, KH, 3, SEP>), grid, dim3(kBlock), 0, s, a); break;
@@ -214,11 +228,139 @@ hipError_t stencil_by_cn(StencilArgs a, int cn, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ 3 x 3 median
+// medianBlur(k = 3) on uint8 images (filter.hpp:302-332) in the stencil's
+// layout. The median of 9 is med3(max of the three row minima, med3 of the row
+// medians, min of the three row maxima) over the window's rows, each row's
+// triple sorted once and shared by the three output rows it reaches.
+__device__ __forceinline__ void sort3u(unsigned& x, unsigned& y, unsigned& z) {
+    const unsigned a = min(x, y), b = max(x, y);
+    const unsigned c = min(b, z);
+    z = max(b, z);
+    x = min(a, c);
+    y = max(a, c);
+}
+__device__ __forceinline__ unsigned med3u(unsigned x, unsigned y, unsigned z) {
+    return max(min(x, y), min(max(x, y), z));
+}
+template <int N>
+__device__ __forceinline__ unsigned win_u8(const unsigned (&w)[N], int q) {
+    return (w[q >> 2] >> (8 * (q & 3))) & 0xffu;
+}
+
+template <int CN>
+__device__ __forceinline__ void median3_frame_sample(const StencilArgs& a, long long idx) {
+    int y, s;
+    frame_pos(a, idx, y, s);
+    const int px = s / CN, c = s - px * CN;
+    unsigned lo[3], md[3], hi[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int sy = border_index(y - 1 + j, a.H, a.border);  // never Constant: sy >= 0
+        const uint8_t* srow = a.src + static_cast<size_t>(sy) * a.sstride;
+        unsigned v[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] = srow[static_cast<size_t>(border_index(px + i - 1, a.W, a.border)) * CN + c];
+        sort3u(v[0], v[1], v[2]);
+        lo[j] = v[0];
+        md[j] = v[1];
+        hi[j] = v[2];
+    }
+    const unsigned m = med3u(max(max(lo[0], lo[1]), lo[2]), med3u(md[0], md[1], md[2]), min(min(hi[0], hi[1]), hi[2]));
+    reinterpret_cast<uint8_t*>(a.dst + static_cast<size_t>(y) * a.dstride)[s] = static_cast<uint8_t>(m);
+}
+
+template <int CN>
+__global__ __launch_bounds__(kBlock) void median3_u8_kernel(StencilArgs a) {
+    constexpr int DH = (CN + 3) / 4;  // halo dwords on each side
+    constexpr int ND = 1 + 2 * DH;
+    constexpr int NG = kStencilRows;
+    constexpr int NR = NG + 2;
+    const int b = blockIdx.x;
+    if (b >= a.nbi) {
+        const long long idx = static_cast<long long>(b - a.nbi) * kBlock + threadIdx.x;
+        if (idx < a.nframe) median3_frame_sample<CN>(a, idx);
+        return;
+    }
+    const int blk = (b & 7) * (a.nbi >> 3) + (b >> 3);
+    if (blk >= a.nblocks) return;
+    const unsigned nrb = (static_cast<unsigned>(a.HI) + NG - 1) / NG;
+    const unsigned idx = static_cast<unsigned>(blk) * kBlock + threadIdx.x;
+    if (idx >= static_cast<unsigned>(a.GI) * nrb) return;
+    const unsigned rb = idx / static_cast<unsigned>(a.GI);
+    const int s0 = (static_cast<int>(idx - rb * static_cast<unsigned>(a.GI)) + a.glo) * 4;
+    const int r0 = static_cast<int>(rb) * NG;
+    unsigned w[NR][ND];
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+        const int rr = min(r0 + t, a.H - 1);  // rows past the image feed only rows that are not stored
+        const unsigned* p = reinterpret_cast<const unsigned*>(a.src + static_cast<size_t>(rr) * a.sstride + s0) - DH;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) w[t][d] = p[d];
+    }
+    // sorted horizontal triples of the last three input rows (a ring)
+    unsigned lo[3][4], md[3][4], hi[3][4];
+    auto sort_row = [&](int t, int slot) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            unsigned x = win_u8(w[t], 4 * DH + m - CN), y = win_u8(w[t], 4 * DH + m), z = win_u8(w[t], 4 * DH + m + CN);
+            sort3u(x, y, z);
+            lo[slot][m] = x;
+            md[slot][m] = y;
+            hi[slot][m] = z;
+        }
+    };
+    sort_row(0, 0);
+    sort_row(1, 1);
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        sort_row(k + 2, (k + 2) % 3);
+        if (r0 + k >= a.HI) break;
+        unsigned out = 0u;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const unsigned p = max(max(lo[0][m], lo[1][m]), lo[2][m]);
+            const unsigned q = med3u(md[0][m], md[1][m], md[2][m]);
+            const unsigned r = min(min(hi[0][m], hi[1][m]), hi[2][m]);
+            out |= med3u(p, q, r) << (8 * m);
+        }
+        *reinterpret_cast<unsigned*>(a.dst + static_cast<size_t>(a.ay + r0 + k) * a.dstride + s0) = out;
+    }
+}
+
 bool aligned16(const void* p, size_t stride) {
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && (stride & 15u) == 0;
 }
 
 }  // namespace
+
+hipError_t launch_median3_u8(const DevImage& src, const DevImage& dst, int border, hipStream_t s) {
+    const int cn = src.cn;
+    if (src.dt != IPFG_U8 || dst.dt != IPFG_U8 || border == IPFG_BORDER_CONSTANT) return hipErrorNotSupported;
+    if (cn != 1 && cn != 3 && cn != 4) return hipErrorNotSupported;
+    if (!aligned16(src.data, src.stride) || !aligned16(dst.data, dst.stride)) return hipErrorNotSupported;
+    const long long L = static_cast<long long>(src.w) * cn;
+    if (L <= 0 || src.h <= 0 || L > (1LL << 30)) return hipErrorNotSupported;
+    StencilArgs a{};
+    a.src = static_cast<const uint8_t*>(src.data);
+    a.dst = static_cast<char*>(dst.data);
+    a.sstride = src.stride;
+    a.dstride = dst.stride;
+    a.W = src.w;
+    a.H = src.h;
+    a.L = static_cast<int>(L);
+    a.ay = 1;
+    a.border = border;
+    dim3 grid;
+    const hipError_t e = stencil_geometry(a, 3, 3, cn, grid);
+    if (e != hipSuccess) return e;
+    switch (cn) {
+        case 1: hipLaunchKernelGGL(median3_u8_kernel<1>, grid, dim3(kBlock), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(median3_u8_kernel<3>, grid, dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL(median3_u8_kernel<4>, grid, dim3(kBlock), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_stencil_u8(const DevImage& src, const DevImage& dst, const float* k, int kw, int kh, int ay,
                              int border, double delta, double bval, hipStream_t s, const float* kx, const float* ky) {
