@@ -18,7 +18,7 @@
 //             thresholds canny_nms_kernel reads
 //  corners    tensor_rows_kernel   products of the derivatives and their
 //                                  horizontal running sums (the reference's
-//                                  boxFilter order), one thread per row
+//                                  boxFilter order), one wave per row
 //             tensor_cols_kernel   vertical running sums and the Harris or
 //                                  minimum-eigenvalue response, one thread per
 //                                  column

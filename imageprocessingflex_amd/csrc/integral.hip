@@ -7,9 +7,10 @@
 // order; integer images give exact integer sums in any order as long as they
 // stay below 2^53.
 //
-//  integral_rows_kernel  one thread per (row, channel): the row's running
-//                        sums, stored in the table's own row y + 1 (the
-//                        column pass adds the row above in place)
+//  integral_rows_kernel  one wave per (row, channel): the row's running
+//                        sums (the chain carried through the lanes), stored
+//                        in the table's own row y + 1 (the column pass adds
+//                        the row above in place)
 //  integral_cols_kernel  one thread per (column, channel): walks down the
 //                        column adding each entry to the one below, in the
 //                        reference's row order; consecutive threads take
