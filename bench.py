@@ -150,7 +150,7 @@ def _c1():
     calls = [Call("gauss_s2/reflect101", "gauss_s2", "img", np.uint8,
                   lambda lib, s, d, st: lib.ipfg_gaussian_blur(s, d, 2.0, 0.0, 0, 3, 0, st),
                   lambda o, im: o.gaussian_blur(im, 2.0, border=3),
-                  "sep_plane_kernel<unsigned char, unsigned char, double, 3, 2, true>")]
+                  "tile_u8_kernel<3, 13, TY> (one-pass tile kernel; the autotuner picks TY or a strip kernel)")]
     return {"images": {"img": img}, "calls": calls, "dtype": "f64",
             "workload": "gaussianBlur sigma 2 (13 taps), Reflect101, 1920x1080x3 uint8"}
 

@@ -138,6 +138,10 @@ std::vector<int> sep_sh_candidates(const SepLaunch& l) {
     }
     for (int v : {16, 32, 48, 64, 96, 128, 192, 256, 384, 512})
         if (v < 2 * h) c.push_back(v);
+    if (l.mode == MODE_CERTIFY && tile_u8_fits(l.src, l.dst, l.kw, l.kh, l.ax, l.ay, l.unit != 0)) {
+        c.push_back(kTileSH + 16);  // the one-pass tile kernel, 16- and 32-row tiles
+        c.push_back(kTileSH + 32);
+    }
     return c;
 }
 
@@ -180,6 +184,12 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
     a.oscale = l.oscale != 0.0 ? l.oscale : 1.0;
     a.unit = l.unit;
     a.kabs = l.kabs;
+    int tile_rows = a.SH >= kTileSH ? (a.SH == kTileSH ? 32 : a.SH - kTileSH) : 0;
+    if (tile_rows) a.SH = 0;  // the strip kernels' own height when the tile kernel does not take the call
+    // Without a measured shape the tile kernel takes the calls it fits: it was
+    // faster than the strip kernels on most measured uint8 shapes, with
+    // 16-row tiles up to a megapixel and 32-row tiles above.
+    if (l.sh <= 0) tile_rows = static_cast<long long>(l.src.w) * l.src.h <= (1LL << 20) ? 16 : 32;
     const int cn = l.src.cn;
     const bool planar = sep_planar(cn, l.src.dt, l.dst.dt);
     if (planar && l.src.dt == IPFG_U8) {
@@ -191,6 +201,11 @@ hipError_t launch_sep(const SepLaunch& l, hipStream_t s) {
         if (cert && plane_ring_is_float(l.dst.dt)) {
             // the ring holds the FP64 horizontal sums rounded to float
             a.bcoef += 1.01 * 5.9604644775390625e-08 * l.kabs;
+            if (l.dst.dt == IPFG_U8 && tile_rows) {
+                const hipError_t e = tile_u8(a, tile_rows, s);
+                if (e != hipErrorNotSupported) return e;
+                (void)hipGetLastError();
+            }
             if (l.dst.dt == IPFG_U8) {
                 const hipError_t e = cn == 1 ? plane_u8_k1(a, s) : cn == 3 ? plane_u8_k3(a, s) : plane_u8_k4(a, s);
                 if (e != hipErrorNotSupported) return e;

@@ -870,5 +870,13 @@ hipError_t plane_u8f_k1(const SepArgs& a, hipStream_t s);  // certified uint8 ->
 hipError_t plane_f32_k1(const SepArgs& a, hipStream_t s);
 hipError_t generic_any(const SepArgs& a, bool cert, hipStream_t s);
 
+// One-pass tile kernel (sep_tile.hip): certified uint8 -> uint8, 1 / 3 / 4
+// channels, kw = kh odd in [3, 15], centred anchor. Launches that carry
+// strip height kTileSH + rows run it with tiles of `rows` (16 or 32; 0 means
+// 32) output rows.
+constexpr int kTileSH = 1 << 22;
+bool tile_u8_fits(const DevImage& src, const DevImage& dst, int kw, int kh, int ax, int ay, bool unit);
+hipError_t tile_u8(const SepArgs& a, int rows, hipStream_t s);
+
 }  // namespace sepk
 }  // namespace ipfg

@@ -619,6 +619,65 @@ def test_every_strip_height_gives_the_reference_result(ipf, oracle):
         ipf.tune_clear()
 
 
+TILE_SH = 1 << 22  # strip heights TILE_SH + rows select the one-pass tile kernel (uint8 -> uint8)
+
+
+@pytest.mark.parametrize("cn", [1, 3, 4])
+def test_tile_kernel_gives_the_reference_result(ipf, oracle, cn):
+    # every kernel size the tile kernel takes, every border, signed weights
+    # (outputs saturate at both ends), a delta, a fractional Constant value,
+    # and images smaller than one tile, ragged at both edges, or one row tall
+    rng = np.random.default_rng(60 + cn)
+    shapes = [(1, 1), (1, 40), (5, 3), (37, 70), (67, 131), (150, 203)]
+    try:
+        for k in range(3, 17, 2):
+            kx = rng.normal(0.0, 1.0, k) / k
+            ky = rng.normal(0.0, 1.0, k) + 1.0 / k
+            for i, (h, w) in enumerate(shapes):
+                img = rng.integers(0, 256, (h, w, cn), dtype=np.uint8)
+                wl, hl = int(np.ceil(np.log2(w))), int(np.ceil(np.log2(h)))
+                for b in range(5):
+                    delta = 3.25 if (k + i + b) % 2 else 0.0
+                    want = oracle.sep_filter(img, kx, ky, None, b, delta, 17.5)
+                    for rows in (16, 32):
+                        ipf.tune_clear()
+                        ipf.tune_import(f"sep 0 0 {cn} {k} {k} 1 {wl} {hl} {TILE_SH + rows}\n")
+                        got = ipf.sep_filter2d(img, kx, ky, b, delta, 17.5)
+                        assert np.array_equal(got, want), (k, h, w, b, rows)
+    finally:
+        ipf.tune_clear()
+
+
+def test_tile_kernel_gaussian_and_recompute_count(ipf, oracle):
+    # gaussian weights on an interior-heavy image, with the recompute count:
+    # the float-rounded sums leave a few doubtful samples per million
+    rng = np.random.default_rng(70)
+    img = rng.integers(0, 256, (333, 517, 3), dtype=np.uint8)
+    try:
+        for sigma, k in ((0.8, 5), (2.0, 13), (2.4, 15)):
+            want = oracle.gaussian_blur(img, sigma)
+            for rows in (16, 32):
+                ipf.tune_clear()
+                ipf.tune_import(f"sep 0 0 3 {k} {k} 1 10 9 {TILE_SH + rows}\n")
+                got = ipf.gaussian_blur(img, sigma)
+                assert np.array_equal(got, want), (sigma, rows)
+                assert ipf.last_fixups() < img.size // 1000, (sigma, rows, ipf.last_fixups())
+    finally:
+        ipf.tune_clear()
+
+
+def test_tile_kernel_full_c1_hash(ipf):
+    from imageprocessingflex_amd import workloads as wl
+    try:
+        img = wl.c1_image()
+        for rows in (16, 32):
+            ipf.tune_clear()
+            ipf.tune_import(f"sep 0 0 3 13 13 1 11 11 {TILE_SH + rows}\n")
+            assert _sha(ipf.gaussian_blur(img, 2.0)) == FULL["c1_gauss2_reflect101_u8"], rows
+    finally:
+        ipf.tune_clear()
+
+
 def test_autotuned_call_matches_untuned(ipf):
     rng = np.random.default_rng(22)
     img = rng.integers(0, 256, (1100, 1000, 1), dtype=np.uint8)
