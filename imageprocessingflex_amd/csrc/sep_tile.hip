@@ -176,8 +176,11 @@ __global__ __launch_bounds__(kBlock) void tile_u8_kernel(SepArgs a, const double
         }
         __syncthreads();
 
-        // ---- horizontal: row r, pixels [Q g, Q g + Q), every channel
-        for (int t = tid; t < T::NH; t += kBlock) {
+        // ---- horizontal: row r, pixels [Q g, Q g + Q), every channel. The
+        // tasks go to the threads in reverse order: the waves left with a
+        // partial last round here take the extra round of the vertical pass,
+        // so every SIMD carries the same share of a tile.
+        for (int t = kBlock - 1 - tid; t < T::NH; t += kBlock) {
             const int r = t / T::G, g = t - r * T::G;
             const float* sp = stage + r * T::SP + T::OFF + g * kQ * CN;
             float f[4 * T::NSPAN4];
