@@ -768,8 +768,10 @@ def measure(wd, ipf, steps: int, warmup: int, cpu: bool, cpu_runs: int = 5):
 
 # The other configurations reported beside the headline (default run): the
 # same measurement with fewer steps; c5 on a prefix of the serving stream.
+# A c1 step is one ~40 us call: 100 steps keep the final synchronisation and
+# the first launch from weighing on its value.
 EXTRA_CONFIGS = ("c1", "c3", "c4", "c5")
-EXTRA_STEPS = 5
+EXTRA_STEPS = {"c1": 100, "c3": 5, "c4": 5}
 EXTRA_C5_REQUESTS = 3000
 
 
@@ -819,7 +821,7 @@ def main():
                 configs[name] = _brief(run_c5(sub, ipf))
             else:
                 w = WORKLOADS[name]()
-                configs[name] = _brief(measure(w, ipf, EXTRA_STEPS, 1, not args.no_cpu, 3 if name == "c4" else 5))
+                configs[name] = _brief(measure(w, ipf, EXTRA_STEPS[name], 1, not args.no_cpu, 3 if name == "c4" else 5))
                 del w
             print(f"# {name} measured in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
         line["configs"] = configs
